@@ -1,0 +1,239 @@
+/*
+ * yrtmi - MI355X (gfx950) OSEM projection library: C ABI.
+ *
+ * Plain C types only: device pointers are passed as plain pointers, sizes as
+ * int64_t, streams as void* (a hipStream_t; NULL = the null stream).  Every
+ * call returns 0 on success or a negative YRTMI_E* code; yrtmi_last_error()
+ * returns a thread-local message describing the last failure.
+ *
+ * Ownership: the caller owns every buffer passed in (host or device).  A
+ * context owns only what it uploads itself (detector LUT, histogram tables,
+ * motion transforms, PSF tables) and frees it in yrtmi_ctx_destroy.  Device
+ * image / projection buffers handed in are aliases (e.g. torch tensors) and
+ * are never freed by the library.
+ *
+ * Calls are asynchronous on the context's stream unless stated otherwise;
+ * yrtmi_ctx_synchronize() waits.  A context must not be used by two host
+ * threads at once (the reference's projector objects have the same rule:
+ * OperatorProjector.cpp:358-364).
+ *
+ * Each entry point names the reference (YRT-PET 2.0.7) interface it replaces.
+ */
+#ifndef YRTMI_H
+#define YRTMI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define YRTMI_OK 0
+#define YRTMI_EINVAL -1   /* bad argument / shape */
+#define YRTMI_EHIP -2     /* HIP runtime failure */
+#define YRTMI_ENOMEM -3   /* allocation failure */
+#define YRTMI_ESTATE -4   /* call out of order (e.g. no LUT uploaded) */
+
+#define YRTMI_PROJ_SIDDON 0
+#define YRTMI_PROJ_DD 1
+#define YRTMI_MAX_RAYS 16
+
+/* Image geometry (ImageParams, include/yrt-pet/datastruct/image/
+ * ImageParams.hpp).  Voxel data are float[nt][nz][ny][nx]. */
+typedef struct
+{
+	int64_t nx, ny, nz, nt;
+	float vx, vy, vz;          /* voxel size, mm */
+	float lx, ly, lz;          /* image length, mm */
+	float ox, oy, oz;          /* offset of the image centre, mm */
+	float fov_radius;          /* ImageParams::setup */
+} yrtmi_image;
+
+/* Projector (ProjectorParams, include/yrt-pet/operators/
+ * ProjectorParams.hpp:27-71, plus the projection-space PSF of
+ * ProjectionPsfManager.hpp:13-35 as rows of taps). */
+typedef struct
+{
+	int type;                  /* YRTMI_PROJ_SIDDON / YRTMI_PROJ_DD */
+	int num_rays;              /* multi-ray Siddon, 1..YRTMI_MAX_RAYS */
+	int use_tof;
+	float tof_width_ps;        /* FWHM */
+	int tof_n_std;             /* <= 0: untruncated Gaussian */
+	const float* psf_kernels;  /* host, psf_num_s x psf_ksize, or NULL */
+	int psf_num_s, psf_ksize;
+	float psf_s_step, psf_k_spacing;
+	float crystal_size_z, crystal_size_trans, crystal_depth;
+	/* 0 (default): ray-plane setup in FP32 with the reference's operation
+	 * order (reference parity, including its FP32 error on rays grazing voxel
+	 * planes); 1: setup in FP64 (accuracy mode). */
+	int setup_fp64;
+} yrtmi_projector_desc;
+
+/* Histogram3D geometry (Histogram3D ctor, src/datastruct/projection/
+ * Histogram3D.cpp:165-203) derived from a regular scanner. */
+typedef struct
+{
+	int64_t dets_per_ring, num_rings, num_doi, max_ring_diff, min_ang_diff;
+} yrtmi_histo_desc;
+
+/* Where the LORs of a call come from.
+ *   YRTMI_SRC_PAIRS        : d1[i], d2[i]                 (list-mode)
+ *   YRTMI_SRC_BINS         : Histogram3D bin ids bins[i]  (e.g. nonzero bins)
+ *   YRTMI_SRC_HISTO_SUBSET : bin = BinIteratorRangeHistogram3D(S, s).get(i)
+ *                            (every bin of a subset; n must be that size)
+ * All pointers are DEVICE pointers; tof/frame/motion_frame are nullable.
+ * frame: dynamic frame (< 0 skips the event, ProjectorSiddon.cpp:191-196).
+ * motion_frame: index into the context's motion transforms. */
+#define YRTMI_SRC_PAIRS 0
+#define YRTMI_SRC_BINS 1
+#define YRTMI_SRC_HISTO_SUBSET 2
+typedef struct
+{
+	int kind;
+	int64_t n;
+	const uint32_t* d1;
+	const uint32_t* d2;
+	const uint32_t* bins;
+	int num_subsets, subset;
+	const float* tof;
+	const int32_t* frame;
+	const uint8_t* motion_frame;
+} yrtmi_lors;
+
+/* Per-LOR correction factors for the fused EM kernel (OSEM_CPU.cpp:329-374,
+ * OSEM_GPU.cu:1159-1341).  DEVICE pointers, all nullable:
+ * meas NULL = list-mode measurement 1.0 (ListMode.cpp:52-55). */
+typedef struct
+{
+	const float* meas;
+	const float* sens;
+	const float* att;
+	const float* randoms;
+	const float* scatter;
+	float global_scale;
+	float denom_threshold;
+} yrtmi_corrections;
+
+/* Regular cylindrical scanner (Scanner ctor, src/datastruct/scanner/
+ * Scanner.cpp:101-127). */
+typedef struct
+{
+	float axial_fov, crystal_size_z, crystal_size_trans, crystal_depth, radius;
+	int64_t dets_per_ring, num_rings, num_doi, dets_per_block;
+} yrtmi_scanner_desc;
+
+typedef struct yrtmi_ctx yrtmi_ctx;
+typedef struct yrtmi_proj yrtmi_proj;
+
+/* ---- context ------------------------------------------------------------
+ * Replaces the device selection / stream setup of OperatorProjectorDevice::
+ * create (src/operators/OperatorProjectorDevice.cu:300-319) and GPUStream. */
+const char* yrtmi_last_error(void);
+int yrtmi_ctx_create(int device, void* stream, yrtmi_ctx** out);
+int yrtmi_ctx_destroy(yrtmi_ctx* ctx);
+int yrtmi_ctx_set_stream(yrtmi_ctx* ctx, void* stream);
+int yrtmi_ctx_synchronize(yrtmi_ctx* ctx);
+
+/* Detector LUT (host arrays n_det x 3 each), kept resident on the device as
+ * SoA.  Replaces the per-batch host LOR precompute (LORsDevice.cu:28-55,
+ * OSEM_GPU.cu:698-805) and Scanner::createLUT (Scanner.cpp:199-213). */
+int yrtmi_lut_upload(yrtmi_ctx* ctx, const float* pos, const float* orient,
+                     int64_t n_det);
+
+/* Regular detector LUT on the host: pos / orient are host arrays of
+ * dets_per_ring*num_rings*num_doi x 3 floats.  Replaces
+ * DetRegular::generateLUT (src/datastruct/scanner/DetRegular.cpp:113-182). */
+int yrtmi_lut_regular(const yrtmi_scanner_desc* s, float* pos, float* orient);
+
+/* Histogram3D bin <-> detector-pair tables on the device (closed form; the
+ * while loop of Histogram3D::getDetPairFromCoords, Histogram3D.cpp:339-357,
+ * becomes a z-bin table).  Needed by the BINS / HISTO_SUBSET sources. */
+int yrtmi_histo_setup(yrtmi_ctx* ctx, const yrtmi_histo_desc* h,
+                      int64_t* num_z_bin, int64_t* num_phi, int64_t* num_r);
+/* Size of subset s of S (BinIteratorRangeHistogram3D::size,
+ * BinIterator.cpp:186-189). */
+int64_t yrtmi_histo_subset_size(yrtmi_ctx* ctx, int num_subsets, int subset);
+
+/* Rigid LOR motion, k host transforms of 12 floats (transform_t,
+ * include/yrt-pet/utils/Types.hpp:57-60), applied to LOR endpoints as
+ * ProjectionData::getLOR does (ProjectionData.cpp:302-318). */
+int yrtmi_motion_upload(yrtmi_ctx* ctx, const float* xf, int k);
+
+/* Projector: uploads PSF tables and fixes the multi-ray offsets (the
+ * rand_r(seed=13) sequence of MultiRayGenerator::getRandomLine,
+ * MultiRayGenerator.cpp:72-114).  Replaces ProjectorParams +
+ * OperatorProjectorSiddon_GPU / OperatorProjectorDD_GPU construction. */
+int yrtmi_proj_create(yrtmi_ctx* ctx, const yrtmi_projector_desc* d,
+                      yrtmi_proj** out);
+int yrtmi_proj_destroy(yrtmi_proj* p);
+/* Launch shape: lanes cooperating on one event (1, 2, 4 or 8; they split the
+ * rays), threads per block, and blocks (0 = sized from the event count). */
+int yrtmi_proj_set_launch(yrtmi_proj* p, int lanes_per_event,
+                          int block_size, int grid_blocks);
+
+/* ---- projections --------------------------------------------------------
+ * Forward: out[i] = (A x)_i for each LOR of `lors` (device out, n floats).
+ * Replaces OperatorProjectorDevice::applyA (OperatorProjectorDevice.cu:
+ * 442-544) -> projectSiddon_kernel / projectDD_kernel<true,...>. */
+int yrtmi_forward(yrtmi_ctx* ctx, const yrtmi_proj* p, const yrtmi_image* im,
+                  const float* img, const yrtmi_lors* lors, float* out);
+/* Back: img += A^T vals (device img is accumulated into, not cleared).
+ * Replaces OperatorProjectorDevice::applyAH (OperatorProjectorDevice.cu:
+ * 546-675) -> project*_kernel<false,...> with atomicAdd. */
+int yrtmi_back(yrtmi_ctx* ctx, const yrtmi_proj* p, const yrtmi_image* im,
+               float* img, const yrtmi_lors* lors, const float* vals);
+
+/* Fused EM subset: u += A^T( m / (A x .* corr + add) .* corr ).
+ * Replaces OSEM_GPU::computeEMUpdateImage (OSEM_GPU.cu:455-510) and
+ * computeEMUpdateImage_kernel (OSEM_GPU.cu:1159-1341). */
+int yrtmi_em_subset(yrtmi_ctx* ctx, const yrtmi_proj* p, const yrtmi_image* im,
+                    const float* x, float* u, const yrtmi_lors* lors,
+                    const yrtmi_corrections* corr);
+
+/* Sensitivity: img += A^T(acf .* sens) over `lors` (typically a whole
+ * histogram subset), the projector used WITHOUT TOF.  att_img (nullable,
+ * device, geometry att_im) gives acf = exp(-0.1 * Siddon(mu)) per LOR
+ * (Corrector.cpp:601-611, Tools.hpp:98-102); sens (nullable, device, per
+ * LOR) the normalisation factor.  Replaces OSEM_GPU::
+ * generateSensitivityImageForCurrentSubset (OSEM_GPU.cu:197-253) and
+ * generateSensImage_kernel (OSEM_GPU.cu:1048-1157). */
+int yrtmi_sensitivity(yrtmi_ctx* ctx, const yrtmi_proj* p,
+                      const yrtmi_image* im, float* img,
+                      const yrtmi_lors* lors, const yrtmi_image* att_im,
+                      const float* att_img, const float* sens);
+
+/* Per-LOR attenuation factors exp(-0.1 * Siddon(mu)) (single ray, no TOF,
+ * image offset not subtracted, as singleForwardProjection does).  Replaces
+ * Corrector_GPU::precomputeAttenuationFactors (Corrector_GPU.cu:19-69). */
+int yrtmi_attenuation_factors(yrtmi_ctx* ctx, const yrtmi_image* att_im,
+                              const float* att_img, const yrtmi_lors* lors,
+                              float* acf);
+
+/* ---- image-space kernels (ImageSpaceKernels.cu:13-190) ------------------ */
+/* x[t][j] *= u[t][j] / s[j] where s[j] > thr; s has nvox (3D, broadcast
+ * over frames) or nt*nvox entries (s_frames = 1 or nt).  Replaces
+ * updateEM_kernel / updateEMDynamic_kernel and Image::updateEMThreshold*
+ * (Image.cpp:664-833). */
+int yrtmi_em_update(yrtmi_ctx* ctx, float* x, const float* u, const float* s,
+                    int64_t nvox, int64_t nt, int64_t s_frames, float thr);
+/* x = mask <= thr ? x*le_scale + le_off : x*gt_scale + gt_off, mask
+ * broadcast over frames when mask_frames == 1 (applyThreshold(Broadcast)
+ * _kernel, Image.cpp:595-649). */
+int yrtmi_threshold(yrtmi_ctx* ctx, float* x, const float* mask, int64_t nvox,
+                    int64_t nt, int64_t mask_frames, float thr, float le_scale,
+                    float le_off, float gt_scale, float gt_off);
+int yrtmi_fill(yrtmi_ctx* ctx, float* x, int64_t n, float value);
+
+/* ---- instrumentation ---------------------------------------------------- */
+/* Total voxel segments (Siddon, summed over rays) or footprint voxels (DD)
+ * touched by `lors`: the algorithmic work unit of the roofline figures.
+ * *total is a HOST pointer; the call synchronises. */
+int yrtmi_count_segments(yrtmi_ctx* ctx, const yrtmi_proj* p,
+                         const yrtmi_image* im, const yrtmi_lors* lors,
+                         int64_t* total);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* YRTMI_H */

@@ -1,0 +1,748 @@
+// yrtmi C ABI: contexts, resident scanner tables, projectors and the entry
+// points of include/yrtmi.h.  Host code only; kernels live in
+// yrtmi_kernels.hip.
+#include "yrtmi.h"
+#include "yrtmi_kernels.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace yrtmi;
+
+namespace
+{
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+	g_err = msg;
+	return code;
+}
+
+int hipFail(hipError_t e, const char* what)
+{
+	return fail(YRTMI_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr)                                   \
+	do                                                  \
+	{                                                   \
+		hipError_t e_ = (expr);                         \
+		if (e_ != hipSuccess)                           \
+			return hipFail(e_, #expr);                  \
+	} while (0)
+
+}  // namespace
+
+struct yrtmi_ctx
+{
+	int device = 0;
+	hipStream_t stream = nullptr;
+	float* lut = nullptr;
+	int64_t ndet = 0;
+	bool histoReady = false;
+	yrtmi_histo_desc histo{};
+	uint32_t numR = 0, numPhi = 0, numZBin = 0, numZBinDiff = 0, rCut = 0;
+	uint32_t* zbin = nullptr;
+	float* motion = nullptr;
+	int nmotion = 0;
+	unsigned long long* counter = nullptr;
+};
+
+struct yrtmi_proj
+{
+	// `ctx` is only compared, never dereferenced after creation: a projector
+	// may outlive its context (e.g. finalizers running in any order).
+	yrtmi_ctx* ctx = nullptr;
+	int device = 0;
+	DProj d{};
+	float* psf = nullptr;  // numS*K taps followed by the flipped copy
+	int lpe = 1;
+	int block = 256;
+	int64_t grid = 0;
+};
+
+namespace
+{
+
+int setDev(yrtmi_ctx* c)
+{
+	if (c == nullptr)
+		return fail(YRTMI_EINVAL, "null context");
+	HIP_TRY(hipSetDevice(c->device));
+	return YRTMI_OK;
+}
+
+int checkImage(const yrtmi_image* im, const char* what)
+{
+	if (im == nullptr)
+		return fail(YRTMI_EINVAL, std::string(what) + ": null image descriptor");
+	if (im->nx <= 0 || im->ny <= 0 || im->nz <= 0 || im->nt <= 0)
+		return fail(YRTMI_EINVAL, std::string(what) + ": image dimensions must be positive");
+	if (im->nx * im->ny * im->nz * im->nt >= (int64_t(1) << 31))
+		return fail(YRTMI_EINVAL, std::string(what) + ": image larger than 2^31 voxels");
+	if (!(im->vx > 0.f && im->vy > 0.f && im->vz > 0.f))
+		return fail(YRTMI_EINVAL, std::string(what) + ": voxel sizes must be positive");
+	return YRTMI_OK;
+}
+
+DImage toDImage(const yrtmi_image* im)
+{
+	DImage d{};
+	d.nx = (int32_t)im->nx;
+	d.ny = (int32_t)im->ny;
+	d.nz = (int32_t)im->nz;
+	d.nt = (int32_t)im->nt;
+	d.vx = im->vx;
+	d.vy = im->vy;
+	d.vz = im->vz;
+	d.lx = im->lx;
+	d.ly = im->ly;
+	d.lz = im->lz;
+	d.ox = im->ox;
+	d.oy = im->oy;
+	d.oz = im->oz;
+	d.fovR = im->fov_radius;
+	d.nxy = d.nx * d.ny;
+	d.nvox = (int64_t)d.nxy * d.nz;
+	return d;
+}
+
+int buildLors(yrtmi_ctx* c, const yrtmi_lors* l, DLors& d, DScanner& s)
+{
+	if (l == nullptr)
+		return fail(YRTMI_EINVAL, "null LOR source");
+	if (c->lut == nullptr)
+		return fail(YRTMI_ESTATE, "no detector LUT uploaded (yrtmi_lut_upload)");
+	if (l->n < 0)
+		return fail(YRTMI_EINVAL, "negative LOR count");
+	d = DLors{};
+	d.kind = l->kind;
+	d.n = l->n;
+	d.tof = l->tof;
+	d.frame = l->frame;
+	d.motion = l->motion_frame;
+	if (l->motion_frame != nullptr && c->motion == nullptr)
+		return fail(YRTMI_ESTATE, "motion frames given but no transforms uploaded");
+	switch (l->kind)
+	{
+	case YRTMI_SRC_PAIRS:
+		if (l->n > 0 && (l->d1 == nullptr || l->d2 == nullptr))
+			return fail(YRTMI_EINVAL, "pair source needs d1 and d2");
+		d.d1 = l->d1;
+		d.d2 = l->d2;
+		break;
+	case YRTMI_SRC_BINS:
+		if (!c->histoReady)
+			return fail(YRTMI_ESTATE, "histogram source without yrtmi_histo_setup");
+		if (l->n > 0 && l->bins == nullptr)
+			return fail(YRTMI_EINVAL, "bin source needs bins");
+		d.bins = l->bins;
+		break;
+	case YRTMI_SRC_HISTO_SUBSET:
+	{
+		if (!c->histoReady)
+			return fail(YRTMI_ESTATE, "histogram source without yrtmi_histo_setup");
+		if (l->num_subsets <= 0 || l->subset < 0 || l->subset >= l->num_subsets)
+			return fail(YRTMI_EINVAL, "bad subset index");
+		const int64_t sz = yrtmi_histo_subset_size(c, l->num_subsets, l->subset);
+		if (l->n > sz)
+			return fail(YRTMI_EINVAL, "LOR count exceeds the subset size");
+		uint32_t nps = c->numPhi / l->num_subsets;
+		if ((uint32_t)l->subset + nps * (uint32_t)l->num_subsets < c->numPhi)
+			nps += 1;
+		d.S = (uint32_t)l->num_subsets;
+		d.s = (uint32_t)l->subset;
+		d.nps = nps;
+		break;
+	}
+	default:
+		return fail(YRTMI_EINVAL, "unknown LOR source kind");
+	}
+	s = DScanner{};
+	s.lut = c->lut;
+	s.ndet = (uint32_t)c->ndet;
+	s.numR = c->numR;
+	s.numPhi = c->numPhi;
+	s.numZBin = c->numZBin;
+	s.rCut = c->rCut;
+	s.numDOI = (uint32_t)(c->histoReady ? c->histo.num_doi : 1);
+	s.numDOIPoss = s.numDOI * s.numDOI;
+	s.dpr = (uint32_t)c->histo.dets_per_ring;
+	s.nrings = (uint32_t)c->histo.num_rings;
+	s.zbin = c->zbin;
+	s.motion = c->motion;
+	s.nmotion = c->nmotion;
+	return YRTMI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* yrtmi_last_error(void)
+{
+	return g_err.c_str();
+}
+
+int yrtmi_ctx_create(int device, void* stream, yrtmi_ctx** out)
+{
+	if (out == nullptr)
+		return fail(YRTMI_EINVAL, "null output pointer");
+	int n = 0;
+	HIP_TRY(hipGetDeviceCount(&n));
+	if (device < 0 || device >= n)
+		return fail(YRTMI_EINVAL, "device index out of range");
+	auto* c = new yrtmi_ctx();
+	c->device = device;
+	c->stream = (hipStream_t)stream;
+	hipError_t e = hipSetDevice(device);
+	if (e == hipSuccess)
+		e = hipMalloc(&c->counter, sizeof(unsigned long long));
+	if (e != hipSuccess)
+	{
+		delete c;
+		return hipFail(e, "yrtmi_ctx_create");
+	}
+	*out = c;
+	return YRTMI_OK;
+}
+
+int yrtmi_ctx_destroy(yrtmi_ctx* c)
+{
+	if (c == nullptr)
+		return YRTMI_OK;
+	(void)hipSetDevice(c->device);
+	(void)hipStreamSynchronize(c->stream);
+	(void)hipFree(c->lut);
+	(void)hipFree(c->zbin);
+	(void)hipFree(c->motion);
+	(void)hipFree(c->counter);
+	delete c;
+	return YRTMI_OK;
+}
+
+int yrtmi_ctx_set_stream(yrtmi_ctx* c, void* stream)
+{
+	if (c == nullptr)
+		return fail(YRTMI_EINVAL, "null context");
+	c->stream = (hipStream_t)stream;
+	return YRTMI_OK;
+}
+
+int yrtmi_ctx_synchronize(yrtmi_ctx* c)
+{
+	int r = setDev(c);
+	if (r)
+		return r;
+	HIP_TRY(hipStreamSynchronize(c->stream));
+	return YRTMI_OK;
+}
+
+int yrtmi_lut_upload(yrtmi_ctx* c, const float* pos, const float* orient,
+                     int64_t n)
+{
+	int r = setDev(c);
+	if (r)
+		return r;
+	if (pos == nullptr || orient == nullptr || n <= 0 || n >= (int64_t(1) << 31))
+		return fail(YRTMI_EINVAL, "bad LUT arguments");
+	std::vector<float> soa(6 * n);
+	for (int64_t i = 0; i < n; i++)
+	{
+		for (int k = 0; k < 3; k++)
+		{
+			soa[k * n + i] = pos[3 * i + k];
+			soa[(3 + k) * n + i] = orient[3 * i + k];
+		}
+	}
+	HIP_TRY(hipStreamSynchronize(c->stream));
+	HIP_TRY(hipFree(c->lut));
+	c->lut = nullptr;
+	HIP_TRY(hipMalloc(&c->lut, soa.size() * sizeof(float)));
+	HIP_TRY(hipMemcpy(c->lut, soa.data(), soa.size() * sizeof(float),
+	                  hipMemcpyHostToDevice));
+	c->ndet = n;
+	return YRTMI_OK;
+}
+
+// DetRegular::generateLUT (src/datastruct/scanner/DetRegular.cpp:113-182):
+// crystal centres at radius R + depth*(doi + 1/2), z = -FOV/2 + (ring+1/2)*gap,
+// orientation (cos, sin, 0) of the block angle.
+int yrtmi_lut_regular(const yrtmi_scanner_desc* sc, float* pos, float* orient)
+{
+	if (sc == nullptr || pos == nullptr || orient == nullptr ||
+	    sc->dets_per_ring <= 0 || sc->num_rings <= 0 || sc->num_doi <= 0 ||
+	    sc->dets_per_block <= 0 || sc->dets_per_ring % sc->dets_per_block != 0)
+		return fail(YRTMI_EINVAL, "bad scanner descriptor");
+	const int64_t numBlocks = sc->dets_per_ring / sc->dets_per_block;
+	const float blockLength = sc->crystal_size_trans * sc->dets_per_block;
+	const double edgeMax = 2 * sc->radius * std::tan(3.14159265358979 / numBlocks);
+	if (blockLength > edgeMax)
+		return fail(YRTMI_EINVAL, "overlap between crystals: scanner radius too small");
+	for (int64_t doi = 0; doi < sc->num_doi; doi++)
+	{
+		const float blockDistance = sc->radius + sc->crystal_depth * doi;
+		for (int64_t ring = 0; ring < sc->num_rings; ring++)
+		{
+			const float zGap = sc->axial_fov / ((float)sc->num_rings);
+			const float zPos = (float)(-sc->axial_fov / 2.0 + (ring + 0.5f) * zGap);
+			for (int64_t block = 0; block < numBlocks; block++)
+			{
+				const float ang = (float)((double)block * 6.28318530717959 /
+				                          (double)numBlocks);
+				const float xB = (blockDistance + 0.5f * sc->crystal_depth) * std::cos(ang);
+				const float yB = (blockDistance + 0.5f * sc->crystal_depth) * std::sin(ang);
+				for (int64_t det = 0; det < sc->dets_per_block; det++)
+				{
+					float rel = 0.0f;
+					if (sc->dets_per_block != 1)
+						rel = (float)(-0.5 + (det + 0.5) / (float)sc->dets_per_block);
+					const float rx = (float)(rel * std::cos(ang + 3.14159265358979 / 2.0));
+					const float ry = (float)(rel * std::sin(ang + 3.14159265358979 / 2.0));
+					const int64_t idx = det + block * sc->dets_per_block +
+					                    ring * sc->dets_per_ring +
+					                    doi * (sc->num_rings * sc->dets_per_ring);
+					pos[3 * idx] = xB + blockLength * rx;
+					pos[3 * idx + 1] = yB + blockLength * ry;
+					pos[3 * idx + 2] = zPos;
+					orient[3 * idx] = std::cos(ang);
+					orient[3 * idx + 1] = std::sin(ang);
+					orient[3 * idx + 2] = 0.0f;
+				}
+			}
+		}
+	}
+	return YRTMI_OK;
+}
+
+int yrtmi_histo_setup(yrtmi_ctx* c, const yrtmi_histo_desc* h,
+                      int64_t* num_z_bin, int64_t* num_phi, int64_t* num_r)
+{
+	int r = setDev(c);
+	if (r)
+		return r;
+	if (h == nullptr || h->dets_per_ring <= 0 || h->num_rings <= 0 ||
+	    h->num_doi <= 0)
+		return fail(YRTMI_EINVAL, "bad histogram descriptor");
+	// Histogram3D ctor (Histogram3D.cpp:165-203)
+	if (h->dets_per_ring % 2 != 0)
+		return fail(YRTMI_EINVAL, "dets_per_ring must be even");
+	if (h->max_ring_diff >= h->num_rings || h->max_ring_diff < 0)
+		return fail(YRTMI_EINVAL, "max ring difference must be < number of rings");
+	if (h->min_ang_diff <= 0)
+		return fail(YRTMI_EINVAL, "minimum angle difference must be > 0");
+	if (h->num_rings >= 65536)
+		return fail(YRTMI_EINVAL, "too many rings");
+	int64_t rCut = h->min_ang_diff / 2;
+	if (h->min_ang_diff % 2 != 0 && h->dets_per_ring % 4 == 0)
+		rCut++;
+	const int64_t doiPoss = h->num_doi * h->num_doi;
+	const int64_t numR = doiPoss * (h->dets_per_ring / 2 + 1 - h->min_ang_diff);
+	const int64_t numPhi = h->dets_per_ring;
+	const int64_t dz = h->max_ring_diff;
+	int64_t numZ = (dz + 1) * h->num_rings - (dz * (dz + 1)) / 2;
+	const int64_t numZDiff = numZ - h->num_rings;
+	numZ += numZDiff;
+	if (numR <= 0)
+		return fail(YRTMI_EINVAL, "minimum angle difference too large");
+	if (numR * numPhi * numZ >= (int64_t(1) << 32))
+		return fail(YRTMI_EINVAL, "histogram larger than 2^32 bins");
+	if (h->dets_per_ring * h->num_rings * h->num_doi > c->ndet && c->lut != nullptr)
+		return fail(YRTMI_EINVAL, "histogram needs more detectors than the LUT has");
+	// z-bin table: the delta-z loop of getDetPairFromCoords (339-357)
+	std::vector<uint32_t> tab(numZ);
+	const int64_t nr = h->num_rings;
+	for (int64_t zb = 0; zb < numZ; zb++)
+	{
+		int64_t z1, z2;
+		if (zb < nr)
+		{
+			z1 = z2 = zb;
+		}
+		else
+		{
+			int64_t cur = (zb - nr) % numZDiff + nr;
+			int64_t planes = nr;
+			int64_t d = 0;
+			while (cur - planes >= 0)
+			{
+				cur -= planes;
+				planes -= 1;
+				d++;
+			}
+			z1 = cur;
+			z2 = z1 + d;
+			if (zb - nr >= numZDiff)
+				std::swap(z1, z2);
+		}
+		tab[zb] = (uint32_t)z1 | ((uint32_t)z2 << 16);
+	}
+	HIP_TRY(hipStreamSynchronize(c->stream));
+	HIP_TRY(hipFree(c->zbin));
+	c->zbin = nullptr;
+	HIP_TRY(hipMalloc(&c->zbin, tab.size() * sizeof(uint32_t)));
+	HIP_TRY(hipMemcpy(c->zbin, tab.data(), tab.size() * sizeof(uint32_t),
+	                  hipMemcpyHostToDevice));
+	c->histo = *h;
+	c->numR = (uint32_t)numR;
+	c->numPhi = (uint32_t)numPhi;
+	c->numZBin = (uint32_t)numZ;
+	c->numZBinDiff = (uint32_t)numZDiff;
+	c->rCut = (uint32_t)rCut;
+	c->histoReady = true;
+	if (num_z_bin)
+		*num_z_bin = numZ;
+	if (num_phi)
+		*num_phi = numPhi;
+	if (num_r)
+		*num_r = numR;
+	return YRTMI_OK;
+}
+
+int64_t yrtmi_histo_subset_size(yrtmi_ctx* c, int S, int s)
+{
+	if (c == nullptr || !c->histoReady || S <= 0 || s < 0 || s >= S)
+		return -1;
+	int64_t nps = c->numPhi / S;
+	if (s + nps * S < (int64_t)c->numPhi)
+		nps += 1;
+	return (int64_t)c->numR * nps * c->numZBin;
+}
+
+int yrtmi_motion_upload(yrtmi_ctx* c, const float* xf, int k)
+{
+	int r = setDev(c);
+	if (r)
+		return r;
+	if (xf == nullptr || k <= 0 || k > 256)
+		return fail(YRTMI_EINVAL, "motion: need 1..256 transforms");
+	HIP_TRY(hipStreamSynchronize(c->stream));
+	HIP_TRY(hipFree(c->motion));
+	c->motion = nullptr;
+	HIP_TRY(hipMalloc(&c->motion, sizeof(float) * 12 * k));
+	HIP_TRY(hipMemcpy(c->motion, xf, sizeof(float) * 12 * k,
+	                  hipMemcpyHostToDevice));
+	c->nmotion = k;
+	return YRTMI_OK;
+}
+
+int yrtmi_proj_create(yrtmi_ctx* c, const yrtmi_projector_desc* d,
+                      yrtmi_proj** out)
+{
+	int r = setDev(c);
+	if (r)
+		return r;
+	if (d == nullptr || out == nullptr)
+		return fail(YRTMI_EINVAL, "null projector descriptor");
+	if (d->type != YRTMI_PROJ_SIDDON && d->type != YRTMI_PROJ_DD)
+		return fail(YRTMI_EINVAL, "unknown projector type");
+	if (d->num_rays < 1 || d->num_rays > YRTMI_MAX_RAYS)
+		return fail(YRTMI_EINVAL, "num_rays must be in [1, 16]");
+	if (d->use_tof && !(d->tof_width_ps > 0.f))
+		return fail(YRTMI_EINVAL, "TOF width must be positive");
+	auto* p = new yrtmi_proj();
+	p->ctx = c;
+	p->device = c->device;
+	DProj& q = p->d;
+	q.type = d->type;
+	q.numRays = (d->type == YRTMI_PROJ_SIDDON) ? d->num_rays : 1;
+	q.useTof = d->use_tof ? 1 : 0;
+	q.setup64 = d->setup_fp64 ? 1 : 0;
+	q.tz = d->crystal_size_z;
+	q.tt = d->crystal_size_trans;
+	// TimeOfFlightHelper ctor (TimeOfFlight.hpp:19-34), FP32 members
+	if (d->use_tof)
+	{
+		const double width_mm = d->tof_width_ps * 0.299792458 * 0.5;
+		q.tof.sigma = (float)(width_mm / (2 * sqrtf(2 * logf(2.0f))));
+		q.tof.trunc = (d->tof_n_std <= 0) ? -1.f : d->tof_n_std * q.tof.sigma;
+		q.tof.norm = (float)(1 / (std::sqrt(2 * 3.14159265358979) * q.tof.sigma));
+	}
+	// multi-ray offsets: rand_r(seed = 13), 4 per extra ray
+	// (MultiRayGenerator.cpp:78-92; the seed is reset for every LOR,
+	// ProjectorSiddon.cpp:258,330)
+	unsigned int seed = 13;
+	for (int i = 0; i < 4 * (q.numRays - 1); i++)
+		q.rayOff[i] = static_cast<float>(rand_r(&seed)) /
+		                  static_cast<float>(RAND_MAX) -
+		              0.5f;
+	if (d->type == YRTMI_PROJ_DD && d->psf_kernels != nullptr)
+	{
+		if (d->psf_ksize <= 0 || d->psf_ksize % 2 == 0 || d->psf_num_s <= 0 ||
+		    !(d->psf_s_step > 0.f) || !(d->psf_k_spacing > 0.f))
+		{
+			delete p;
+			return fail(YRTMI_EINVAL, "PSF kernels must have odd length and positive steps");
+		}
+		const int64_t nk = (int64_t)d->psf_num_s * d->psf_ksize;
+		std::vector<float> tab(2 * nk);
+		for (int64_t i = 0; i < d->psf_num_s; i++)
+			for (int64_t j = 0; j < d->psf_ksize; j++)
+			{
+				const float v = d->psf_kernels[i * d->psf_ksize + j];
+				tab[i * d->psf_ksize + j] = v;
+				tab[nk + i * d->psf_ksize + (d->psf_ksize - 1 - j)] = v;
+			}
+		hipError_t e = hipMalloc(&p->psf, tab.size() * sizeof(float));
+		if (e == hipSuccess)
+			e = hipMemcpy(p->psf, tab.data(), tab.size() * sizeof(float),
+			              hipMemcpyHostToDevice);
+		if (e != hipSuccess)
+		{
+			(void)hipFree(p->psf);
+			delete p;
+			return hipFail(e, "yrtmi_proj_create (PSF upload)");
+		}
+		q.psf.k = p->psf;
+		q.psf.kflip = p->psf + nk;
+		q.psf.numS = d->psf_num_s;
+		q.psf.K = d->psf_ksize;
+		q.psf.sStep = d->psf_s_step;
+		q.psf.kSpacing = d->psf_k_spacing;
+		// getHalfWidth_mm (ProjectionPsfManager.cpp:52-56)
+		q.psf.halfWidth = (float)((d->psf_ksize + 1) / 2) * d->psf_k_spacing;
+	}
+	p->lpe = (q.numRays >= 4) ? 4 : (q.numRays >= 2 ? 2 : 1);
+	*out = p;
+	return YRTMI_OK;
+}
+
+int yrtmi_proj_destroy(yrtmi_proj* p)
+{
+	if (p == nullptr)
+		return YRTMI_OK;
+	if (p->psf)
+	{
+		// hipFree waits for the device's outstanding work
+		(void)hipSetDevice(p->device);
+		(void)hipFree(p->psf);
+	}
+	delete p;
+	return YRTMI_OK;
+}
+
+int yrtmi_proj_set_launch(yrtmi_proj* p, int lpe, int block, int grid)
+{
+	if (p == nullptr)
+		return fail(YRTMI_EINVAL, "null projector");
+	if (lpe != 1 && lpe != 2 && lpe != 4 && lpe != 8)
+		return fail(YRTMI_EINVAL, "lanes_per_event must be 1, 2, 4 or 8");
+	if (block < 64 || block > 256 || block % 64 != 0)
+		return fail(YRTMI_EINVAL, "block size must be 64, 128, 192 or 256");
+	if (grid < 0)
+		return fail(YRTMI_EINVAL, "grid must be >= 0");
+	p->lpe = lpe;
+	p->block = block;
+	p->grid = grid;
+	return YRTMI_OK;
+}
+
+static int prepare(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
+                   const yrtmi_lors* l, KArgs& a)
+{
+	int r = setDev(c);
+	if (r)
+		return r;
+	if (p == nullptr || p->ctx != c)
+		return fail(YRTMI_EINVAL, "projector does not belong to this context");
+	r = checkImage(im, "image");
+	if (r)
+		return r;
+	a = KArgs{};
+	a.im = toDImage(im);
+	a.pr = p->d;
+	return buildLors(c, l, a.L, a.sc);
+}
+
+int yrtmi_forward(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
+                  const float* img, const yrtmi_lors* l, float* out)
+{
+	KArgs a;
+	int r = prepare(c, p, im, l, a);
+	if (r)
+		return r;
+	if (l->n == 0)
+		return YRTMI_OK;
+	if (img == nullptr || out == nullptr)
+		return fail(YRTMI_EINVAL, "forward: null image or output");
+	a.x = img;
+	a.out = out;
+	HIP_TRY(launchEvents(MODE_FWD, a, p->lpe, p->block, p->grid, c->stream));
+	return YRTMI_OK;
+}
+
+int yrtmi_back(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
+               float* img, const yrtmi_lors* l, const float* vals)
+{
+	KArgs a;
+	int r = prepare(c, p, im, l, a);
+	if (r)
+		return r;
+	if (l->n == 0)
+		return YRTMI_OK;
+	if (img == nullptr || vals == nullptr)
+		return fail(YRTMI_EINVAL, "back: null image or values");
+	a.y = img;
+	a.vals = vals;
+	HIP_TRY(launchEvents(MODE_BACK, a, p->lpe, p->block, p->grid, c->stream));
+	return YRTMI_OK;
+}
+
+int yrtmi_em_subset(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
+                    const float* x, float* u, const yrtmi_lors* l,
+                    const yrtmi_corrections* corr)
+{
+	KArgs a;
+	int r = prepare(c, p, im, l, a);
+	if (r)
+		return r;
+	if (l->n == 0)
+		return YRTMI_OK;
+	if (x == nullptr || u == nullptr)
+		return fail(YRTMI_EINVAL, "em_subset: null image");
+	a.x = x;
+	a.y = u;
+	a.c.g = 1.0f;
+	a.c.denomThr = 1e-35f;
+	if (corr != nullptr)
+	{
+		a.c.meas = corr->meas;
+		a.c.sens = corr->sens;
+		a.c.att = corr->att;
+		a.c.randoms = corr->randoms;
+		a.c.scatter = corr->scatter;
+		a.c.g = corr->global_scale;
+		a.c.denomThr = corr->denom_threshold;
+	}
+	HIP_TRY(launchEvents(MODE_EM, a, p->lpe, p->block, p->grid, c->stream));
+	return YRTMI_OK;
+}
+
+int yrtmi_sensitivity(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
+                      float* img, const yrtmi_lors* l,
+                      const yrtmi_image* att_im, const float* att_img,
+                      const float* sens)
+{
+	KArgs a;
+	int r = prepare(c, p, im, l, a);
+	if (r)
+		return r;
+	if (l->n == 0)
+		return YRTMI_OK;
+	if (img == nullptr)
+		return fail(YRTMI_EINVAL, "sensitivity: null image");
+	if (l->frame != nullptr)
+		return fail(YRTMI_EINVAL, "sensitivity: LORs must not carry frames");
+	if (att_img != nullptr)
+	{
+		r = checkImage(att_im, "attenuation image");
+		if (r)
+			return r;
+		a.attIm = toDImage(att_im);
+		a.att = att_img;
+	}
+	a.y = img;
+	a.vals = sens;
+	a.L.tof = nullptr;
+	HIP_TRY(launchEvents(MODE_SENS, a, p->lpe, p->block, p->grid, c->stream));
+	return YRTMI_OK;
+}
+
+int yrtmi_attenuation_factors(yrtmi_ctx* c, const yrtmi_image* att_im,
+                              const float* att_img, const yrtmi_lors* l,
+                              float* acf)
+{
+	int r = setDev(c);
+	if (r)
+		return r;
+	r = checkImage(att_im, "attenuation image");
+	if (r)
+		return r;
+	KArgs a{};
+	r = buildLors(c, l, a.L, a.sc);
+	if (r)
+		return r;
+	if (l->n == 0)
+		return YRTMI_OK;
+	if (att_img == nullptr || acf == nullptr)
+		return fail(YRTMI_EINVAL, "attenuation: null image or output");
+	a.attIm = toDImage(att_im);
+	a.att = att_img;
+	a.out = acf;
+	HIP_TRY(launchAttFactors(a, c->stream));
+	return YRTMI_OK;
+}
+
+int yrtmi_em_update(yrtmi_ctx* c, float* x, const float* u, const float* s,
+                    int64_t nvox, int64_t nt, int64_t sf, float thr)
+{
+	int r = setDev(c);
+	if (r)
+		return r;
+	if (x == nullptr || u == nullptr || s == nullptr || nvox <= 0 || nt <= 0 ||
+	    (sf != 1 && sf != nt))
+		return fail(YRTMI_EINVAL, "em_update: bad arguments");
+	HIP_TRY(launchEmUpdate(x, u, s, nvox, nt, sf, thr, c->stream));
+	return YRTMI_OK;
+}
+
+int yrtmi_threshold(yrtmi_ctx* c, float* x, const float* m, int64_t nvox,
+                    int64_t nt, int64_t mf, float thr, float les, float leo,
+                    float gts, float gto)
+{
+	int r = setDev(c);
+	if (r)
+		return r;
+	if (x == nullptr || m == nullptr || nvox <= 0 || nt <= 0 ||
+	    (mf != 1 && mf != nt))
+		return fail(YRTMI_EINVAL, "threshold: bad arguments");
+	HIP_TRY(launchThreshold(x, m, nvox, nt, mf, thr, les, leo, gts, gto,
+	                        c->stream));
+	return YRTMI_OK;
+}
+
+int yrtmi_fill(yrtmi_ctx* c, float* x, int64_t n, float v)
+{
+	int r = setDev(c);
+	if (r)
+		return r;
+	if (x == nullptr || n < 0)
+		return fail(YRTMI_EINVAL, "fill: bad arguments");
+	if (n == 0)
+		return YRTMI_OK;
+	HIP_TRY(launchFill(x, n, v, c->stream));
+	return YRTMI_OK;
+}
+
+int yrtmi_count_segments(yrtmi_ctx* c, const yrtmi_proj* p,
+                         const yrtmi_image* im, const yrtmi_lors* l,
+                         int64_t* total)
+{
+	KArgs a;
+	int r = prepare(c, p, im, l, a);
+	if (r)
+		return r;
+	if (total == nullptr)
+		return fail(YRTMI_EINVAL, "count: null output");
+	*total = 0;
+	if (l->n == 0)
+		return YRTMI_OK;
+	unsigned long long zero = 0;
+	HIP_TRY(hipMemcpyAsync(c->counter, &zero, sizeof(zero),
+	                       hipMemcpyHostToDevice, c->stream));
+	a.counter = c->counter;
+	HIP_TRY(launchEvents(MODE_COUNT, a, p->lpe, p->block, p->grid, c->stream));
+	unsigned long long h = 0;
+	HIP_TRY(hipMemcpyAsync(&h, c->counter, sizeof(h), hipMemcpyDeviceToHost,
+	                       c->stream));
+	HIP_TRY(hipStreamSynchronize(c->stream));
+	*total = (int64_t)h;
+	return YRTMI_OK;
+}
+
+}  // extern "C"

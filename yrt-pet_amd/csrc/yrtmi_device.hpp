@@ -1,0 +1,367 @@
+// yrtmi device functions for gfx950: LOR fetch from SoA events and the
+// resident detector LUT, TOF Gaussian, projection-space PSF integral, and the
+// Siddon / distance-driven ray kernels.
+//
+// Arithmetic: FP32 storage and accumulation; the ray-plane intersection
+// setup (FOV-cylinder clip, slab alphas, TOF window, first crossings, first
+// voxel index) runs in FP64 for near-grazing stability.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace yrtmi
+{
+
+constexpr int kMaxRays = 16;
+
+enum Mode : int
+{
+	MODE_FWD = 0,
+	MODE_BACK = 1,
+	MODE_EM = 2,
+	MODE_SENS = 3,
+	MODE_COUNT = 4
+};
+
+// Image geometry (ImageParams).
+struct DImage
+{
+	int32_t nx, ny, nz, nt;
+	float vx, vy, vz, lx, ly, lz, ox, oy, oz, fovR;
+	int32_t nxy;
+	int64_t nvox;
+};
+
+// TimeOfFlightHelper constants (TimeOfFlight.hpp:19-34): sigma, truncation
+// width (<= 0: none) and Gaussian norm, all mm.
+struct DTof
+{
+	float sigma, trunc, norm;
+};
+
+// Projection-space PSF tables (ProjectionPsfManager.cpp:27-103).
+struct DPsf
+{
+	const float* k;      // numS x K
+	const float* kflip;  // numS x K, reversed taps (backprojection)
+	int32_t numS, K;
+	float sStep, kSpacing, halfWidth;
+};
+
+struct DProj
+{
+	int32_t type;
+	int32_t numRays;
+	int32_t useTof;
+	int32_t setup64;  // FP64 ray-plane setup instead of the reference FP32 order
+	DTof tof;
+	float tz, tt;  // crystal size z / transaxial
+	float rayOff[4 * (kMaxRays - 1)];
+	DPsf psf;
+};
+
+// Device-resident scanner tables.
+struct DScanner
+{
+	const float* lut;  // SoA: x[n], y[n], z[n], ox[n], oy[n], oz[n]
+	uint32_t ndet;
+	// Histogram3D geometry
+	uint32_t numR, numPhi, numZBin, rCut, numDOI, numDOIPoss, dpr, nrings;
+	const uint32_t* zbin;  // z1 | (z2 << 16) per z bin
+	const float* motion;   // k x 12 (transform_t)
+	int32_t nmotion;
+};
+
+struct DLors
+{
+	int32_t kind;  // 0 pairs, 1 bin list, 2 histogram subset iterator
+	int64_t n;
+	const uint32_t* d1;
+	const uint32_t* d2;
+	const uint32_t* bins;
+	uint32_t S, s, nps;  // subset iterator: S subsets, subset s, phis in it
+	const float* tof;
+	const int32_t* frame;
+	const uint8_t* motion;
+};
+
+struct DCorr
+{
+	const float* meas;
+	const float* sens;
+	const float* att;
+	const float* randoms;
+	const float* scatter;
+	float g;
+	float denomThr;
+};
+
+struct KArgs
+{
+	DImage im;
+	DProj pr;
+	DScanner sc;
+	DLors L;
+	DCorr c;
+	const float* x;   // image read by forward projection
+	float* y;         // image accumulated by backprojection
+	const float* vals;  // MODE_BACK values / MODE_SENS per-LOR factor
+	float* out;       // MODE_FWD output
+	DImage attIm;     // MODE_SENS attenuation image geometry
+	const float* att;  // MODE_SENS attenuation image (nullable)
+	unsigned long long* counter;  // MODE_COUNT
+};
+
+struct Lor
+{
+	float3 p1, p2, n1, n2;
+	float tof;
+	int32_t frame;
+};
+
+// ---------------------------------------------------------------------------
+// Histogram3D closed-form bin -> detector pair
+// (Histogram3D::getDetPairInSameRing, Histogram3D.cpp:428-450, and
+//  getDetPairFromCoords 299-363 with the z loop tabulated).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void histoPair(const DScanner& S, uint32_t r,
+                                          uint32_t phi, uint32_t zb,
+                                          uint32_t& d1, uint32_t& d2)
+{
+	const int32_t ntot = (int32_t)S.dpr;
+	const int32_t rr = (int32_t)(r / S.numDOIPoss);
+	const int32_t d02 = ntot / 2 + (int32_t)(phi & 1u);
+	const int32_t sh = rr - ntot / 4 + (int32_t)S.rCut;
+	int32_t dr1 = sh;
+	int32_t dr2 = d02 - sh;
+	if (dr1 < 0)
+		dr1 += ntot;
+	if (dr2 < 0)
+		dr2 += ntot;
+	int32_t a = (dr1 + (int32_t)(phi >> 1)) % ntot;
+	int32_t b = (dr2 + (int32_t)(phi >> 1)) % ntot;
+	if (a > b)
+	{
+		const int32_t t = a;
+		a = b;
+		b = t;
+	}
+	if (a == b)
+	{
+		d1 = d2 = 0;
+		return;
+	}
+	const uint32_t doiCase = r % S.numDOIPoss;
+	const uint32_t doi1 = doiCase % S.numDOI;
+	const uint32_t doi2 = doiCase / S.numDOI;
+	const uint32_t z = S.zbin[zb];
+	const uint32_t z1 = z & 0xffffu, z2 = z >> 16;
+	d1 = (uint32_t)a + z1 * S.dpr + doi1 * (S.dpr * S.nrings);
+	d2 = (uint32_t)b + z2 * S.dpr + doi2 * (S.dpr * S.nrings);
+}
+
+// Returns false for an event that cannot be resolved (detector id outside
+// the LUT, bin outside the histogram); such events are skipped.
+__device__ __forceinline__ bool eventPair(const DScanner& S, const DLors& L,
+                                          int64_t e, uint32_t& d1,
+                                          uint32_t& d2)
+{
+	if (L.kind == 0)
+	{
+		d1 = L.d1[e];
+		d2 = L.d2[e];
+		return d1 < S.ndet && d2 < S.ndet;
+	}
+	uint32_t r, phi, zb;
+	const uint32_t npr = S.numPhi * S.numR;
+	if (L.kind == 1)
+	{
+		const uint32_t bin = L.bins[e];
+		zb = bin / npr;
+		if (zb >= S.numZBin)
+		{
+			d1 = d2 = 0;
+			return false;
+		}
+		const uint32_t rem = bin - zb * npr;
+		phi = rem / S.numR;
+		r = rem - phi * S.numR;
+	}
+	else
+	{
+		// BinIteratorRangeHistogram3D::getSafe (BinIterator.cpp:191-198)
+		const uint32_t idx = (uint32_t)e;
+		const uint32_t per = L.nps * S.numR;
+		zb = idx / per;
+		const uint32_t rem = idx - zb * per;
+		const uint32_t ps = rem / S.numR;
+		r = rem - ps * S.numR;
+		phi = L.S * ps + L.s;
+	}
+	histoPair(S, r, phi, zb, d1, d2);
+	return true;
+}
+
+__device__ __forceinline__ float3 lutPos(const DScanner& S, uint32_t d)
+{
+	return make_float3(S.lut[d], S.lut[S.ndet + d], S.lut[2 * S.ndet + d]);
+}
+__device__ __forceinline__ float3 lutOri(const DScanner& S, uint32_t d)
+{
+	return make_float3(S.lut[3 * S.ndet + d], S.lut[4 * S.ndet + d],
+	                   S.lut[5 * S.ndet + d]);
+}
+
+// Matrix * Vector3D + translation (Matrix.cpp:177-186,
+// ProjectionData.cpp:307-318).
+__device__ __forceinline__ float3 moveP(const float* m, float3 p)
+{
+	float3 o;
+	o.x = __fadd_rn(__fadd_rn(__fadd_rn(__fmul_rn(m[0], p.x), __fmul_rn(m[1], p.y)),
+	                          __fmul_rn(m[2], p.z)),
+	                m[3]);
+	o.y = __fadd_rn(__fadd_rn(__fadd_rn(__fmul_rn(m[4], p.x), __fmul_rn(m[5], p.y)),
+	                          __fmul_rn(m[6], p.z)),
+	                m[7]);
+	o.z = __fadd_rn(__fadd_rn(__fadd_rn(__fmul_rn(m[8], p.x), __fmul_rn(m[9], p.y)),
+	                          __fmul_rn(m[10], p.z)),
+	                m[11]);
+	return o;
+}
+
+// ProjectionData::collectProjectionProperties (ProjectionData.cpp:198-273):
+// endpoints (with motion), detector orientations, TOF, dynamic frame.
+// Unresolvable events (bad detector id / bin / frame / motion index) come
+// back with frame = -1 so that every mode skips them.
+__device__ __forceinline__ Lor fetchLor(const DScanner& S, const DLors& L,
+                                        int64_t e, int32_t nt)
+{
+	uint32_t d1, d2;
+	bool ok = eventPair(S, L, e, d1, d2);
+	if (!ok)
+		d1 = d2 = 0;
+	Lor l;
+	l.p1 = lutPos(S, d1);
+	l.p2 = lutPos(S, d2);
+	l.n1 = lutOri(S, d1);
+	l.n2 = lutOri(S, d2);
+	if (L.motion != nullptr)
+	{
+		const int mi = (int)L.motion[e];
+		if (mi < S.nmotion)
+		{
+			const float* m = S.motion + 12 * mi;
+			l.p1 = moveP(m, l.p1);
+			l.p2 = moveP(m, l.p2);
+		}
+		else
+			ok = false;
+	}
+	l.tof = (L.tof != nullptr) ? L.tof[e] : 0.f;
+	l.frame = (L.frame != nullptr) ? L.frame[e] : 0;
+	if (!ok || l.frame >= nt)
+		l.frame = -1;
+	return l;
+}
+
+// ---------------------------------------------------------------------------
+// multi-ray line (MultiRayGenerator::getRandomLine, isParallel = hasDepth =
+// false, MultiRayGenerator.cpp:54-114)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float3 transDir(float3 n)
+{
+	// normalize(n x z) with Vector3D::normalize's 1e-24 guard
+	const float cx = __fsub_rn(__fmul_rn(n.y, 1.f), __fmul_rn(n.z, 0.f));
+	const float cy = __fsub_rn(__fmul_rn(n.z, 0.f), __fmul_rn(n.x, 1.f));
+	const float cz = __fsub_rn(__fmul_rn(n.x, 0.f), __fmul_rn(n.y, 0.f));
+	const float nn = __fsqrt_rn(
+	    __fadd_rn(__fadd_rn(__fmul_rn(cx, cx), __fmul_rn(cy, cy)), __fmul_rn(cz, cz)));
+	if ((double)nn > 10e-25)
+		return make_float3(__fdiv_rn(cx, nn), __fdiv_rn(cy, nn), __fdiv_rn(cz, nn));
+	return make_float3(0.f, 0.f, 0.f);
+}
+
+__device__ __forceinline__ void rayLine(const DProj& pr, const Lor& l, int r,
+                                        float3& q1, float3& q2)
+{
+	if (r == 0)
+	{
+		q1 = l.p1;
+		q2 = l.p2;
+		return;
+	}
+	const float* o = pr.rayOff + 4 * (r - 1);
+	const float3 t1 = transDir(l.n1);
+	const float3 t2 = transDir(l.n2);
+	const float s1 = __fmul_rn(o[0], pr.tz), s2 = __fmul_rn(o[1], pr.tt);
+	const float s3 = __fmul_rn(o[2], pr.tz), s4 = __fmul_rn(o[3], pr.tt);
+	q1.x = __fadd_rn(__fadd_rn(l.p1.x, __fmul_rn(0.f, s1)), __fmul_rn(t1.x, s2));
+	q1.y = __fadd_rn(__fadd_rn(l.p1.y, __fmul_rn(0.f, s1)), __fmul_rn(t1.y, s2));
+	q1.z = __fadd_rn(__fadd_rn(l.p1.z, __fmul_rn(1.f, s1)), __fmul_rn(t1.z, s2));
+	q2.x = __fsub_rn(__fadd_rn(l.p2.x, __fmul_rn(0.f, s3)), __fmul_rn(t2.x, s4));
+	q2.y = __fsub_rn(__fadd_rn(l.p2.y, __fmul_rn(0.f, s3)), __fmul_rn(t2.y, s4));
+	q2.z = __fsub_rn(__fadd_rn(l.p2.z, __fmul_rn(1.f, s3)), __fmul_rn(t2.z, s4));
+}
+
+// ---------------------------------------------------------------------------
+// TOF (TimeOfFlight.hpp:153-185).  Segment weight = Gaussian at the segment
+// midpoint: with a = alpha, x_c = (a_mid*d - pc)/sigma, pc = d/2 + tof_mm.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float tofMm(float tof_ps)
+{
+	return (float)((double)tof_ps * (double)0.299792458f * 0.5);
+}
+
+__device__ __forceinline__ float gaussW(float xc, float norm)
+{
+	return __expf(-0.5f * xc * xc) * norm;
+}
+
+// ---------------------------------------------------------------------------
+// Projection-PSF integral (ProjectorUtils.hpp:43-98 via
+// ProjectionPsfManager::getWeight, ProjectionPsfManager.cpp:93-103).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float psfTap(const float* k, int K, int i)
+{
+	return (i < 0 || i >= K) ? 0.f : k[i];
+}
+
+__device__ float psfWeight(const DPsf& P, const float* k, float x0, float x1)
+{
+	const int hw = (P.K + 1) / 2;
+	const float ks = P.kSpacing;
+	if (x0 > hw * ks || x1 < -hw * ks || x0 >= x1)
+		return 0.f;
+	const int K = P.K;
+	const int half = (K - 1) / 2;
+	if (x0 > (half + 1) * ks || x1 < -(half + 1) * ks)
+		return 0.f;
+	const float x0s = x0 / ks;
+	const int x0i = (int)floorf(x0s + (float)half);
+	const float tau0 = x0s + (float)(-(-half + x0i));
+	const float v0lo = psfTap(k, K, x0i);
+	const float v0hi = (x0i < -1 || x0i >= K - 1) ? 0.f : k[x0i + 1];
+	const float v0 = v0lo * (1.f - tau0) + v0hi * tau0;
+	const float x1s = x1 / ks;
+	const int x1i = (int)floorf(x1s + (float)half);
+	const float tau1 = x1s - (float)(-half + x1i);
+	const float v1lo = psfTap(k, K, x1i);
+	const float v1hi = (x1i < -1 || x1i >= K - 1) ? 0.f : k[x1i + 1];
+	const float v1 = v1lo * (1.f - tau1) + v1hi * tau1;
+	float out = 0.f;
+	if (x0i == x1i)
+	{
+		out = 0.5f * (v0 + v1) * (x1 - x0);
+	}
+	else
+	{
+		out += 0.5f * (v0 + v0hi) * ks * (1.f - tau0);
+		out += 0.5f * (v1lo + v1) * ks * tau1;
+		for (int xi = x0i + 1; xi < x1i; xi++)
+			out += ks * 0.5f * (psfTap(k, K, xi) + psfTap(k, K, xi + 1));
+	}
+	return out;
+}
+
+}  // namespace yrtmi

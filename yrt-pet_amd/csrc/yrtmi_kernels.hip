@@ -1,0 +1,365 @@
+// yrtmi kernels for gfx950: one event (or histogram bin) per group of
+// `LPE` lanes, the lanes splitting the rays of multi-ray Siddon and
+// reducing the forward sum with cross-lane shuffles.  Events stream from HBM
+// as SoA (d1, d2, tof, frame); detector positions come from the resident
+// LUT; images are FP32 float[nt][nz][ny][nx].
+#include "yrtmi_kernels.hpp"
+
+namespace yrtmi
+{
+
+template <int LPE>
+__device__ __forceinline__ float groupSum(float v)
+{
+#pragma unroll
+	for (int o = 1; o < LPE; o <<= 1)
+		v += __shfl_xor(v, o, LPE);
+	return v;
+}
+
+// Forward projection (MODE_FWD) or segment count (MODE_COUNT, `img`
+// unused) of the rays of one event handled by this lane (rays lane,
+// lane+LPE, ...), Siddon or DD.  Returns this lane's partial sum.
+template <int TYPE, bool TOF, int LPE, bool S64, int FMODE = MODE_FWD>
+__device__ __forceinline__ float fwdLane(const KArgs& a, const Lor& l,
+                                         int lane, const float* img,
+                                         uint32_t& cnt)
+{
+	static_assert(FMODE == MODE_FWD || FMODE == MODE_COUNT, "forward modes only");
+	const float3 off = make_float3(a.im.ox, a.im.oy, a.im.oz);
+	if (TYPE == 1)
+	{
+		if (lane != 0)
+			return 0.f;
+		const float3 d1 = make_float3(l.p1.x - off.x, l.p1.y - off.y, l.p1.z - off.z);
+		const float3 d2 = make_float3(l.p2.x - off.x, l.p2.y - off.y, l.p2.z - off.z);
+		return ddRay<FMODE, TOF, S64>(a.im, const_cast<float*>(img), d1, d2, l.n1,
+		                         l.n2, 0.f, a.pr, l.tof, cnt);
+	}
+	float s = 0.f;
+	for (int r = lane; r < a.pr.numRays; r += LPE)
+	{
+		float3 q1, q2;
+		rayLine(a.pr, l, r, q1, q2);
+		q1 = make_float3(q1.x - off.x, q1.y - off.y, q1.z - off.z);
+		q2 = make_float3(q2.x - off.x, q2.y - off.y, q2.z - off.z);
+		s += siddonRay<FMODE, TOF, S64>(a.im, const_cast<float*>(img), q1, q2,
+		                           0.f, a.pr.tof, l.tof, cnt);
+	}
+	return s;
+}
+
+template <int TYPE, bool TOF, int LPE, bool S64, int MODE>
+__device__ __forceinline__ void backLane(const KArgs& a, const Lor& l,
+                                         int lane, float* img, float v,
+                                         uint32_t& cnt)
+{
+	const float3 off = make_float3(a.im.ox, a.im.oy, a.im.oz);
+	if (TYPE == 1)
+	{
+		if (lane != 0)
+			return;
+		const float3 d1 = make_float3(l.p1.x - off.x, l.p1.y - off.y, l.p1.z - off.z);
+		const float3 d2 = make_float3(l.p2.x - off.x, l.p2.y - off.y, l.p2.z - off.z);
+		ddRay<MODE, TOF, S64>(a.im, img, d1, d2, l.n1, l.n2, v, a.pr, l.tof, cnt);
+		return;
+	}
+	// ProjectorSiddon::backProjection splits the value over the rays
+	// (ProjectorSiddon.cpp:321-328).
+	const float vr = (a.pr.numRays > 1) ? v / (float)a.pr.numRays : v;
+	for (int r = lane; r < a.pr.numRays; r += LPE)
+	{
+		float3 q1, q2;
+		rayLine(a.pr, l, r, q1, q2);
+		q1 = make_float3(q1.x - off.x, q1.y - off.y, q1.z - off.z);
+		q2 = make_float3(q2.x - off.x, q2.y - off.y, q2.z - off.z);
+		siddonRay<MODE, TOF, S64>(a.im, img, q1, q2, vr, a.pr.tof, l.tof, cnt);
+	}
+}
+
+template <int MODE, int TYPE, bool TOF, int LPE, bool S64>
+__global__ void __launch_bounds__(256) kEvents(KArgs a)
+{
+	const int64_t total = a.L.n * LPE;
+	const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+	uint32_t cnt = 0;
+	for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+	     t += stride)
+	{
+		const int64_t e = t / LPE;
+		const int lane = (int)(t & (LPE - 1));
+		const Lor l = fetchLor(a.sc, a.L, e, a.im.nt);
+		const int64_t fo = (int64_t)l.frame * a.im.nvox;
+		if (MODE == MODE_FWD)
+		{
+			float s = 0.f;
+			if (l.frame >= 0)
+				s = fwdLane<TYPE, TOF, LPE, S64>(a, l, lane, a.x + fo, cnt);
+			s = groupSum<LPE>(s);
+			if (lane == 0)
+				a.out[e] = (TYPE == 0 && a.pr.numRays > 1)
+				               ? s / (float)a.pr.numRays
+				               : s;
+		}
+		else if (MODE == MODE_BACK)
+		{
+			if (l.frame >= 0)
+				backLane<TYPE, TOF, LPE, S64, MODE_BACK>(a, l, lane, a.y + fo,
+				                                    a.vals[e], cnt);
+		}
+		else if (MODE == MODE_COUNT)
+		{
+			if (l.frame >= 0)
+				fwdLane<TYPE, TOF, LPE, S64, MODE_COUNT>(a, l, lane, nullptr, cnt);
+		}
+		else if (MODE == MODE_EM)
+		{
+			// computeEMUpdateImage (OSEM_CPU.cpp:329-374); skipping zero
+			// measurement / zero factor / negative frame as the GPU
+			// reference does (OSEM_GPU.cu:1222-1225) contributes exactly
+			// what the CPU path backprojects (a zero value).
+			const float m = a.c.meas ? a.c.meas[e] : 1.0f;
+			const float sf = a.c.sens ? a.c.sens[e] : 1.0f;
+			const float af = a.c.att ? a.c.att[e] : 1.0f;
+			if (m == 0.f || sf == 0.f || af == 0.f || l.frame < 0)
+				continue;
+			float s = fwdLane<TYPE, TOF, LPE, S64>(a, l, lane, a.x + fo, cnt);
+			s = groupSum<LPE>(s);
+			float upd = (TYPE == 0 && a.pr.numRays > 1) ? s / (float)a.pr.numRays : s;
+			if (a.c.sens)
+				upd *= sf;
+			if (a.c.att)
+				upd *= af;
+			upd *= a.c.g;
+			if (a.c.randoms)
+				upd += a.c.randoms[e];
+			if (a.c.scatter)
+				upd += a.c.scatter[e];
+			if (fabsf(upd) > a.c.denomThr)
+			{
+				float v = m / upd;
+				if (a.c.sens)
+					v *= sf;
+				if (a.c.att)
+					v *= af;
+				v *= a.c.g;
+				backLane<TYPE, TOF, LPE, S64, MODE_BACK>(a, l, lane, a.y + fo, v, cnt);
+			}
+		}
+		else if (MODE == MODE_SENS)
+		{
+			// generateSensImage (OSEM_CPU.cpp:146-156, Corrector_CPU.cpp:
+			// 17-45): value = acf * sens; acf from a single-ray Siddon of the
+			// mu image without the image offset (Corrector.cpp:601-611).
+			float v = a.vals ? a.vals[e] : 1.0f;
+			if (a.att)
+			{
+				uint32_t c2 = 0;
+				const float mu = siddonRay<MODE_FWD, false, S64>(
+				    a.attIm, const_cast<float*>(a.att), l.p1, l.p2, 0.f,
+				    a.pr.tof, 0.f, c2);
+				v *= expf(-mu * 0.1f);
+			}
+			if (l.frame >= 0)
+				backLane<TYPE, false, LPE, S64, MODE_BACK>(a, l, lane, a.y, v, cnt);
+		}
+	}
+	if (MODE == MODE_COUNT)
+	{
+		unsigned long long c = cnt;
+#pragma unroll
+		for (int o = 32; o >= 1; o >>= 1)
+			c += __shfl_xor(c, o);
+		if ((threadIdx.x & 63) == 0 && c != 0)
+			atomicAdd(a.counter, c);
+	}
+}
+
+// Per-LOR attenuation factor exp(-0.1 * Siddon(mu)) (Corrector.cpp:601-611).
+__global__ void __launch_bounds__(256) kAttFactors(KArgs a)
+{
+	const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+	for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < a.L.n;
+	     e += stride)
+	{
+		const Lor l = fetchLor(a.sc, a.L, e, 1);
+		uint32_t c2 = 0;
+		float mu = 0.f;
+		if (l.frame >= 0)
+			mu = siddonRay<MODE_FWD, false, false>(a.attIm, const_cast<float*>(a.att),
+			                                l.p1, l.p2, 0.f, a.pr.tof, 0.f, c2);
+		a.out[e] = expf(-mu * 0.1f);
+	}
+}
+
+// updateEM(Dynamic)_kernel (ImageSpaceKernels.cu:13-66): x *= u / s where
+// s > thr, s broadcast over frames when sFrames == 1.
+__global__ void __launch_bounds__(256) kEmUpdate(float* __restrict__ x,
+                                                 const float* __restrict__ u,
+                                                 const float* __restrict__ s,
+                                                 int64_t nvox, int64_t nt,
+                                                 int64_t sFrames, float thr)
+{
+	const int64_t n = nvox * nt;
+	const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+	for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+	     i += stride)
+	{
+		const float sv = s[sFrames == 1 ? i % nvox : i];
+		if (sv > thr)
+			x[i] *= u[i] / sv;
+	}
+}
+
+// applyThreshold(Broadcast)_kernel (ImageSpaceKernels.cu:79-130).
+__global__ void __launch_bounds__(256) kThreshold(float* __restrict__ x,
+                                                  const float* __restrict__ m,
+                                                  int64_t nvox, int64_t nt,
+                                                  int64_t mFrames, float thr,
+                                                  float les, float leo,
+                                                  float gts, float gto)
+{
+	const int64_t n = nvox * nt;
+	const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+	for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+	     i += stride)
+	{
+		const float mv = m[mFrames == 1 ? i % nvox : i];
+		x[i] = (mv <= thr) ? x[i] * les + leo : x[i] * gts + gto;
+	}
+}
+
+__global__ void __launch_bounds__(256) kFill(float* __restrict__ x, int64_t n,
+                                             float v)
+{
+	const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+	for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+	     i += stride)
+		x[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// launch helpers
+// ---------------------------------------------------------------------------
+template <int MODE, int TYPE, bool TOF, bool S64>
+static hipError_t launchLpe(const KArgs& a, int lpe, dim3 g, dim3 b,
+                            hipStream_t st)
+{
+	if (TYPE == 1)
+	{
+		// DD: one lane per event
+		kEvents<MODE, TYPE, TOF, 1, S64><<<g, b, 0, st>>>(a);
+		return hipGetLastError();
+	}
+	switch (lpe)
+	{
+	case 1:
+		kEvents<MODE, TYPE, TOF, 1, S64><<<g, b, 0, st>>>(a);
+		break;
+	case 2:
+		kEvents<MODE, TYPE, TOF, 2, S64><<<g, b, 0, st>>>(a);
+		break;
+	case 4:
+		kEvents<MODE, TYPE, TOF, 4, S64><<<g, b, 0, st>>>(a);
+		break;
+	case 8:
+		kEvents<MODE, TYPE, TOF, 8, S64><<<g, b, 0, st>>>(a);
+		break;
+	default:
+		return hipErrorInvalidValue;
+	}
+	return hipGetLastError();
+}
+
+template <int MODE, int TYPE, bool TOF>
+static hipError_t launchSetup(const KArgs& a, int lpe, dim3 g, dim3 b,
+                              hipStream_t st)
+{
+	return a.pr.setup64 ? launchLpe<MODE, TYPE, TOF, true>(a, lpe, g, b, st)
+	                    : launchLpe<MODE, TYPE, TOF, false>(a, lpe, g, b, st);
+}
+
+template <int MODE>
+static hipError_t launchMode(const KArgs& a, int lpe, dim3 g, dim3 b,
+                             hipStream_t st)
+{
+	const bool tof = a.pr.useTof != 0 && MODE != MODE_SENS;
+	if (a.pr.type == 1)
+		return tof ? launchSetup<MODE, 1, true>(a, 1, g, b, st)
+		           : launchSetup<MODE, 1, false>(a, 1, g, b, st);
+	return tof ? launchSetup<MODE, 0, true>(a, lpe, g, b, st)
+	           : launchSetup<MODE, 0, false>(a, lpe, g, b, st);
+}
+
+hipError_t launchEvents(int mode, const KArgs& a, int lpe, int block,
+                        int64_t grid, hipStream_t st)
+{
+	if (a.pr.type == 1)
+		lpe = 1;
+	const int64_t threads = a.L.n * lpe;
+	int64_t nb = (threads + block - 1) / block;
+	if (grid > 0 && grid < nb)
+		nb = grid;
+	if (nb < 1)
+		nb = 1;
+	if (nb > 0x7fffffff)
+		nb = 0x7fffffff;
+	const dim3 g((unsigned)nb), b((unsigned)block);
+	switch (mode)
+	{
+	case MODE_FWD:
+		return launchMode<MODE_FWD>(a, lpe, g, b, st);
+	case MODE_BACK:
+		return launchMode<MODE_BACK>(a, lpe, g, b, st);
+	case MODE_EM:
+		return launchMode<MODE_EM>(a, lpe, g, b, st);
+	case MODE_SENS:
+		return launchMode<MODE_SENS>(a, lpe, g, b, st);
+	case MODE_COUNT:
+		return launchMode<MODE_COUNT>(a, lpe, g, b, st);
+	default:
+		return hipErrorInvalidValue;
+	}
+}
+
+static dim3 gridFor(int64_t n)
+{
+	int64_t nb = (n + 255) / 256;
+	if (nb > 65536)
+		nb = 65536;
+	if (nb < 1)
+		nb = 1;
+	return dim3((unsigned)nb);
+}
+
+hipError_t launchAttFactors(const KArgs& a, hipStream_t st)
+{
+	kAttFactors<<<gridFor(a.L.n), 256, 0, st>>>(a);
+	return hipGetLastError();
+}
+
+hipError_t launchEmUpdate(float* x, const float* u, const float* s,
+                          int64_t nvox, int64_t nt, int64_t sFrames, float thr,
+                          hipStream_t st)
+{
+	kEmUpdate<<<gridFor(nvox * nt), 256, 0, st>>>(x, u, s, nvox, nt, sFrames,
+	                                              thr);
+	return hipGetLastError();
+}
+
+hipError_t launchThreshold(float* x, const float* m, int64_t nvox, int64_t nt,
+                           int64_t mFrames, float thr, float les, float leo,
+                           float gts, float gto, hipStream_t st)
+{
+	kThreshold<<<gridFor(nvox * nt), 256, 0, st>>>(x, m, nvox, nt, mFrames,
+	                                               thr, les, leo, gts, gto);
+	return hipGetLastError();
+}
+
+hipError_t launchFill(float* x, int64_t n, float v, hipStream_t st)
+{
+	kFill<<<gridFor(n), 256, 0, st>>>(x, n, v);
+	return hipGetLastError();
+}
+
+}  // namespace yrtmi

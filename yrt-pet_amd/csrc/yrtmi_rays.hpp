@@ -1,0 +1,654 @@
+// Siddon and distance-driven ray kernels (device functions).
+//
+// Two setup precisions, chosen per projector (template S64):
+//  - S64 = false (default): the ray-plane setup in FP32 with the reference's
+//    operation order and no FMA contraction, so that projections agree with
+//    the reference CPU path to rounding level even on rays that graze voxel
+//    planes, where FP32 setup itself is inaccurate.
+//  - S64 = true: the setup (FOV-cylinder clip, slab alphas, TOF window, first
+//    crossings, first voxel) in FP64; it matches the FP64 oracle on such
+//    grazing rays.
+// Traversal, weights and accumulation are FP32 in both.
+#pragma once
+
+#include "yrtmi_device.hpp"
+
+namespace yrtmi
+{
+
+// util::get_alpha (ProjectorUtils.hpp:15-41): slab [r0, r1] entry / exit
+// alphas; a ray flat in this axis and outside the slab gives the empty range
+// amin = 1, amax = 0 (callers decide what an empty range means).
+template <typename T>
+__device__ __forceinline__ void getAlpha(T r0, T r1, T p1, T p2, T inv, T& amin,
+                                         T& amax)
+{
+#pragma clang fp contract(off)
+	amin = T(0);
+	amax = T(1);
+	if (p1 != p2)
+	{
+		const T a0 = (r0 - p1) * inv;
+		const T a1 = (r1 - p1) * inv;
+		if (a0 < a1)
+		{
+			amin = a0;
+			amax = a1;
+		}
+		else
+		{
+			amin = a1;
+			amax = a0;
+		}
+	}
+	else if (p1 < r0 || p1 > r1)
+	{
+		amax = T(0);
+		amin = T(1);
+	}
+}
+
+// Siddon ray (ProjectorSiddon::project_helper<IS_FWD, FLAG_INCR=true,
+// FLAG_TOF>, ProjectorSiddon.cpp:478-767).  `img` points at the frame's
+// voxels.  MODE_FWD returns the line integral; MODE_BACK adds val*w
+// atomically; MODE_COUNT counts applied segments.
+template <int MODE, bool TOF, bool S64>
+__device__ __forceinline__ float siddonRay(const DImage& P,
+                                           float* __restrict__ img, float3 p1,
+                                           float3 p2, float val,
+                                           const DTof& tof, float tofv,
+                                           uint32_t& cnt)
+{
+#pragma clang fp contract(off)
+	const float FMAX = 3.402823466e+38f;
+	const bool flat_x = (p1.x == p2.x);
+	const bool flat_y = (p1.y == p2.y);
+	const bool flat_z = (p1.z == p2.z);
+	int dir_x, dir_y, dir_z;
+	float invxf, invyf, invzf;
+	float a_cur, amaxf, axn = FMAX, ayn = FMAX, azn = FMAX;
+	float x_cur, y_cur, z_cur, dnorm;
+	float tA = 0.f, tB = 0.f;  // TOF: S64 x_c = a_mid*tA - tB; S32 pc = tB
+	// first-voxel data (S64 only)
+	double x1d = 0, y1d = 0, z1d = 0, ddx = 0, ddy = 0, ddz = 0;
+	if constexpr (S64)
+	{
+		x1d = p1.x;
+		y1d = p1.y;
+		z1d = p1.z;
+		ddx = (double)p2.x - x1d;
+		ddy = (double)p2.y - y1d;
+		ddz = (double)p2.z - z1d;
+		double amin = 0.0, amax = 1.0;
+		const double A = ddx * ddx + ddy * ddy;
+		if (A != 0.0)
+		{
+			const double R = P.fovR;
+			const double B = 2.0 * (ddx * x1d + ddy * y1d);
+			const double Cq = x1d * x1d + y1d * y1d - R * R;
+			const double Delta = B * B - 4.0 * A * Cq;
+			if (Delta <= 0.0)
+				return 0.f;
+			const double sq = sqrt(Delta);
+			amin = fmax(amin, (-B - sq) / (2.0 * A));
+			amax = fmin(amax, (-B + sq) / (2.0 * A));
+		}
+		const double dnormd = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+		const double invx = flat_x ? 0.0 : 1.0 / ddx;
+		const double invy = flat_y ? 0.0 : 1.0 / ddy;
+		const double invz = flat_z ? 0.0 : 1.0 / ddz;
+		dir_x = (invx >= 0.0) ? 1 : -1;
+		dir_y = (invy >= 0.0) ? 1 : -1;
+		dir_z = (invz >= 0.0) ? 1 : -1;
+		const double hx = 0.5 * (double)P.lx, hy = 0.5 * (double)P.ly,
+		             hz = 0.5 * (double)P.lz;
+		// util::get_alpha (ProjectorUtils.hpp:15-41)
+		if (!flat_x)
+		{
+			const double a0 = (-hx - x1d) * invx, a1 = (hx - x1d) * invx;
+			amin = fmax(amin, fmin(a0, a1));
+			amax = fmin(amax, fmax(a0, a1));
+		}
+		else if (x1d < -hx || x1d > hx)
+			return 0.f;
+		if (!flat_y)
+		{
+			const double a0 = (-hy - y1d) * invy, a1 = (hy - y1d) * invy;
+			amin = fmax(amin, fmin(a0, a1));
+			amax = fmin(amax, fmax(a0, a1));
+		}
+		else if (y1d < -hy || y1d > hy)
+			return 0.f;
+		if (!flat_z)
+		{
+			const double a0 = (-hz - z1d) * invz, a1 = (hz - z1d) * invz;
+			amin = fmax(amin, fmin(a0, a1));
+			amax = fmin(amax, fmax(a0, a1));
+		}
+		else if (z1d < -hz || z1d > hz)
+			return 0.f;
+		if (TOF)
+		{
+			const double tmm = (double)tofMm(tofv);
+			if (tof.trunc > 0.f)
+			{
+				amin = fmax(amin, 0.5 + (tmm - (double)tof.trunc) / dnormd);
+				amax = fmin(amax, 0.5 + (tmm + (double)tof.trunc) / dnormd);
+			}
+			tB = (float)((0.5 * dnormd + tmm) / (double)tof.sigma);
+			tA = (float)(dnormd / (double)tof.sigma);
+		}
+		if ((invx >= 0.0 && x1d > hx) || (invx < 0.0 && x1d < -hx) ||
+		    (invy >= 0.0 && y1d > hy) || (invy < 0.0 && y1d < -hy) ||
+		    (invz >= 0.0 && z1d > hz) || (invz < 0.0 && z1d < -hz))
+			return 0.f;
+		if (!(amin < amax))
+			return 0.f;
+		double xc = (invx > 0.0) ? -hx : hx;
+		double yc = (invy > 0.0) ? -hy : hy;
+		double zc = (invz > 0.0) ? -hz : hz;
+		if (!flat_x)
+		{
+			xc += ceil((double)dir_x * (amin * ddx - xc + x1d) / (double)P.vx) *
+			      dir_x * (double)P.vx;
+			axn = (float)((xc - x1d) * invx);
+		}
+		if (!flat_y)
+		{
+			yc += ceil((double)dir_y * (amin * ddy - yc + y1d) / (double)P.vy) *
+			      dir_y * (double)P.vy;
+			ayn = (float)((yc - y1d) * invy);
+		}
+		if (!flat_z)
+		{
+			zc += ceil((double)dir_z * (amin * ddz - zc + z1d) / (double)P.vz) *
+			      dir_z * (double)P.vz;
+			azn = (float)((zc - z1d) * invz);
+		}
+		x_cur = (float)xc;
+		y_cur = (float)yc;
+		z_cur = (float)zc;
+		invxf = (float)invx;
+		invyf = (float)invy;
+		invzf = (float)invz;
+		dnorm = (float)dnormd;
+		amaxf = (float)amax;
+		a_cur = (float)amin;
+	}
+	else
+	{
+		// reference FP32 order (ProjectorSiddon.cpp:499-594)
+		const float A = (p2.x - p1.x) * (p2.x - p1.x) + (p2.y - p1.y) * (p2.y - p1.y);
+		const float B = 2.0f * ((p2.x - p1.x) * p1.x + (p2.y - p1.y) * p1.y);
+		const float Cq = p1.x * p1.x + p1.y * p1.y - P.fovR * P.fovR;
+		const float Delta = B * B - 4.0f * A * Cq;
+		float t0 = 0.f, t1 = 1.f;
+		if (A != 0.f)
+		{
+			if (Delta <= 0.f)
+				return 0.f;
+			t0 = (-B - sqrtf(Delta)) / (2.0f * A);
+			t1 = (-B + sqrtf(Delta)) / (2.0f * A);
+		}
+		const float ex = p1.x - p2.x, ey = p1.y - p2.y, ez = p1.z - p2.z;
+		dnorm = sqrtf(ex * ex + ey * ey + ez * ez);
+		invxf = flat_x ? 0.0f : 1.0f / (p2.x - p1.x);
+		invyf = flat_y ? 0.0f : 1.0f / (p2.y - p1.y);
+		invzf = flat_z ? 0.0f : 1.0f / (p2.z - p1.z);
+		dir_x = (invxf >= 0.0f) ? 1 : -1;
+		dir_y = (invyf >= 0.0f) ? 1 : -1;
+		dir_z = (invzf >= 0.0f) ? 1 : -1;
+		const float x0 = -P.lx * 0.5f, x1 = P.lx * 0.5f;
+		const float y0 = -P.ly * 0.5f, y1 = P.ly * 0.5f;
+		const float z0 = -P.lz * 0.5f, z1 = P.lz * 0.5f;
+		float amin = fmaxf(0.0f, t0), amax = fminf(1.0f, t1);
+		if (!flat_x)
+		{
+			const float a0 = (-0.5f * P.lx - p1.x) * invxf, a1 = (0.5f * P.lx - p1.x) * invxf;
+			amin = fmaxf(amin, (a0 < a1) ? a0 : a1);
+			amax = fminf(amax, (a0 < a1) ? a1 : a0);
+		}
+		else if (p1.x < -0.5f * P.lx || p1.x > 0.5f * P.lx)
+			return 0.f;
+		if (!flat_y)
+		{
+			const float a0 = (-0.5f * P.ly - p1.y) * invyf, a1 = (0.5f * P.ly - p1.y) * invyf;
+			amin = fmaxf(amin, (a0 < a1) ? a0 : a1);
+			amax = fminf(amax, (a0 < a1) ? a1 : a0);
+		}
+		else if (p1.y < -0.5f * P.ly || p1.y > 0.5f * P.ly)
+			return 0.f;
+		if (!flat_z)
+		{
+			const float a0 = (-0.5f * P.lz - p1.z) * invzf, a1 = (0.5f * P.lz - p1.z) * invzf;
+			amin = fmaxf(amin, (a0 < a1) ? a0 : a1);
+			amax = fminf(amax, (a0 < a1) ? a1 : a0);
+		}
+		else if (p1.z < -0.5f * P.lz || p1.z > 0.5f * P.lz)
+			return 0.f;
+		if (TOF)
+		{
+			// TimeOfFlightHelper::getAlphaRange (TimeOfFlight.hpp:153-173)
+			const float tmm = tofMm(tofv);
+			if (tof.trunc > 0.f)
+			{
+				const float lo = (float)fmax(0.0, 0.5 + (double)((tmm - tof.trunc) / dnorm));
+				const float hi = (float)fmin(1.0, 0.5 + (double)((tmm + tof.trunc) / dnorm));
+				amin = fmaxf(amin, lo);
+				amax = fminf(amax, hi);
+			}
+			tB = (float)(0.5 * (double)dnorm + (double)tmm);  // pc
+		}
+		x_cur = (invxf > 0.0f) ? x0 : x1;
+		y_cur = (invyf > 0.0f) ? y0 : y1;
+		z_cur = (invzf > 0.0f) ? z0 : z1;
+		if ((invxf >= 0.0f && p1.x > x1) || (invxf < 0.0f && p1.x < x0) ||
+		    (invyf >= 0.0f && p1.y > y1) || (invyf < 0.0f && p1.y < y0) ||
+		    (invzf >= 0.0f && p1.z > z1) || (invzf < 0.0f && p1.z < z0))
+			return 0.f;
+		if (!(amin < amax))
+			return 0.f;
+		a_cur = amin;
+		amaxf = amax;
+		if (!flat_x)
+		{
+			const float kx = ceilf((float)dir_x * (a_cur * (p2.x - p1.x) - x_cur + p1.x) / P.vx);
+			x_cur += (float)((int)kx * dir_x) * P.vx;
+			axn = (x_cur - p1.x) * invxf;
+		}
+		if (!flat_y)
+		{
+			const float ky = ceilf((float)dir_y * (a_cur * (p2.y - p1.y) - y_cur + p1.y) / P.vy);
+			y_cur += (float)((int)ky * dir_y) * P.vy;
+			ayn = (y_cur - p1.y) * invyf;
+		}
+		if (!flat_z)
+		{
+			const float kz = ceilf((float)dir_z * (a_cur * (p2.z - p1.z) - z_cur + p1.z) / P.vz);
+			z_cur += (float)((int)kz * dir_z) * P.vz;
+			azn = (z_cur - p1.z) * invzf;
+		}
+	}
+	// ---- FP32 traversal (622-766) ----
+	const float sdx = (float)dir_x * P.vx, sdy = (float)dir_y * P.vy,
+	            sdz = (float)dir_z * P.vz;
+	float axp = axn, ayp = ayn, azp = azn;
+	int vx = 0, vy = 0, vz = 0;
+	int off = 0;
+	int dprev = 0;
+	bool first = true;
+	float acc = 0.f;
+	while (a_cur < amaxf)
+	{
+		int dn = 0;
+		float a_next = -1.f;
+		if (axp <= ayp && axp <= azp)
+		{
+			a_next = axn;
+			x_cur += sdx;
+			axn = (x_cur - p1.x) * invxf;
+			dn |= 1;
+		}
+		if (ayp <= axp && ayp <= azp)
+		{
+			a_next = ayn;
+			y_cur += sdy;
+			ayn = (y_cur - p1.y) * invyf;
+			dn |= 2;
+		}
+		if (azp <= axp && azp <= ayp)
+		{
+			a_next = azn;
+			z_cur += sdz;
+			azn = (z_cur - p1.z) * invzf;
+			dn |= 4;
+		}
+		if (a_next > amaxf)
+			a_next = amaxf;
+		if (a_cur >= a_next)
+		{
+			axp = axn;
+			ayp = ayn;
+			azp = azn;
+			continue;
+		}
+		if (first)
+		{
+			if constexpr (S64)
+			{
+				const double am = 0.5 * ((double)a_cur + (double)a_next);
+				vx = (int)((x1d + am * ddx + 0.5 * (double)P.lx) / (double)P.vx);
+				vy = (int)((y1d + am * ddy + 0.5 * (double)P.ly) / (double)P.vy);
+				vz = (int)((z1d + am * ddz + 0.5 * (double)P.lz) / (double)P.vz);
+			}
+			else
+			{
+				const float am = 0.5f * (a_cur + a_next);
+				vx = (int)((p1.x + am * (p2.x - p1.x) + P.lx / 2.0f) * (1.0f / P.vx));
+				vy = (int)((p1.y + am * (p2.y - p1.y) + P.ly / 2.0f) * (1.0f / P.vy));
+				vz = (int)((p1.z + am * (p2.z - p1.z) + P.lz / 2.0f) * (1.0f / P.vz));
+			}
+			if (vx < 0 || vx >= P.nx || vy < 0 || vy >= P.ny || vz < 0 ||
+			    vz >= P.nz)
+				break;
+			off = vz * P.nxy + vy * P.nx;
+			first = false;
+		}
+		else
+		{
+			if (dprev & 1)
+			{
+				vx += dir_x;
+				if (vx < 0 || vx >= P.nx)
+					break;
+			}
+			if (dprev & 2)
+			{
+				vy += dir_y;
+				if (vy < 0 || vy >= P.ny)
+					break;
+				off += dir_y * P.nx;
+			}
+			if (dprev & 4)
+			{
+				vz += dir_z;
+				if (vz < 0 || vz >= P.nz)
+					break;
+				off += dir_z * P.nxy;
+			}
+		}
+		dprev = dn;
+		float w = (a_next - a_cur) * dnorm;
+		if (TOF)
+		{
+			float xcn;
+			if constexpr (S64)
+				xcn = fmaf(0.5f * (a_cur + a_next), tA, -tB);
+			else  // TimeOfFlightHelper::getWeight (TimeOfFlight.hpp:175-185)
+				xcn = (0.5f * (a_cur * dnorm + a_next * dnorm) - tB) / tof.sigma;
+			w *= gaussW(xcn, tof.norm);
+		}
+		const int idx = off + vx;
+		if (MODE == MODE_FWD)
+			acc = fmaf(w, img[idx], acc);
+		else if (MODE == MODE_BACK)
+			unsafeAtomicAdd(img + idx, val * w);
+		else
+			cnt++;
+		a_cur = a_next;
+		axp = axn;
+		ayp = ayn;
+		azp = azn;
+	}
+	return acc;
+}
+
+// Distance-driven ray (ProjectorDD::dd_project_ref, ProjectorDD.cpp:260-578).
+// d1/d2: endpoints with the image offset subtracted.  Footprint arithmetic
+// keeps the reference's FP32 order without contraction: the voxel range of
+// the PSF-extended footprint is a rounding decision (rintf) whose outcome
+// must not change with FMA fusion.
+template <int MODE, bool TOF, bool S64>
+__device__ float ddRay(const DImage& P, float* __restrict__ img, float3 d1,
+                       float3 d2, float3 n1, float3 n2, float val,
+                       const DProj& pr, float tofv, uint32_t& cnt)
+{
+#pragma clang fp contract(off)
+	const float d12x = d1.x - d2.x, d12y = d1.y - d2.y, d12z = d1.z - d2.z;
+	const bool flag_y = fabsf(d12y) > fabsf(d12x);
+	const float d_norm = sqrtf(d12x * d12x + d12y * d12y + d12z * d12z);
+	const bool usePsf = pr.psf.k != nullptr;
+	const float* pk = nullptr;
+	float ext = 0.f;
+	if (usePsf)
+	{
+		// ProjectionPsfManager::getKernel (ProjectionPsfManager.cpp:63-91)
+		float npx = d2.y - d1.y;
+		float npy = d1.x - d2.x;
+		const float nn = sqrtf(npx * npx + npy * npy);
+		int s = 0;
+		if (nn != 0.f)
+		{
+			npx /= nn;
+			npy /= nn;
+			const float sv = fabsf(d1.x * npx + d1.y * npy);
+			s = (int)floorf(sv / pr.psf.sStep);
+			if (s > pr.psf.numS - 1)
+				s = pr.psf.numS - 1;
+		}
+		pk = (MODE == MODE_FWD || MODE == MODE_COUNT ? pr.psf.k : pr.psf.kflip) +
+		     s * pr.psf.K;
+		ext = pr.psf.halfWidth;
+	}
+	const float dx = P.vx, dy = P.vy, dz = P.vz;
+	int x_i_0, y_i_0, x_i_1, y_i_1;
+	if constexpr (S64)
+	{
+		const double x1d = d1.x, y1d = d1.y, z1d = d1.z;
+		const double ddx = (double)d2.x - x1d, ddy = (double)d2.y - y1d,
+		             ddz = (double)d2.z - z1d;
+		const double rx = 0.5 * ((double)P.lx - dx), ry = 0.5 * ((double)P.ly - dy),
+		             rz = 0.5 * ((double)P.lz - dz);
+		double axmin, axmax, aymin, aymax, azmin, azmax;
+		getAlpha<double>(-rx, rx, x1d, d2.x, (d1.x == d2.x) ? 0.0 : 1.0 / ddx, axmin, axmax);
+		getAlpha<double>(-ry, ry, y1d, d2.y, (d1.y == d2.y) ? 0.0 : 1.0 / ddy, aymin, aymax);
+		getAlpha<double>(-rz, rz, z1d, d2.z, (d1.z == d2.z) ? 0.0 : 1.0 / ddz, azmin, azmax);
+		// an empty range is not an early exit: the reference still sweeps the
+		// clamped slice range (ProjectorDD.cpp:319-336,447)
+		double amin = fmax(fmax(0.0, axmin), fmax(aymin, azmin));
+		double amax = fmin(fmin(1.0, axmax), fmin(aymax, azmax));
+		if (TOF && pr.tof.trunc > 0.f)
+		{
+			const double tmm = (double)tofMm(tofv);
+			const double dn = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+			amin = fmax(amin, 0.5 + (tmm - (double)pr.tof.trunc) / dn);
+			amax = fmin(amax, 0.5 + (tmm + (double)pr.tof.trunc) / dn);
+		}
+		x_i_0 = (int)floor((x1d + amin * ddx) / dx + 0.5 * (double)(P.nx - 1) + 0.5);
+		y_i_0 = (int)floor((y1d + amin * ddy) / dy + 0.5 * (double)(P.ny - 1) + 0.5);
+		x_i_1 = (int)floor((x1d + amax * ddx) / dx + 0.5 * (double)(P.nx - 1) + 0.5);
+		y_i_1 = (int)floor((y1d + amax * ddy) / dy + 0.5 * (double)(P.ny - 1) + 0.5);
+	}
+	else
+	{
+		// reference FP32 setup (ProjectorDD.cpp:305-336)
+		const float ix = (d1.x == d2.x) ? 0.0f : 1.0f / (d2.x - d1.x);
+		const float iy = (d1.y == d2.y) ? 0.0f : 1.0f / (d2.y - d1.y);
+		const float iz = (d1.z == d2.z) ? 0.0f : 1.0f / (d2.z - d1.z);
+		float axmin, axmax, aymin, aymax, azmin, azmax;
+		getAlpha<float>(-0.5f * (P.lx - dx), 0.5f * (P.lx - dx), d1.x, d2.x, ix, axmin, axmax);
+		getAlpha<float>(-0.5f * (P.ly - dy), 0.5f * (P.ly - dy), d1.y, d2.y, iy, aymin, aymax);
+		getAlpha<float>(-0.5f * (P.lz - dz), 0.5f * (P.lz - dz), d1.z, d2.z, iz, azmin, azmax);
+		// an empty range is not an early exit (see above)
+		float amin = fmaxf(fmaxf(0.0f, axmin), fmaxf(aymin, azmin));
+		float amax = fminf(fminf(1.0f, axmax), fminf(aymax, azmax));
+		if (TOF && pr.tof.trunc > 0.f)
+		{
+			const float tmm = tofMm(tofv);
+			const float lo = (float)fmax(0.0, 0.5 + (double)((tmm - pr.tof.trunc) / d_norm));
+			const float hi = (float)fmin(1.0, 0.5 + (double)((tmm + pr.tof.trunc) / d_norm));
+			amin = fmaxf(amin, lo);
+			amax = fminf(amax, hi);
+		}
+		const float x_0 = d1.x + amin * (d2.x - d1.x);
+		const float y_0 = d1.y + amin * (d2.y - d1.y);
+		const float x_1 = d1.x + amax * (d2.x - d1.x);
+		const float y_1 = d1.y + amax * (d2.y - d1.y);
+		x_i_0 = (int)floorf(x_0 / dx + 0.5f * (float)(P.nx - 1) + 0.5f);
+		y_i_0 = (int)floorf(y_0 / dy + 0.5f * (float)(P.ny - 1) + 0.5f);
+		x_i_1 = (int)floorf(x_1 / dx + 0.5f * (float)(P.nx - 1) + 0.5f);
+		y_i_1 = (int)floorf(y_1 / dy + 0.5f * (float)(P.ny - 1) + 0.5f);
+	}
+	// ---- FP32 footprint (338-577) ----
+	float d1_i, d2_i, n1_i, n2_i;
+	if (flag_y)
+	{
+		d1_i = d1.y;
+		d2_i = d2.y;
+		n1_i = n1.y;
+		n2_i = n2.y;
+	}
+	else
+	{
+		d1_i = d1.x;
+		d2_i = d2.x;
+		n1_i = n1.x;
+		n2_i = n2.x;
+	}
+	const float n1xy2 = n1.x * n1.x + n1.y * n1.y;
+	const float n1xy = sqrtf(n1xy2);
+	const float n1px = n1.y / n1xy, n1py = -n1.x / n1xy;
+	const float n1zn = sqrtf((n1.x * n1.z) * (n1.x * n1.z) +
+	                         (n1.y * n1.z) * (n1.y * n1.z) + n1xy2);
+	const float n1pi = (n1_i * n1.z) / n1zn, n1pz = -n1xy2 / n1zn;
+	const float n2xy2 = n2.x * n2.x + n2.y * n2.y;
+	const float n2xy = sqrtf(n2xy2);
+	const float n2px = n2.y / n2xy, n2py = -n2.x / n2xy;
+	const float n2zn = sqrtf((n2.x * n2.z) * (n2.x * n2.z) +
+	                         (n2.y * n2.z) * (n2.y * n2.z) + n2xy2);
+	const float n2pi = (n2_i * n2.z) / n2zn, n2pz = -n2xy2 / n2zn;
+	const float htt = pr.tt * 0.5f;
+	const float d1xyloX = d1.x - htt * n1px, d1xyloY = d1.y - htt * n1py;
+	const float d1xyhiX = d1.x + htt * n1px, d1xyhiY = d1.y + htt * n1py;
+	const float d2xyloX = d2.x - htt * n2px, d2xyloY = d2.y - htt * n2py;
+	const float d2xyhiX = d2.x + htt * n2px, d2xyhiY = d2.y + htt * n2py;
+	const float htz = pr.tz * 0.5f;
+	const float d1zloI = d1_i - htz * n1pi, d1zloZ = d1.z - htz * n1pz;
+	const float d1zhiI = d1_i + htz * n1pi, d1zhiZ = d1.z + htz * n1pz;
+	const float d2zloI = d2_i - htz * n2pi, d2zloZ = d2.z - htz * n2pz;
+	const float d2zhiI = d2_i + htz * n2pi, d2zhiZ = d2.z + htz * n2pz;
+	int xy_i_0, xy_i_1, nyx;
+	float lxy, lyx, dxy, dyx;
+	float d1_xy_lo, d1_xy_hi, d2_xy_lo, d2_xy_hi;
+	float d1_yx_lo, d1_yx_hi, d2_yx_lo, d2_yx_hi;
+	int strideXY, strideYX;
+	if (flag_y)
+	{
+		xy_i_0 = max(0, min(y_i_0, y_i_1));
+		xy_i_1 = min(P.ny - 1, max(y_i_0, y_i_1));
+		lxy = P.ly;
+		dxy = P.vy;
+		lyx = P.lx;
+		dyx = P.vx;
+		nyx = P.nx;
+		d1_xy_lo = d1xyloY;
+		d1_xy_hi = d1xyhiY;
+		d2_xy_lo = d2xyloY;
+		d2_xy_hi = d2xyhiY;
+		d1_yx_lo = d1xyloX;
+		d1_yx_hi = d1xyhiX;
+		d2_yx_lo = d2xyloX;
+		d2_yx_hi = d2xyhiX;
+		strideXY = P.nx;
+		strideYX = 1;
+	}
+	else
+	{
+		xy_i_0 = max(0, min(x_i_0, x_i_1));
+		xy_i_1 = min(P.nx - 1, max(x_i_0, x_i_1));
+		lxy = P.lx;
+		dxy = P.vx;
+		lyx = P.ly;
+		dyx = P.vy;
+		nyx = P.ny;
+		d1_xy_lo = d1xyloX;
+		d1_xy_hi = d1xyhiX;
+		d2_xy_lo = d2xyloX;
+		d2_xy_hi = d2xyhiX;
+		d1_yx_lo = d1xyloY;
+		d1_yx_hi = d1xyhiY;
+		d2_yx_lo = d2xyloY;
+		d2_yx_hi = d2xyhiY;
+		strideXY = 1;
+		strideYX = P.nx;
+	}
+	const float dxy_cos_theta =
+	    (d1_i != d2_i) ? dxy / (fabsf(d1_i - d2_i) / d_norm) : dxy;
+	const float yx_off = (float)(nyx - 1) / 2.f;
+	const float inv_dyx = 1.0f / dyx;
+	const float z_off = (float)(P.nz - 1) * 0.5f;
+	const float inv_dz = 1.0f / dz;
+	const float half_dyx = dyx * 0.5f;
+	const float hvz = P.vz * 0.5f;
+	float pc = 0.f;
+	if (TOF)
+		pc = (float)(0.5 * (double)d_norm + (double)tofMm(tofv));
+	float acc = 0.f;
+	for (int xyi = xy_i_0; xyi <= xy_i_1; xyi++)
+	{
+		const float pix_xy = -0.5f * lxy + ((float)xyi + 0.5f) * dxy;
+		const float a_xy_lo = (pix_xy - d1_xy_lo) / (d2_xy_hi - d1_xy_lo);
+		const float a_xy_hi = (pix_xy - d1_xy_hi) / (d2_xy_lo - d1_xy_hi);
+		const float a_z_lo = (pix_xy - d1zloI) / (d2zloI - d1zloI);
+		const float a_z_hi = (pix_xy - d1zhiI) / (d2zhiI - d1zhiI);
+		float r0 = d1_yx_lo + a_xy_lo * (d2_yx_hi - d1_yx_lo);
+		float r1 = d1_yx_hi + a_xy_hi * (d2_yx_lo - d1_yx_hi);
+		if (r0 > r1)
+		{
+			const float t = r0;
+			r0 = r1;
+			r1 = t;
+		}
+		const float widthFrac_yx = r1 - r0;
+		const float r0ov = r0, r1ov = r1;
+		r0 -= ext;
+		r1 += ext;
+		const int yx0 = max(0, (int)rintf(r0 * inv_dyx + yx_off));
+		const int yx1 = min(nyx - 1, (int)rintf(r1 * inv_dyx + yx_off));
+		float z0r = d1zloZ + a_z_lo * (d2zloZ - d1zloZ);
+		float z1r = d1zhiZ + a_z_hi * (d2zhiZ - d1zhiZ);
+		if (z0r > z1r)
+		{
+			const float t = z0r;
+			z0r = z1r;
+			z1r = t;
+		}
+		const float widthFrac_z = z1r - z0r;
+		const int zi0 = max(0, (int)rintf(z0r * inv_dz + z_off));
+		const int zi1 = min(P.nz - 1, (int)rintf(z1r * inv_dz + z_off));
+		float tw = 1.f;
+		if (TOF)
+		{
+			// getWeight at the slab's [a_lo, a_hi] (ProjectorDD.cpp:507-515)
+			const float alo = (pix_xy - d1_i - 0.5f * dxy) / (d2_i - d1_i);
+			const float ahi = (pix_xy - d1_i + 0.5f * dxy) / (d2_i - d1_i);
+			const float xcn = (0.5f * (alo * d_norm + ahi * d_norm) - pc) / pr.tof.sigma;
+			tw = gaussW(xcn, pr.tof.norm);
+		}
+		for (int yxi = yx0; yxi <= yx1; yxi++)
+		{
+			const float pix_yx = -0.5f * lyx + ((float)yxi + 0.5f) * dyx;
+			const float p0 = pix_yx - half_dyx;
+			const float p1 = pix_yx + half_dyx;
+			if (!(r1 >= p0 && r0 < p1))
+				continue;
+			const float wxy =
+			    usePsf ? psfWeight(pr.psf, pk, p0 - r1ov, p1 - r0ov)
+			           : (fminf(p1, r1ov) - fmaxf(p0, r0ov));
+			const float wxy_s = wxy / widthFrac_yx;
+			for (int zi = zi0; zi <= zi1; zi++)
+			{
+				const float pix_z = -0.5f * P.lz + ((float)zi + 0.5f) * P.vz;
+				const float zp0 = pix_z - hvz;
+				const float zp1 = pix_z + hvz;
+				if (!(z1r >= zp0 && z0r < zp1))
+					continue;
+				const float wz = fminf(zp1, z1r) - fmaxf(zp0, z0r);
+				const float wz_s = wz / widthFrac_z;
+				const int io = zi * P.nxy + xyi * strideXY + yxi * strideYX;
+				float w = wxy_s * wz_s * dxy_cos_theta;
+				if (TOF)
+					w *= tw;
+				if (MODE == MODE_FWD)
+					acc = fmaf(w, img[io], acc);
+				else if (MODE == MODE_BACK)
+					unsafeAtomicAdd(img + io, val * w);
+				else
+					cnt++;
+			}
+		}
+	}
+	return acc;
+}
+
+}  // namespace yrtmi

@@ -1,0 +1,66 @@
+"""Build the yrtmi native library in-tree with hipcc for gfx950.
+
+    python -m yrtmi.build          (from yrt-pet_amd/)
+
+Produces ``yrtmi/libyrtmi.so`` (HIP kernels + C ABI).  The library is always
+rebuilt when this runs; loaders never build on their own.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG_ROOT = os.path.dirname(HERE)            # yrt-pet_amd/
+REPO = os.path.dirname(PKG_ROOT)
+CSRC = os.path.join(PKG_ROOT, "csrc")
+INCLUDE = os.path.join(REPO, "include")
+ARCH = "gfx950"
+
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
+    "-munsafe-fp-atomics",
+    # packed FP32 instructions are disabled on this pool
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+]
+
+LIB_SOURCES = ["yrtmi_kernels.hip", "yrtmi_capi.hip"]
+
+
+def hipcc():
+    p = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not os.path.exists(p):
+        raise RuntimeError("hipcc not found")
+    return p
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, cwd=CSRC, stdout=subprocess.PIPE,
+                       stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l)
+    if r.returncode != 0:
+        raise RuntimeError(f"build failed ({' '.join(cmd[:3])} ...):\n{out}")
+    return out
+
+
+def build_library(verbose=False):
+    out = os.path.join(HERE, "libyrtmi.so")
+    tmp = out + ".tmp"
+    cmd = [hipcc(), *HIP_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", "-shared",
+           "-o", tmp, *LIB_SOURCES]
+    log = _run(cmd)
+    os.replace(tmp, out)
+    if verbose and log.strip():
+        print(log)
+    return out
+
+
+def main():
+    print(build_library(verbose=True))
+
+
+if __name__ == "__main__":
+    sys.exit(main())
