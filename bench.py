@@ -1,0 +1,233 @@
+#!/usr/bin/env python3
+"""Benchmark: list-mode TOF OSEM with 4-ray Siddon on one MI355X.
+
+Workload (the 64-ring x 512-crystal scanner configuration): scanner radius
+400 mm, 3.2 x 3.2 x 20 mm crystals, 100 M list-mode events (uint32 d1, d2 +
+float32 TOF) drawn from a seeded phantom (water cylinder r = 150 mm, length
+180 mm, plus 6 spheres of radius 4-14 mm at 5:1 contrast), TOF 250 ps FWHM
+truncated at 5 sigma, image 200 x 200 x 128 at 2 mm, 10 subsets.
+
+One step = one full OSEM iteration: for each of the 10 subsets, the fused
+EM kernel (forward projection, ratio, backprojection) over that subset's
+events followed by the multiplicative image update.  Events, the detector
+LUT and all images are resident in HBM before timing starts; the
+sensitivity image (backprojection of all 536.9 M scanner LORs x 4 rays) is
+generated once in setup and timed separately.
+
+Prints one JSON line (see the `out` dict at the end).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [REPO, os.path.join(REPO, "yrt-pet_amd")]
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--events", type=int, default=100_000_000)
+    ap.add_argument("--subsets", type=int, default=10)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--lpe", type=int, default=4, help="lanes per event")
+    ap.add_argument("--block", type=int, default=256)
+    ap.add_argument("--grid", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--quiet", action="store_true")
+    return ap.parse_args()
+
+
+def log(args, *a):
+    if not args.quiet:
+        print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU: --gpus must be 1 "
+                                   "(multi-GPU sharding is described in DESIGN.md, not built)"}))
+        return 2
+    import torch
+    import yrtmi
+    from yrtmi import synth
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    t_setup = time.time()
+    sc = yrtmi.Scanner("cfg2", 64 * 3.2, 3.2, 3.2, 20.0, 400.0, 512, 64)
+    im = yrtmi.ImageParams(200, 200, 128, 2.0, 2.0, 2.0)
+    rng = np.random.default_rng(args.seed)
+    spheres = synth.random_spheres_in_cylinder(rng, 150.0, 180.0,
+                                               list(np.linspace(4.0, 14.0, 6)), 5.0)
+    act = synth.cylinder_spheres(im, 150.0, 180.0, 1.0, spheres, supersample=2)
+    log(args, f"phantom ready ({time.time() - t_setup:.1f}s)")
+    ev = synth.sample_events(sc, im, act, args.events, seed=args.seed,
+                             tof_fwhm_ps=250.0, device=dev)
+    torch.cuda.synchronize()
+    log(args, f"{args.events} events drawn ({time.time() - t_setup:.1f}s)")
+    # subset-major permutation: event i belongs to subset i mod S
+    # (BinIteratorChronologicalInterleaved); each subset becomes contiguous
+    S = args.subsets
+    n = args.events
+    perm = torch.cat([torch.arange(s, n, S, device=dev) for s in range(S)])
+    d1 = ev["d1"][perm].contiguous()
+    d2 = ev["d2"][perm].contiguous()
+    tof = ev["tof"][perm].contiguous()
+    del ev, perm
+    bounds = [0]
+    for s in range(S):
+        bounds.append(bounds[-1] + len(range(s, n, S)))
+
+    ctx = yrtmi.Context(sc)
+    pp = yrtmi.ProjectorParams(sc, numRays=4)
+    pp.addTOF(250.0, 5)
+    proj = ctx.projector(pp)
+    proj.set_launch(args.lpe, args.block, args.grid)
+    # sensitivity image: all scanner LORs (uniform histogram), no TOF
+    psens = ctx.projector(pp, with_tof=False)
+    psens.set_launch(args.lpe, args.block, args.grid)
+    sens = torch.zeros(im.shape, device=dev)
+    torch.cuda.synchronize()
+    t0 = time.time()
+    SS = 16
+    for s in range(SS):
+        ctx.sensitivity(psens, im, sens, yrtmi.LorSet.histo_subset(ctx, SS, s))
+    torch.cuda.synchronize()
+    sens_s = time.time() - t0
+    n_lors = sum(yrtmi.LorSet.histo_subset(ctx, SS, s).n for s in range(SS))
+    log(args, f"sensitivity: {n_lors} LORs x 4 rays in {sens_s:.2f}s")
+    # hard threshold (voxels <= 1 -> 0), list-mode scaling 1/S, mask, init
+    ctx.threshold(sens, sens, im.nvox, 1.0, 0.0, 0.0, 1.0, 0.0)
+    sens *= float(np.float32(1.0) / np.float32(S))
+    x = torch.full(im.shape, 0.125, device=dev)
+    ctx.threshold(x, sens, im.nvox, 0.0, 0.0, 0.0, 1.0, 0.0)
+    u = torch.zeros(im.shape, device=dev)
+    sub = [yrtmi.LorSet.pairs(d1[bounds[s]:bounds[s + 1]], d2[bounds[s]:bounds[s + 1]],
+                              tof=tof[bounds[s]:bounds[s + 1]]) for s in range(S)]
+    # algorithmic work: voxel segments per subset (instrumented count)
+    segs = [ctx.count_segments(proj, im, L) for L in sub]
+    log(args, f"segments/event {sum(segs) / n:.1f}; setup {time.time() - t_setup:.1f}s")
+
+    def iteration(evs=None):
+        for s in range(S):
+            ctx.fill(u, 0.0)
+            if evs is not None:
+                evs[s][0].record()
+            ctx.em_subset(proj, im, x, u, sub[s])
+            if evs is not None:
+                evs[s][1].record()
+            ctx.em_update(x, u, sens, im.nvox)
+
+    for _ in range(args.warmup):
+        iteration()
+    torch.cuda.synchronize()
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for _ in range(S)]
+        iteration(evs)
+        kernel_ms.append(evs)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    per_launch = []
+    for evs in kernel_ms:
+        for s, (a, b) in enumerate(evs):
+            per_launch.append((s, a.elapsed_time(b)))
+    s_per_iter = wall / args.steps
+    ev_proj_per_s = 2.0 * n * args.steps / wall
+    # roofline of the fused EM kernel: R_ev = 12 B/event (d1, d2, tof) +
+    # 12 B per voxel segment (4 B forward read + 8 B atomic read-modify-write)
+    alg_bytes = [12.0 * (bounds[s + 1] - bounds[s]) + 12.0 * segs[s] for s in range(S)]
+    avg_ms = float(np.mean([ms for _, ms in per_launch]))
+    avg_bytes = float(np.mean([alg_bytes[s] for s, _ in per_launch]))
+    achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
+    log(args, f"{s_per_iter:.3f} s/iter, EM kernel avg {avg_ms:.2f} ms, {achieved:.0f} GB/s alg")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(args, sc, im, pp, d1, d2, tof, bounds, x)
+    out = {
+        "metric": "fwd+back event-projections/s, s/OSEM iter",
+        "value": ev_proj_per_s,
+        "unit": "event-projections/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": s_per_iter * 1e3,
+        "s_per_osem_iter": s_per_iter,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded phantom, physically sampled TOF list-mode events)",
+        "config": {"workload": "list-mode TOF OSEM, 4-ray Siddon, 64x512 scanner R400, "
+                               "100M events, 200x200x128 @2mm, 10 subsets",
+                   "events": n, "subsets": S, "rays": 4, "tof_fwhm_ps": 250.0,
+                   "tof_n_std": 5, "image": [200, 200, 128], "voxel_mm": 2.0,
+                   "lanes_per_event": args.lpe, "block": args.block,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "kEvents<MODE_EM,Siddon,TOF,LPE=4>",
+                     "avg_launch_ms": avg_ms, "alg_bytes_per_launch": avg_bytes,
+                     "segments_per_event": sum(segs) / n},
+        "sensitivity_s": sens_s,
+        "sensitivity_lors": n_lors,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+def cpu_baseline(args, sc, im, pp, d1, d2, tof, bounds, x):
+    """The CPU restatement of the reference EM subset (oracle, std::thread
+    over contiguous event chunks like util::parallelForChunked) on the first
+    events of subset 0, sized to ~cpu_seconds of work."""
+    import oracle as O
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    from helpers import oracle_image, oracle_proj, oracle_scanner
+    nthreads = max(1, min(16, os.cpu_count() or 1))
+    osc = oracle_scanner(sc)
+    pos, ori = O.lut(osc)
+    po = oracle_proj(pp)
+    oi = oracle_image(im)
+    xh = x.cpu().numpy()
+
+    def run(k):
+        a, b, t = (d1[:k].cpu().numpy().astype(np.uint32), d2[:k].cpu().numpy().astype(np.uint32),
+                   tof[:k].cpu().numpy())
+        L = O.lors_from_pairs(pos, ori, a, b, tof=t)
+        uh = np.zeros(im.shape, np.float32)
+        t0 = time.perf_counter()
+        O.em_subset(oi, xh, uh, po, L, nthreads=nthreads)
+        return time.perf_counter() - t0
+
+    k0 = 20000
+    dt0 = run(k0)
+    k = int(min(bounds[1], max(k0, k0 / dt0 * args.cpu_seconds)))
+    dt = run(k)
+    rate = 2.0 * k / dt
+    log(args, f"cpu baseline: {k} events in {dt:.1f}s on {nthreads} threads")
+    return {"value": rate, "unit": "event-projections/s", "cores": nthreads, "kind": "port",
+            "sample": f"EM subset (forward+backprojection) of the first {k} events of subset 0, "
+                      f"oracle FP32 restatement, {nthreads} std::threads",
+            "seconds": dt}
+
+
+if __name__ == "__main__":
+    sys.exit(main())
