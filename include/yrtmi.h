@@ -170,6 +170,16 @@ int yrtmi_proj_destroy(yrtmi_proj* p);
  * rays), threads per block, and blocks (0 = sized from the event count). */
 int yrtmi_proj_set_launch(yrtmi_proj* p, int lanes_per_event,
                           int block_size, int grid_blocks);
+/* How backprojections accumulate.  YRTMI_ACCUM_XCD (default): each XCD adds
+ * into its own private copy of the target image (chosen by the hardware XCC
+ * id), and the copies are folded into the target by one streaming pass after
+ * the kernel; the context keeps the copies (n_xcc x image size) between
+ * calls.  YRTMI_ACCUM_DIRECT: FP32 atomics straight into the target (no
+ * extra memory).  Both are the reference's atomicAdd accumulation
+ * (SiddonKernels.cuh:426-428) up to summation order. */
+#define YRTMI_ACCUM_DIRECT 0
+#define YRTMI_ACCUM_XCD 1
+int yrtmi_proj_set_accumulation(yrtmi_proj* p, int mode);
 
 /* ---- projections --------------------------------------------------------
  * Forward: out[i] = (A x)_i for each LOR of `lors` (device out, n floats).

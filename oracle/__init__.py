@@ -325,6 +325,18 @@ def multiray_offsets(num_rays):
     return out
 
 
+def multiray_lines(p1, p2, n1, n2, thick_z, thick_trans, num_lines):
+    """Rays 1..num_lines of MultiRayGenerator::getRandomLine for one LOR
+    (seed 13, not parallel, no depth).  Pinned by: the reference generator
+    compiled in oracle/_ref (tests/golden/ref_vectors.npz)."""
+    f = lambda a: np.ascontiguousarray(a, np.float32)
+    out = np.zeros((num_lines, 6), np.float32)
+    lib().yo_multiray_lines(_p(f(p1)), _p(f(p2)), _p(f(n1)), _p(f(n2)),
+                            C.c_float(thick_z), C.c_float(thick_trans),
+                            int(num_lines), _p(out))
+    return out
+
+
 def apply_motion(p, xf):
     """ProjectionData::getLOR motion branch (src/datastruct/projection/
     ProjectionData.cpp:302-318) with Matrix*Vector3D (src/geometry/Matrix.cpp:

@@ -1219,6 +1219,41 @@ int yo_multiray_offsets(int num_rays, float* out)
 	return 0;
 }
 
+// Lines 1..num_lines of the multi-ray generator for one LOR (the same code
+// path as the projectors' rays 1..numRays-1), out: num_lines x 6.
+int yo_multiray_lines(const float* p1, const float* p2, const float* n1,
+                      const float* n2, float tz, float tt, int num_lines,
+                      float* out)
+{
+	std::vector<float> offs(4 * num_lines);
+	yo_multiray_offsets(num_lines + 1, offs.data());
+	yo_projector pr{};
+	pr.num_rays = num_lines + 1;
+	pr.ray_offsets = offs.data();
+	pr.crystal_size_z = tz;
+	pr.crystal_size_trans = tt;
+	yo_image im{1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 0, 0, 0, 1};
+	Projector<float> P(&im, &pr);
+	Lor<float> l{};
+	l.p1 = {p1[0], p1[1], p1[2]};
+	l.p2 = {p2[0], p2[1], p2[2]};
+	l.n1 = {n1[0], n1[1], n1[2]};
+	l.n2 = {n2[0], n2[1], n2[2]};
+	for (int r = 1; r <= num_lines; r++)
+	{
+		V3<float> q1, q2;
+		P.rayLine(l, r, q1, q2);
+		float* o = out + 6 * (r - 1);
+		o[0] = q1.x;
+		o[1] = q1.y;
+		o[2] = q1.z;
+		o[3] = q2.x;
+		o[4] = q2.y;
+		o[5] = q2.z;
+	}
+	return 0;
+}
+
 int yo_tof_helper(float width_ps, int n_std, float* sigma, float* trunc,
                   float* norm)
 {

@@ -88,6 +88,9 @@ int64_t yo_histo3d_subset_size(const yo_scanner* sc, int num_subsets,
 int yo_histo3d_subset_bins(const yo_scanner* sc, int num_subsets, int subset,
                            int64_t* bins);
 int yo_multiray_offsets(int num_rays, float* out);
+int yo_multiray_lines(const float* p1, const float* p2, const float* n1,
+                      const float* n2, float tz, float tt, int num_lines,
+                      float* out);
 
 /* ---- TOF / PSF / DD helpers (exposed for pinning) ---- */
 int yo_tof_helper(float width_ps, int n_std, float* sigma, float* trunc,

@@ -51,6 +51,11 @@ struct yrtmi_ctx
 	float* motion = nullptr;
 	int nmotion = 0;
 	unsigned long long* counter = nullptr;
+	// per-XCD partial images for backprojection (nxcc x partStride floats),
+	// kept zeroed between uses
+	float* part = nullptr;
+	int64_t partStride = 0;
+	int nxcc = 1;
 };
 
 struct yrtmi_proj
@@ -64,6 +69,7 @@ struct yrtmi_proj
 	int lpe = 1;
 	int block = 256;
 	int64_t grid = 0;
+	int accum = YRTMI_ACCUM_XCD;
 };
 
 namespace
@@ -180,6 +186,47 @@ int buildLors(yrtmi_ctx* c, const yrtmi_lors* l, DLors& d, DScanner& s)
 	return YRTMI_OK;
 }
 
+// Make sure the context holds nxcc zeroed partial images of at least n
+// floats each.  Allocates only when a larger image than before is seen.
+int ensurePartials(yrtmi_ctx* c, int64_t n)
+{
+	const int64_t stride = (n + 63) / 64 * 64;
+	if (c->part != nullptr && c->partStride >= stride)
+		return YRTMI_OK;
+	HIP_TRY(hipStreamSynchronize(c->stream));
+	HIP_TRY(hipFree(c->part));
+	c->part = nullptr;
+	c->partStride = 0;
+	HIP_TRY(hipMalloc(&c->part, sizeof(float) * stride * c->nxcc));
+	HIP_TRY(hipMemset(c->part, 0, sizeof(float) * stride * c->nxcc));
+	c->partStride = stride;
+	return YRTMI_OK;
+}
+
+// Set up per-XCD accumulation for a backprojection of n voxels into y, or
+// leave the kernel writing y directly.
+int setupAccum(yrtmi_ctx* c, const yrtmi_proj* p, KArgs& a, int64_t n)
+{
+	a.part = nullptr;
+	if (p->accum != YRTMI_ACCUM_XCD || c->nxcc <= 1)
+		return YRTMI_OK;
+	int r = ensurePartials(c, n);
+	if (r)
+		return r;
+	a.part = c->part;
+	a.partStride = c->partStride;
+	a.nxcc = c->nxcc;
+	return YRTMI_OK;
+}
+
+int foldAccum(yrtmi_ctx* c, const KArgs& a, float* y, int64_t n)
+{
+	if (a.part == nullptr)
+		return YRTMI_OK;
+	HIP_TRY(launchFoldPartials(y, a.part, a.partStride, a.nxcc, n, c->stream));
+	return YRTMI_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -203,6 +250,14 @@ int yrtmi_ctx_create(int device, void* stream, yrtmi_ctx** out)
 	hipError_t e = hipSetDevice(device);
 	if (e == hipSuccess)
 		e = hipMalloc(&c->counter, sizeof(unsigned long long));
+	if (e == hipSuccess)
+	{
+		int nx = 1;
+		if (hipDeviceGetAttribute(&nx, hipDeviceAttributeNumberOfXccs, device) ==
+		        hipSuccess &&
+		    nx >= 1 && nx <= 16)
+			c->nxcc = nx;
+	}
 	if (e != hipSuccess)
 	{
 		delete c;
@@ -222,6 +277,7 @@ int yrtmi_ctx_destroy(yrtmi_ctx* c)
 	(void)hipFree(c->zbin);
 	(void)hipFree(c->motion);
 	(void)hipFree(c->counter);
+	(void)hipFree(c->part);
 	delete c;
 	return YRTMI_OK;
 }
@@ -526,6 +582,16 @@ int yrtmi_proj_destroy(yrtmi_proj* p)
 	return YRTMI_OK;
 }
 
+int yrtmi_proj_set_accumulation(yrtmi_proj* p, int mode)
+{
+	if (p == nullptr)
+		return fail(YRTMI_EINVAL, "null projector");
+	if (mode != YRTMI_ACCUM_DIRECT && mode != YRTMI_ACCUM_XCD)
+		return fail(YRTMI_EINVAL, "unknown accumulation mode");
+	p->accum = mode;
+	return YRTMI_OK;
+}
+
 int yrtmi_proj_set_launch(yrtmi_proj* p, int lpe, int block, int grid)
 {
 	if (p == nullptr)
@@ -589,7 +655,14 @@ int yrtmi_back(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
 		return fail(YRTMI_EINVAL, "back: null image or values");
 	a.y = img;
 	a.vals = vals;
+	const int64_t ny = a.im.nvox * a.im.nt;
+	r = setupAccum(c, p, a, ny);
+	if (r)
+		return r;
 	HIP_TRY(launchEvents(MODE_BACK, a, p->lpe, p->block, p->grid, c->stream));
+	r = foldAccum(c, a, img, ny);
+	if (r)
+		return r;
 	return YRTMI_OK;
 }
 
@@ -619,7 +692,14 @@ int yrtmi_em_subset(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
 		a.c.g = corr->global_scale;
 		a.c.denomThr = corr->denom_threshold;
 	}
+	const int64_t ny = a.im.nvox * a.im.nt;
+	r = setupAccum(c, p, a, ny);
+	if (r)
+		return r;
 	HIP_TRY(launchEvents(MODE_EM, a, p->lpe, p->block, p->grid, c->stream));
+	r = foldAccum(c, a, u, ny);
+	if (r)
+		return r;
 	return YRTMI_OK;
 }
 
@@ -649,7 +729,13 @@ int yrtmi_sensitivity(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
 	a.y = img;
 	a.vals = sens;
 	a.L.tof = nullptr;
+	r = setupAccum(c, p, a, a.im.nvox);
+	if (r)
+		return r;
 	HIP_TRY(launchEvents(MODE_SENS, a, p->lpe, p->block, p->grid, c->stream));
+	r = foldAccum(c, a, img, a.im.nvox);
+	if (r)
+		return r;
 	return YRTMI_OK;
 }
 

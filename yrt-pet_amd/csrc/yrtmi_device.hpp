@@ -111,7 +111,20 @@ struct KArgs
 	DImage attIm;     // MODE_SENS attenuation image geometry
 	const float* att;  // MODE_SENS attenuation image (nullable)
 	unsigned long long* counter;  // MODE_COUNT
+	// Backprojection target: when `part` is set, each XCD accumulates into
+	// its own copy part + xcc * partStride (folded into `y` afterwards).
+	float* part;
+	int64_t partStride;
+	int32_t nxcc;
 };
+
+// Hardware id of the XCD (accelerator complex die) running this wave.
+__device__ __forceinline__ uint32_t xccId()
+{
+	uint32_t x;
+	asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(x));
+	return x & 0xfu;
+}
 
 struct Lor
 {

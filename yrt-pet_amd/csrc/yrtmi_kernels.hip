@@ -83,6 +83,16 @@ __global__ void __launch_bounds__(256) kEvents(KArgs a)
 	const int64_t total = a.L.n * LPE;
 	const int64_t stride = (int64_t)gridDim.x * blockDim.x;
 	uint32_t cnt = 0;
+	// backprojection target: this XCD's private partial image when enabled
+	float* ydst = a.y;
+	if (MODE == MODE_BACK || MODE == MODE_EM || MODE == MODE_SENS)
+	{
+		if (a.part != nullptr)
+		{
+			const int k = min((int)xccId(), a.nxcc - 1);
+			ydst = a.part + (int64_t)k * a.partStride;
+		}
+	}
 	for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
 	     t += stride)
 	{
@@ -104,7 +114,7 @@ __global__ void __launch_bounds__(256) kEvents(KArgs a)
 		else if (MODE == MODE_BACK)
 		{
 			if (l.frame >= 0)
-				backLane<TYPE, TOF, LPE, S64, MODE_BACK>(a, l, lane, a.y + fo,
+				backLane<TYPE, TOF, LPE, S64, MODE_BACK>(a, l, lane, ydst + fo,
 				                                    a.vals[e], cnt);
 		}
 		else if (MODE == MODE_COUNT)
@@ -143,7 +153,7 @@ __global__ void __launch_bounds__(256) kEvents(KArgs a)
 				if (a.c.att)
 					v *= af;
 				v *= a.c.g;
-				backLane<TYPE, TOF, LPE, S64, MODE_BACK>(a, l, lane, a.y + fo, v, cnt);
+				backLane<TYPE, TOF, LPE, S64, MODE_BACK>(a, l, lane, ydst + fo, v, cnt);
 			}
 		}
 		else if (MODE == MODE_SENS)
@@ -161,7 +171,7 @@ __global__ void __launch_bounds__(256) kEvents(KArgs a)
 				v *= expf(-mu * 0.1f);
 			}
 			if (l.frame >= 0)
-				backLane<TYPE, false, LPE, S64, MODE_BACK>(a, l, lane, a.y, v, cnt);
+				backLane<TYPE, false, LPE, S64, MODE_BACK>(a, l, lane, ydst, v, cnt);
 		}
 	}
 	if (MODE == MODE_COUNT)
@@ -353,6 +363,61 @@ hipError_t launchThreshold(float* x, const float* m, int64_t nvox, int64_t nt,
 {
 	kThreshold<<<gridFor(nvox * nt), 256, 0, st>>>(x, m, nvox, nt, mFrames,
 	                                               thr, les, leo, gts, gto);
+	return hipGetLastError();
+}
+
+// y[j] += sum_k part[k][j]; part[k][j] = 0 (ready for the next
+// backprojection).  float4 over n4 = n/4 quads, scalar tail.
+__global__ void __launch_bounds__(256) kFoldPartials(float* __restrict__ y,
+                                                     float* __restrict__ part,
+                                                     int64_t stride, int nk,
+                                                     int64_t n, int vec)
+{
+	const int64_t n4 = vec ? (n >> 2) : 0;
+	const int64_t gs = (int64_t)gridDim.x * blockDim.x;
+	const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+	for (int64_t i = t0; i < n4; i += gs)
+	{
+		float4 s = z;
+		for (int k = 0; k < nk; k++)
+		{
+			float4* p = reinterpret_cast<float4*>(part + k * stride) + i;
+			const float4 v = *p;
+			s.x += v.x;
+			s.y += v.y;
+			s.z += v.z;
+			s.w += v.w;
+			*p = z;
+		}
+		float4* q = reinterpret_cast<float4*>(y) + i;
+		float4 o = *q;
+		o.x += s.x;
+		o.y += s.y;
+		o.z += s.z;
+		o.w += s.w;
+		*q = o;
+	}
+	for (int64_t i = 4 * n4 + t0; i < n; i += gs)
+	{
+		float s = 0.f;
+		for (int k = 0; k < nk; k++)
+		{
+			s += part[k * stride + i];
+			part[k * stride + i] = 0.f;
+		}
+		y[i] += s;
+	}
+}
+
+hipError_t launchFoldPartials(float* y, float* part, int64_t stride, int nk,
+                              int64_t n, hipStream_t st)
+{
+	// float4 path needs 16-byte aligned rows (stride is a multiple of 64)
+	const int vec = ((uintptr_t)y % 16 == 0 && (uintptr_t)part % 16 == 0 &&
+	                 stride % 4 == 0) ? 1 : 0;
+	kFoldPartials<<<gridFor(vec ? (n + 3) / 4 : n), 256, 0, st>>>(y, part, stride,
+	                                                               nk, n, vec);
 	return hipGetLastError();
 }
 

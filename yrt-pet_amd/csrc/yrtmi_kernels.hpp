@@ -19,5 +19,9 @@ hipError_t launchThreshold(float* x, const float* m, int64_t nvox, int64_t nt,
                            int64_t mFrames, float thr, float les, float leo,
                            float gts, float gto, hipStream_t st);
 hipError_t launchFill(float* x, int64_t n, float v, hipStream_t st);
+// y += sum of the nk per-XCD partial images (row stride `stride`), which are
+// left zeroed.
+hipError_t launchFoldPartials(float* y, float* part, int64_t stride, int nk,
+                              int64_t n, hipStream_t st);
 
 }  // namespace yrtmi

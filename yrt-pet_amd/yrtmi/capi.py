@@ -89,6 +89,7 @@ _SIGS = {
     "yrtmi_proj_create": (C.c_int, [_P, C.POINTER(ProjectorDesc), C.POINTER(_P)]),
     "yrtmi_proj_destroy": (C.c_int, [_P]),
     "yrtmi_proj_set_launch": (C.c_int, [_P, C.c_int, C.c_int, C.c_int]),
+    "yrtmi_proj_set_accumulation": (C.c_int, [_P, C.c_int]),
     "yrtmi_forward": (C.c_int, [_P, _P, C.POINTER(Image), _P, C.POINTER(Lors), _P]),
     "yrtmi_back": (C.c_int, [_P, _P, C.POINTER(Image), _P, C.POINTER(Lors), _P]),
     "yrtmi_em_subset": (C.c_int, [_P, _P, C.POINTER(Image), _P, _P,

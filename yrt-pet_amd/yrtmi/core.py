@@ -264,6 +264,11 @@ class Projector:
             lanes_per_event = 4 if r >= 4 else (2 if r >= 2 else 1)
         capi.check(capi.lib().yrtmi_proj_set_launch(self.h, lanes_per_event, block, grid))
 
+    def set_accumulation(self, per_xcd=True):
+        """Backprojection accumulation: per-XCD private images folded after
+        the kernel (default) or direct atomics into the target."""
+        capi.check(capi.lib().yrtmi_proj_set_accumulation(self.h, 1 if per_xcd else 0))
+
     def __del__(self):
         try:
             if getattr(self, "h", None):
