@@ -46,6 +46,10 @@ def parse():
     ap.add_argument("--grid", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--accum", choices=("xcd", "direct"), default="xcd",
+                    help="backprojection accumulation: per-XCD images or direct atomics")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="run one 256 MiB device copy first (PMC calibration)")
     ap.add_argument("--quiet", action="store_true")
     return ap.parse_args()
 
@@ -67,6 +71,13 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.calib_copy:
+        # a plain copy of known size: 256 MiB read + 256 MiB written
+        a = torch.ones(64 << 20, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     t_setup = time.time()
     sc = yrtmi.Scanner("cfg2", 64 * 3.2, 3.2, 3.2, 20.0, 400.0, 512, 64)
     im = yrtmi.ImageParams(200, 200, 128, 2.0, 2.0, 2.0)
@@ -97,9 +108,11 @@ def main():
     pp.addTOF(250.0, 5)
     proj = ctx.projector(pp)
     proj.set_launch(args.lpe, args.block, args.grid)
+    proj.set_accumulation(args.accum == "xcd")
     # sensitivity image: all scanner LORs (uniform histogram), no TOF
     psens = ctx.projector(pp, with_tof=False)
     psens.set_launch(args.lpe, args.block, args.grid)
+    psens.set_accumulation(args.accum == "xcd")
     sens = torch.zeros(im.shape, device=dev)
     torch.cuda.synchronize()
     t0 = time.time()
@@ -111,7 +124,7 @@ def main():
     n_lors = sum(yrtmi.LorSet.histo_subset(ctx, SS, s).n for s in range(SS))
     log(args, f"sensitivity: {n_lors} LORs x 4 rays in {sens_s:.2f}s")
     # hard threshold (voxels <= 1 -> 0), list-mode scaling 1/S, mask, init
-    ctx.threshold(sens, sens, im.nvox, 1.0, 0.0, 0.0, 1.0, 0.0)
+    ctx.threshold(sens, sens.clone(), im.nvox, 1.0, 0.0, 0.0, 1.0, 0.0)
     sens *= float(np.float32(1.0) / np.float32(S))
     x = torch.full(im.shape, 0.125, device=dev)
     ctx.threshold(x, sens, im.nvox, 0.0, 0.0, 0.0, 1.0, 0.0)

@@ -241,8 +241,8 @@ def rigid_transforms(rng, k, max_rot_deg=3.0, max_trans_mm=5.0):
         ang = math.radians(rng.uniform(-max_rot_deg, max_rot_deg))
         K = np.array([[0, -ax[2], ax[1]], [ax[2], 0, -ax[0]], [-ax[1], ax[0], 0]])
         R = np.eye(3) + math.sin(ang) * K + (1 - math.cos(ang)) * K @ K
-        t = rng.uniform(-1, 1, 3)
-        t *= max_trans_mm / max(1.0, np.linalg.norm(t)) if np.linalg.norm(t) > 1 else max_trans_mm
+        t = rng.normal(size=3)
+        t *= rng.uniform(0, max_trans_mm) / np.linalg.norm(t)  # |t| <= max_trans_mm
         out[i] = np.array([R[0, 0], R[0, 1], R[0, 2], t[0], R[1, 0], R[1, 1], R[1, 2], t[1],
                            R[2, 0], R[2, 1], R[2, 2], t[2]], np.float32)
     return out
