@@ -7,12 +7,16 @@ float32 TOF) drawn from a seeded phantom (water cylinder r = 150 mm, length
 180 mm, plus 6 spheres of radius 4-14 mm at 5:1 contrast), TOF 250 ps FWHM
 truncated at 5 sigma, image 200 x 200 x 128 at 2 mm, 10 subsets.
 
-One step = one full OSEM iteration: for each of the 10 subsets, the fused
-EM kernel (forward projection, ratio, backprojection) over that subset's
-events followed by the multiplicative image update.  Events, the detector
-LUT and all images are resident in HBM before timing starts; the
-sensitivity image (backprojection of all 536.9 M scanner LORs x 4 rays) is
-generated once in setup and timed separately.
+One step = one full OSEM iteration: for each of the 10 subsets, the EM
+update image of that subset's events (forward projection and ratio per
+event, then backprojection) followed by the multiplicative image update.
+Default (--accum tiled): the backprojection goes through per-subset tile
+plans (the Siddon walks binned by 32x32x16 image tiles, accumulated in LDS),
+built once in setup; --accum xcd / direct run the fused one-kernel EM with
+global atomics instead.  Events, plans, the detector LUT and all images are
+resident in HBM before timing starts; the sensitivity image
+(backprojection of all 536.9 M scanner LORs x 4 rays) and the plans are
+made once in setup and timed separately.
 
 Prints one JSON line (see the `out` dict at the end).
 """
@@ -46,8 +50,10 @@ def parse():
     ap.add_argument("--grid", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--accum", choices=("xcd", "direct"), default="xcd",
-                    help="backprojection accumulation: per-XCD images or direct atomics")
+    ap.add_argument("--accum", choices=("tiled", "xcd", "direct"), default="tiled",
+                    help="backprojection: tile plans with LDS accumulation, or global "
+                         "atomics into per-XCD images / the target directly")
+    ap.add_argument("--chunk", type=int, default=0, help="tile plan records per workgroup")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one 256 MiB device copy first (PMC calibration)")
     ap.add_argument("--quiet", action="store_true")
@@ -107,6 +113,7 @@ def main():
     pp = yrtmi.ProjectorParams(sc, numRays=4)
     pp.addTOF(250.0, 5)
     proj = ctx.projector(pp)
+    tiled = args.accum == "tiled"
     proj.set_launch(args.lpe, args.block, args.grid)
     proj.set_accumulation(args.accum == "xcd")
     # sensitivity image: all scanner LORs (uniform histogram), no TOF
@@ -118,7 +125,13 @@ def main():
     t0 = time.time()
     SS = 16
     for s in range(SS):
-        ctx.sensitivity(psens, im, sens, yrtmi.LorSet.histo_subset(ctx, SS, s))
+        Ls = yrtmi.LorSet.histo_subset(ctx, SS, s)
+        if tiled:
+            pl = ctx.plan(psens, im, Ls, args.chunk)
+            ctx.sensitivity(psens, im, sens, Ls, plan=pl)
+            del pl
+        else:
+            ctx.sensitivity(psens, im, sens, Ls)
     torch.cuda.synchronize()
     sens_s = time.time() - t0
     n_lors = sum(yrtmi.LorSet.histo_subset(ctx, SS, s).n for s in range(SS))
@@ -134,15 +147,35 @@ def main():
     # algorithmic work: voxel segments per subset (instrumented count)
     segs = [ctx.count_segments(proj, im, L) for L in sub]
     log(args, f"segments/event {sum(segs) / n:.1f}; setup {time.time() - t_setup:.1f}s")
+    plans, plan_s, plan_bytes, plan_records = [], 0.0, 0, 0
+    vbuf = None
+    if tiled:
+        torch.cuda.synchronize()
+        t0 = time.time()
+        plans = [ctx.plan(proj, im, L, args.chunk) for L in sub]
+        torch.cuda.synchronize()
+        plan_s = time.time() - t0
+        infos = [p_.info() for p_ in plans]
+        plan_bytes = sum(i["bytes"] for i in infos)
+        plan_records = sum(i["records"] for i in infos)
+        vbuf = torch.empty(max(L.n for L in sub), device=dev)
+        log(args, f"tile plans: {plan_records} records ({plan_records / (4 * n):.2f} per ray), "
+                  f"{plan_bytes / 2**30:.1f} GiB, {plan_s:.2f}s")
 
     def iteration(evs=None):
         for s in range(S):
             ctx.fill(u, 0.0)
             if evs is not None:
                 evs[s][0].record()
-            ctx.em_subset(proj, im, x, u, sub[s])
+            if tiled:
+                ctx.em_ratio(proj, im, x, sub[s], vbuf)
+                if evs is not None:
+                    evs[s][1].record()
+                ctx.back(proj, im, u, sub[s], vbuf, plan=plans[s])
+            else:
+                ctx.em_subset(proj, im, x, u, sub[s])
             if evs is not None:
-                evs[s][1].record()
+                evs[s][2].record()
             ctx.em_update(x, u, sens, im.nvox)
 
     for _ in range(args.warmup):
@@ -151,25 +184,41 @@ def main():
     kernel_ms = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+        evs = [tuple(torch.cuda.Event(enable_timing=True) for _ in range(3))
                for _ in range(S)]
         iteration(evs)
         kernel_ms.append(evs)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    per_launch = []
-    for evs in kernel_ms:
-        for s, (a, b) in enumerate(evs):
-            per_launch.append((s, a.elapsed_time(b)))
     s_per_iter = wall / args.steps
     ev_proj_per_s = 2.0 * n * args.steps / wall
-    # roofline of the fused EM kernel: R_ev = 12 B/event (d1, d2, tof) +
-    # 12 B per voxel segment (4 B forward read + 8 B atomic read-modify-write)
-    alg_bytes = [12.0 * (bounds[s + 1] - bounds[s]) + 12.0 * segs[s] for s in range(S)]
-    avg_ms = float(np.mean([ms for _, ms in per_launch]))
-    avg_bytes = float(np.mean([alg_bytes[s] for s, _ in per_launch]))
-    achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
-    log(args, f"{s_per_iter:.3f} s/iter, EM kernel avg {avg_ms:.2f} ms, {achieved:.0f} GB/s alg")
+    # Algorithmic bytes per launch (R_ev = 12 B/event: d1, d2, tof; an atomic
+    # counts as an 8 B read-modify-write):
+    #   forward + ratio:  R_ev + 4 B per voxel segment + 4 B ratio stored
+    #   backprojection:   R_ev + 4 B value + 8 B per voxel segment
+    #   fused EM:         R_ev + 12 B per voxel segment
+    nev = [bounds[s + 1] - bounds[s] for s in range(S)]
+    kernels = []
+
+    def roof(name, idx, bytes_per_subset):
+        ms = [evs[s][idx[0]].elapsed_time(evs[s][idx[1]]) for evs in kernel_ms for s in range(S)]
+        by = [bytes_per_subset[s] for _ in kernel_ms for s in range(S)]
+        avg_ms, avg_b = float(np.mean(ms)), float(np.mean(by))
+        ach = avg_b / (avg_ms * 1e-3) / 1e9
+        kernels.append({"kernel": name, "avg_launch_ms": avg_ms, "alg_bytes_per_launch": avg_b,
+                        "achieved": ach, "frac": ach / HBM_PEAK_GBS})
+
+    if tiled:
+        roof("kEmRatio<TOF,LPE=4> (forward + ratio)", (0, 1),
+             [12.0 * nev[s] + 4.0 * segs[s] + 4.0 * nev[s] for s in range(S)])
+        roof("kTileBack<TOF> (tile-binned backprojection)", (1, 2),
+             [12.0 * nev[s] + 4.0 * nev[s] + 8.0 * segs[s] for s in range(S)])
+    else:
+        roof("kEvents<MODE_EM,Siddon,TOF,LPE=4> (fused EM)", (0, 2),
+             [12.0 * nev[s] + 12.0 * segs[s] for s in range(S)])
+    dom = max(kernels, key=lambda k: k["avg_launch_ms"])
+    log(args, f"{s_per_iter:.3f} s/iter; " + "; ".join(
+        f"{k['kernel']}: {k['avg_launch_ms']:.2f} ms, {k['achieved']:.0f} GB/s alg" for k in kernels))
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -194,13 +243,18 @@ def main():
                    "tof_n_std": 5, "image": [200, 200, 128], "voxel_mm": 2.0,
                    "lanes_per_event": args.lpe, "block": args.block,
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "kEvents<MODE_EM,Siddon,TOF,LPE=4>",
-                     "avg_launch_ms": avg_ms, "alg_bytes_per_launch": avg_bytes,
+        "roofline": {"bound": "hbm", "achieved": dom["achieved"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": dom["frac"], "traffic": None,
+                     "kernel": dom["kernel"], "avg_launch_ms": dom["avg_launch_ms"],
+                     "alg_bytes_per_launch": dom["alg_bytes_per_launch"],
                      "segments_per_event": sum(segs) / n},
+        "kernels": kernels,
+        "accumulation": args.accum,
         "sensitivity_s": sens_s,
         "sensitivity_lors": n_lors,
+        "plan_s": plan_s,
+        "plan_bytes": plan_bytes,
+        "plan_records": plan_records,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

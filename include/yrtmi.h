@@ -242,6 +242,106 @@ int yrtmi_count_segments(yrtmi_ctx* ctx, const yrtmi_proj* p,
                          const yrtmi_image* im, const yrtmi_lors* lors,
                          int64_t* total);
 
+/* ---- device memory and data movement ------------------------------------
+ * Owned device buffers of the host layer (ImageDeviceOwned,
+ * ProjectionListDeviceOwned: src/datastruct/image/ImageDevice.cu:153-230,
+ * src/datastruct/projection/ProjectionListDevice.cu:148-210).  Copies are
+ * synchronous on the context's stream.  kind: 0 host->device, 1 device->host,
+ * 2 device->device. */
+int yrtmi_device_alloc(yrtmi_ctx* ctx, int64_t bytes, void** out);
+int yrtmi_device_free(yrtmi_ctx* ctx, void* p);
+int yrtmi_copy(yrtmi_ctx* ctx, void* dst, const void* src, int64_t bytes,
+               int kind);
+/* The nonzero bins of histogram subset s of S (BinIteratorRangeHistogram3D
+ * order, BinIterator.cpp:147-197) and their values, for the EM subset (the
+ * reference GPU path skips zero-measurement bins, OSEM_GPU.cu:1222-1225).
+ * values: DEVICE float[num_z_bin * num_phi * num_r]; bins_out / meas_out:
+ * DEVICE arrays of at least yrtmi_histo_subset_size(S, s) entries; *count
+ * (HOST) receives the number written.  Deterministic order; synchronises. */
+int yrtmi_histo_subset_nonzero(yrtmi_ctx* ctx, const float* values, int S,
+                               int s, uint32_t* bins_out, float* meas_out,
+                               int64_t* count);
+/* dst[i] = src[s + i * S] for the events of list-mode subset s of S
+ * (BinIteratorChronologicalInterleaved, BinIterator.cpp:261-265), elements
+ * of elem_bytes (1, 2, 4 or 8) bytes; n_total elements in src.  *count
+ * (HOST, nullable) receives the subset size (n_total - s + S - 1) / S. */
+int yrtmi_subset_interleaved(yrtmi_ctx* ctx, const void* src, int elem_bytes,
+                             int64_t n_total, int S, int s, void* dst,
+                             int64_t* count);
+/* out[i] = src[idx[i]] for i < n (per-bin corrections of a bin list);
+ * indices >= src_n give 0.  invert != 0: nonzero values become their
+ * reciprocal (Corrector::getSensitivity with invertSensitivity,
+ * Corrector.cpp:295-308). */
+int yrtmi_gather(yrtmi_ctx* ctx, const float* src, int64_t src_n,
+                 const uint32_t* idx, int64_t n, int invert, float* out);
+/* out[i] = histo[bin of element i of histogram subset s of S], for every
+ * element of the subset (per-bin factors in YRTMI_SRC_HISTO_SUBSET order);
+ * invert as for yrtmi_gather. */
+int yrtmi_histo_subset_gather(yrtmi_ctx* ctx, const float* histo, int S, int s,
+                              int invert, float* out);
+/* y = a * x + b * y over n floats (x NULL: y = b * y).  Image sums and
+ * scalings of the OSEM driver (e.g. the mask as the sum of the subsets'
+ * sensitivity images, OSEM_CPU.cpp:255-269; list-mode sensitivity / S,
+ * OSEM.cpp:1007-1035). */
+int yrtmi_axpby(yrtmi_ctx* ctx, int64_t n, float a, const float* x, float b,
+                float* y);
+
+/* ---- tile plans (Siddon backprojection through LDS image tiles) --------- */
+/* No counterpart in the reference: a performance extension of the
+ * backprojection behind yrtmi_back / yrtmi_em_subset / yrtmi_sensitivity
+ * (OperatorProjectorSiddon_GPU::applyAH, OSEM_GPU::computeEMUpdateImage,
+ * OSEM_GPU::generateSensitivityImageForCurrentSubset).
+ *
+ * A plan bins the Siddon walk of every (LOR, ray) of `lors` by the
+ * 32 x 32 x 16-voxel image tiles it crosses, recording where it enters each.
+ * It depends on geometry only (detector pairs or bins, TOF values, frames,
+ * motion frames, the projector, the image geometry), not on image or LOR
+ * values: a list-mode or histogram subset is planned once per
+ * reconstruction.  The *_planned calls then accumulate each tile in LDS and
+ * add it to the image with coalesced atomics; results equal the unplanned
+ * calls' up to FP32 summation order (and, on voxel sizes that are not short
+ * binary fractions, ulp-level shifts of segment ends at tile boundaries).
+ *
+ * Limits: Siddon only; nx, ny, nz <= 1024; n * num_rays < 2^32; at most 8192
+ * tiles (frames included).  chunk_records: records per workgroup
+ * (0: 4096).  The call synchronises the context's stream and allocates
+ * device memory (yrtmi_plan_info's bytes); the plan owns it. */
+typedef struct yrtmi_plan yrtmi_plan;
+int yrtmi_plan_create(yrtmi_ctx* ctx, const yrtmi_proj* p,
+                      const yrtmi_image* im, const yrtmi_lors* lors,
+                      int chunk_records, yrtmi_plan** out);
+int yrtmi_plan_destroy(yrtmi_plan* plan);
+/* records: (LOR, ray, tile) entries; chunks: workgroups per use; bytes:
+ * device memory held.  Host pointers, any may be NULL. */
+int yrtmi_plan_info(const yrtmi_plan* plan, int64_t* records, int64_t* chunks,
+                    int64_t* bytes);
+/* As yrtmi_back / yrtmi_em_subset / yrtmi_sensitivity through `plan`, which
+ * must come from the same projector, image geometry and LOR set (checked:
+ * projector, geometry and LOR descriptor fields; for the sensitivity call
+ * the plan's projector must have TOF off).  Calls on one plan must not
+ * overlap (em_subset keeps its per-LOR ratios in the plan). */
+int yrtmi_back_planned(yrtmi_ctx* ctx, const yrtmi_proj* p,
+                       const yrtmi_image* im, float* img,
+                       const yrtmi_lors* lors, const float* vals,
+                       const yrtmi_plan* plan);
+int yrtmi_em_subset_planned(yrtmi_ctx* ctx, const yrtmi_proj* p,
+                            const yrtmi_image* im, const float* x, float* u,
+                            const yrtmi_lors* lors,
+                            const yrtmi_corrections* corr,
+                            const yrtmi_plan* plan);
+/* The first half of yrtmi_em_subset on its own (Siddon): v[i] = the EM
+ * ratio of LOR i (m / corrected forward projection, times the
+ * multiplicative corrections), 0 where the reference skips the LOR.
+ * yrtmi_em_subset_planned = yrtmi_em_ratio + yrtmi_back_planned(v). */
+int yrtmi_em_ratio(yrtmi_ctx* ctx, const yrtmi_proj* p, const yrtmi_image* im,
+                   const float* x, const yrtmi_lors* lors,
+                   const yrtmi_corrections* corr, float* v);
+int yrtmi_sensitivity_planned(yrtmi_ctx* ctx, const yrtmi_proj* p,
+                              const yrtmi_image* im, float* img,
+                              const yrtmi_lors* lors,
+                              const yrtmi_image* att_im, const float* att_img,
+                              const float* sens, const yrtmi_plan* plan);
+
 #ifdef __cplusplus
 }
 #endif

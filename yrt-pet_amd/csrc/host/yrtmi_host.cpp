@@ -1,0 +1,929 @@
+// C++ host layer over the yrtmi C ABI (see yrtmi_host.hpp).  Host code only:
+// every device operation goes through include/yrtmi.h.
+#include "yrtmi_host.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <fstream>
+#include <sstream>
+#include <type_traits>
+
+namespace yrtmi
+{
+
+void check(int rc, const char* what)
+{
+	if (rc == YRTMI_OK)
+		return;
+	std::string m = yrtmi_last_error();
+	if (what != nullptr)
+		m = std::string(what) + ": " + m;
+	throw Error(m);
+}
+
+// ---------------------------------------------------------------------------
+// Scanner
+// ---------------------------------------------------------------------------
+Scanner::Scanner(std::string name, float axialFOV_, float crystalSize_z_,
+                 float crystalSize_trans_, float crystalDepth_,
+                 float scannerRadius_, size_t detsPerRing_, size_t numRings_,
+                 size_t numDOI_, size_t maxRingDiff_, size_t minAngDiff_,
+                 size_t detsPerBlock_)
+    : scannerName(std::move(name)), axialFOV(axialFOV_),
+      crystalSize_z(crystalSize_z_), crystalSize_trans(crystalSize_trans_),
+      crystalDepth(crystalDepth_), scannerRadius(scannerRadius_),
+      detsPerRing(detsPerRing_), numRings(numRings_), numDOI(numDOI_),
+      maxRingDiff(maxRingDiff_), minAngDiff(minAngDiff_),
+      detsPerBlock(detsPerBlock_)
+{
+	if (numRings == 0 || detsPerRing == 0 || numDOI == 0)
+		throw Error("Scanner: detsPerRing, numRings and numDOI must be positive");
+	if (maxRingDiff >= numRings)
+		throw Error("Scanner: maxRingDiff must be smaller than numRings");
+	if (minAngDiff == 0)
+		throw Error("Scanner: minAngDiff must be positive");
+}
+
+size_t Scanner::getNumDets() const
+{
+	return detsPerRing * numRings * numDOI;
+}
+
+void Scanner::makeLUT() const
+{
+	if (!m_pos.empty())
+		return;
+	const size_t n = getNumDets();
+	m_pos.assign(3 * n, 0.f);
+	m_ori.assign(3 * n, 0.f);
+	yrtmi_scanner_desc d{axialFOV, crystalSize_z, crystalSize_trans, crystalDepth,
+	                     scannerRadius, (int64_t)detsPerRing, (int64_t)numRings,
+	                     (int64_t)numDOI, (int64_t)detsPerBlock};
+	check(yrtmi_lut_regular(&d, m_pos.data(), m_ori.data()), "Scanner LUT");
+}
+
+const std::vector<float>& Scanner::lutPositions() const
+{
+	makeLUT();
+	return m_pos;
+}
+
+const std::vector<float>& Scanner::lutOrientations() const
+{
+	makeLUT();
+	return m_ori;
+}
+
+// ---------------------------------------------------------------------------
+// ImageParams
+// ---------------------------------------------------------------------------
+ImageParams::ImageParams(int64_t nx_, int64_t ny_, int64_t nz_, float lx,
+                         float ly, float lz, float ox, float oy, float oz,
+                         int64_t nt_)
+    : nx(nx_), ny(ny_), nz(nz_), nt(nt_), length_x(lx), length_y(ly),
+      length_z(lz), vx(-1.f), vy(-1.f), vz(-1.f), off_x(ox), off_y(oy), off_z(oz)
+{
+	setup();
+}
+
+ImageParams ImageParams::fromParams(int64_t nx, int64_t ny, int64_t nz, float vx,
+                                    float vy, float vz, int64_t nt)
+{
+	ImageParams p(nx, ny, nz, (float)nx * vx, (float)ny * vy, (float)nz * vz, 0.f,
+	              0.f, 0.f, nt);
+	// keep the given voxel sizes (completeDimInfo returns early when both
+	// the voxel size and the length are set)
+	p.vx = vx;
+	p.vy = vy;
+	p.vz = vz;
+	p.setup();
+	return p;
+}
+
+void ImageParams::setup()
+{
+	if (nx <= 0 || ny <= 0 || nz <= 0 || nt <= 0)
+		throw Error("ImageParams: dimensions must be positive");
+	if (!(length_x > 0.f && length_y > 0.f && length_z > 0.f))
+		throw Error("ImageParams: lengths must be positive");
+	// completeDimInfo (ImageParams.cpp:220-266): v = length / n when unset
+	if (!(vx > 0.f))
+		vx = length_x / (float)nx;
+	if (!(vy > 0.f))
+		vy = length_y / (float)ny;
+	if (!(vz > 0.f))
+		vz = length_z / (float)nz;
+	// ImageParams::setup (ImageParams.cpp:216-217)
+	fovRadius = std::max(length_x / 2.0f, length_y / 2.0f);
+	fovRadius -= std::max(vx, vy) / 1000.0f;
+}
+
+bool ImageParams::isSameAs(const ImageParams& o) const
+{
+	return nx == o.nx && ny == o.ny && nz == o.nz && nt == o.nt &&
+	       length_x == o.length_x && length_y == o.length_y &&
+	       length_z == o.length_z && off_x == o.off_x && off_y == o.off_y &&
+	       off_z == o.off_z;
+}
+
+yrtmi_image ImageParams::desc(int64_t frames, bool zeroOffset) const
+{
+	yrtmi_image d{};
+	d.nx = nx;
+	d.ny = ny;
+	d.nz = nz;
+	d.nt = frames > 0 ? frames : nt;
+	d.vx = vx;
+	d.vy = vy;
+	d.vz = vz;
+	d.lx = length_x;
+	d.ly = length_y;
+	d.lz = length_z;
+	d.ox = zeroOffset ? 0.f : off_x;
+	d.oy = zeroOffset ? 0.f : off_y;
+	d.oz = zeroOffset ? 0.f : off_z;
+	d.fov_radius = fovRadius;
+	return d;
+}
+
+// ---------------------------------------------------------------------------
+// ProjectorParams
+// ---------------------------------------------------------------------------
+ProjectorParams::ProjectorParams(const Scanner& s) : scanner(s) {}
+
+void ProjectorParams::setProjector(const std::string& name)
+{
+	std::string n = name;
+	std::transform(n.begin(), n.end(), n.begin(), ::toupper);
+	if (n == "S" || n == "SIDDON")
+		projectorType = ProjectorType::SIDDON;
+	else if (n == "D" || n == "DD")
+		projectorType = ProjectorType::DD;
+	else
+		throw Error("Unknown projector: " + name);
+}
+
+void ProjectorParams::addTOF(float w, int nstd)
+{
+	m_tofWidth_ps = w;
+	m_tofNumStd = nstd;
+	m_hasTof = true;
+}
+
+void ProjectorParams::setProjPsf(const std::string& fname)
+{
+	// one CSV row: sStep, kSpacing, K, then K * numS taps
+	std::ifstream f(fname);
+	if (!f)
+		throw Error("Could not open projection PSF file " + fname);
+	std::string line;
+	std::getline(f, line);
+	std::stringstream ss(line);
+	std::vector<float> row;
+	std::string tok;
+	while (std::getline(ss, tok, ','))
+		row.push_back(std::stof(tok));
+	if (row.size() < 4)
+		throw Error("Projection PSF file too short: " + fname);
+	const int K = (int)row[2];
+	if (K <= 0 || K % 2 == 0)
+		throw Error("Kernels must be of odd length");
+	const int numS = (int)((row.size() - 3) / (size_t)K);
+	setProjPsf(std::vector<float>(row.begin() + 3, row.begin() + 3 + (size_t)numS * K),
+	           numS, K, row[0], row[1]);
+	projPsf_fname = fname;
+}
+
+void ProjectorParams::setProjPsf(const std::vector<float>& k, int numS, int K,
+                                 float sStep, float kSpacing)
+{
+	if (numS <= 0 || K <= 0 || k.size() != (size_t)numS * K)
+		throw Error("Projection PSF: kernel table size mismatch");
+	m_psf = k;
+	m_psfNumS = numS;
+	m_psfK = K;
+	m_psfSStep = sStep;
+	m_psfKSpacing = kSpacing;
+}
+
+yrtmi_projector_desc ProjectorParams::desc(bool withTOF) const
+{
+	yrtmi_projector_desc d{};
+	d.type = projectorType == ProjectorType::DD ? YRTMI_PROJ_DD : YRTMI_PROJ_SIDDON;
+	d.num_rays = numRays;
+	d.use_tof = (withTOF && m_hasTof) ? 1 : 0;
+	d.tof_width_ps = m_tofWidth_ps;
+	d.tof_n_std = m_tofNumStd;
+	d.psf_kernels = m_psf.empty() ? nullptr : m_psf.data();
+	d.psf_num_s = m_psfNumS;
+	d.psf_ksize = m_psfK;
+	d.psf_s_step = m_psfSStep;
+	d.psf_k_spacing = m_psfKSpacing;
+	d.crystal_size_z = scanner.crystalSize_z;
+	d.crystal_size_trans = scanner.crystalSize_trans;
+	d.crystal_depth = scanner.crystalDepth;
+	d.setup_fp64 = setupFP64 ? 1 : 0;
+	return d;
+}
+
+// ---------------------------------------------------------------------------
+// Context
+// ---------------------------------------------------------------------------
+Context::Context(const Scanner& s, int device, void* stream)
+    : m_scanner(s), m_device(device)
+{
+	check(yrtmi_ctx_create(device, stream, &m_ctx), "context");
+	const auto& pos = s.lutPositions();
+	const auto& ori = s.lutOrientations();
+	check(yrtmi_lut_upload(m_ctx, pos.data(), ori.data(), (int64_t)s.getNumDets()),
+	      "LUT upload");
+	if (s.detsPerRing % 2 == 0)
+	{
+		yrtmi_histo_desc h{(int64_t)s.detsPerRing, (int64_t)s.numRings, (int64_t)s.numDOI,
+		                   (int64_t)s.maxRingDiff, (int64_t)s.minAngDiff};
+		check(yrtmi_histo_setup(m_ctx, &h, &m_numZBin, &m_numPhi, &m_numR),
+		      "histogram setup");
+	}
+}
+
+Context::~Context()
+{
+	yrtmi_ctx_destroy(m_ctx);
+}
+
+void Context::setStream(void* stream)
+{
+	check(yrtmi_ctx_set_stream(m_ctx, stream), "set stream");
+}
+
+void Context::synchronize() const
+{
+	check(yrtmi_ctx_synchronize(m_ctx), "synchronize");
+}
+
+void Context::uploadMotion(const std::vector<float>& xf)
+{
+	if (xf.size() % 12 != 0)
+		throw Error("motion transforms: 12 floats each");
+	check(yrtmi_motion_upload(m_ctx, xf.data(), (int)(xf.size() / 12)), "motion upload");
+}
+
+// ---------------------------------------------------------------------------
+// DeviceArray
+// ---------------------------------------------------------------------------
+template <typename T>
+DeviceArray<T>::DeviceArray(const Context& ctx, size_t n) : m_ctx(ctx), m_n(n)
+{
+	void* p = nullptr;
+	check(yrtmi_device_alloc(ctx.get(), (int64_t)(n * sizeof(T)), &p), "device alloc");
+	m_p = static_cast<T*>(p);
+}
+
+template <typename T>
+DeviceArray<T>::DeviceArray(const Context& ctx, T* alias, size_t n)
+    : m_ctx(ctx), m_p(alias), m_n(n), m_alias(true)
+{
+	if (alias == nullptr && n > 0)
+		throw Error("alias of a null device pointer");
+}
+
+template <typename T>
+DeviceArray<T>::~DeviceArray()
+{
+	if (!m_alias && m_p != nullptr)
+		yrtmi_device_free(m_ctx.get(), m_p);
+}
+
+template <typename T>
+void DeviceArray<T>::copyFromHost(const T* src, size_t n)
+{
+	if (n > m_n)
+		throw Error("copy larger than the device array");
+	check(yrtmi_copy(m_ctx.get(), m_p, src, (int64_t)(n * sizeof(T)), 0), "upload");
+}
+
+template <typename T>
+void DeviceArray<T>::copyToHost(T* dst, size_t n) const
+{
+	if (n > m_n)
+		throw Error("copy larger than the device array");
+	check(yrtmi_copy(m_ctx.get(), dst, m_p, (int64_t)(n * sizeof(T)), 1), "download");
+}
+
+template class DeviceArray<float>;
+template class DeviceArray<uint32_t>;
+template class DeviceArray<int32_t>;
+template class DeviceArray<uint8_t>;
+
+// ---------------------------------------------------------------------------
+// ImageDevice
+// ---------------------------------------------------------------------------
+ImageDevice::ImageDevice(const Context& ctx, const ImageParams& params)
+    : m_ctx(ctx), m_params(params),
+      m_buf(new DeviceArray<float>(ctx, params.numVoxels() * (size_t)params.nt))
+{
+	setValue(0.f);
+}
+
+ImageDevice::ImageDevice(const Context& ctx, const ImageParams& params, float* ptr)
+    : m_ctx(ctx), m_params(params),
+      m_buf(new DeviceArray<float>(ctx, ptr, params.numVoxels() * (size_t)params.nt))
+{
+}
+
+void ImageDevice::setValue(float v)
+{
+	check(yrtmi_fill(m_ctx.get(), m_buf->data(), (int64_t)m_buf->size(), v), "fill");
+}
+
+void ImageDevice::copyFromHost(const float* src)
+{
+	m_buf->copyFromHost(src, m_buf->size());
+}
+
+void ImageDevice::copyToHost(float* dst) const
+{
+	m_buf->copyToHost(dst, m_buf->size());
+}
+
+void ImageDevice::copyFrom(const ImageDevice& o)
+{
+	if (o.size() != size())
+		throw Error("image copy: size mismatch");
+	check(yrtmi_copy(m_ctx.get(), m_buf->data(), o.getDevicePointer(),
+	                 (int64_t)(size() * sizeof(float)), 2),
+	      "image copy");
+}
+
+// ---------------------------------------------------------------------------
+// ListModeDevice / Histogram3DDevice
+// ---------------------------------------------------------------------------
+ListModeDevice::ListModeDevice(const Context& ctx, size_t n, const uint32_t* d1,
+                               const uint32_t* d2, const float* tof,
+                               const int32_t* frame, const uint8_t* motion,
+                               const float* randoms)
+    : m_ctx(ctx), m_n(n)
+{
+	m_d1.reset(new DeviceArray<uint32_t>(ctx, n));
+	m_d2.reset(new DeviceArray<uint32_t>(ctx, n));
+	m_d1->copyFromHost(d1, n);
+	m_d2->copyFromHost(d2, n);
+	if (tof)
+	{
+		m_tof.reset(new DeviceArray<float>(ctx, n));
+		m_tof->copyFromHost(tof, n);
+	}
+	if (frame)
+	{
+		m_frame.reset(new DeviceArray<int32_t>(ctx, n));
+		m_frame->copyFromHost(frame, n);
+	}
+	if (motion)
+	{
+		m_motion.reset(new DeviceArray<uint8_t>(ctx, n));
+		m_motion->copyFromHost(motion, n);
+	}
+	if (randoms)
+	{
+		m_randoms.reset(new DeviceArray<float>(ctx, n));
+		m_randoms->copyFromHost(randoms, n);
+	}
+	m_values.reset(new DeviceArray<float>(ctx, n));
+}
+
+std::unique_ptr<ListModeDevice> ListModeDevice::alias(const Context& ctx, size_t n,
+                                                      uint32_t* d1, uint32_t* d2,
+                                                      float* tof, int32_t* frame,
+                                                      uint8_t* motion, float* randoms)
+{
+	std::unique_ptr<ListModeDevice> lm(new ListModeDevice(ctx));
+	lm->m_n = n;
+	lm->m_d1.reset(new DeviceArray<uint32_t>(ctx, d1, n));
+	lm->m_d2.reset(new DeviceArray<uint32_t>(ctx, d2, n));
+	if (tof)
+		lm->m_tof.reset(new DeviceArray<float>(ctx, tof, n));
+	if (frame)
+		lm->m_frame.reset(new DeviceArray<int32_t>(ctx, frame, n));
+	if (motion)
+		lm->m_motion.reset(new DeviceArray<uint8_t>(ctx, motion, n));
+	if (randoms)
+		lm->m_randoms.reset(new DeviceArray<float>(ctx, randoms, n));
+	lm->m_values.reset(new DeviceArray<float>(ctx, n));
+	return lm;
+}
+
+yrtmi_lors ListModeDevice::lors() const
+{
+	yrtmi_lors l{};
+	l.kind = YRTMI_SRC_PAIRS;
+	l.n = (int64_t)m_n;
+	l.d1 = d1();
+	l.d2 = d2();
+	l.num_subsets = 1;
+	l.tof = tof();
+	l.frame = frame();
+	l.motion_frame = motion();
+	return l;
+}
+
+Histogram3DDevice::Histogram3DDevice(const Context& ctx) : m_ctx(ctx)
+{
+	if (!ctx.hasHistogram())
+		throw Error("Histogram3D needs an even number of detectors per ring");
+	m_buf.reset(new DeviceArray<float>(ctx, (size_t)ctx.histogramSize()));
+	check(yrtmi_fill(ctx.get(), m_buf->data(), (int64_t)m_buf->size(), 0.f), "fill");
+}
+
+Histogram3DDevice::Histogram3DDevice(const Context& ctx, float* ptr) : m_ctx(ctx)
+{
+	if (!ctx.hasHistogram())
+		throw Error("Histogram3D needs an even number of detectors per ring");
+	m_buf.reset(new DeviceArray<float>(ctx, ptr, (size_t)ctx.histogramSize()));
+}
+
+void Histogram3DDevice::copyFromHost(const float* src)
+{
+	m_buf->copyFromHost(src, m_buf->size());
+}
+
+void Histogram3DDevice::copyToHost(float* dst) const
+{
+	m_buf->copyToHost(dst, m_buf->size());
+}
+
+yrtmi_lors Histogram3DDevice::lors() const
+{
+	yrtmi_lors l{};
+	l.kind = YRTMI_SRC_HISTO_SUBSET;
+	l.n = (int64_t)m_buf->size();
+	l.num_subsets = 1;
+	l.subset = 0;
+	return l;
+}
+
+// ---------------------------------------------------------------------------
+// OperatorProjectorMI
+// ---------------------------------------------------------------------------
+OperatorProjectorMI::OperatorProjectorMI(const Context& ctx, const ProjectorParams& pp,
+                                         bool withTOF)
+    : m_ctx(ctx)
+{
+	yrtmi_projector_desc d = pp.desc(withTOF);
+	check(yrtmi_proj_create(ctx.get(), &d, &m_proj), "projector");
+	m_siddon = pp.projectorType == ProjectorType::SIDDON;
+	m_numRays = m_siddon ? pp.numRays : 1;
+	setLaunch(0);
+}
+
+OperatorProjectorMI::~OperatorProjectorMI()
+{
+	yrtmi_proj_destroy(m_proj);
+}
+
+void OperatorProjectorMI::setLaunch(int lpe, int block, int grid)
+{
+	if (lpe <= 0)
+		lpe = m_numRays >= 4 ? 4 : (m_numRays >= 2 ? 2 : 1);
+	check(yrtmi_proj_set_launch(m_proj, lpe, block, grid), "launch shape");
+}
+
+static void needFrames(const ImageDevice& img, int numFrames)
+{
+	if (img.getParams().nt != numFrames)
+		throw Error("image frames do not match the projection data's dynamic framing");
+}
+
+void OperatorProjectorMI::applyA(const ImageDevice& img, ListModeDevice& proj)
+{
+	needFrames(img, proj.numFrames);
+	const yrtmi_image im = img.getParams().desc();
+	const yrtmi_lors l = proj.lors();
+	check(yrtmi_forward(m_ctx.get(), m_proj, &im, img.getDevicePointer(), &l, proj.values()),
+	      "applyA");
+}
+
+void OperatorProjectorMI::applyAH(const ListModeDevice& proj, ImageDevice& img)
+{
+	needFrames(img, proj.numFrames);
+	const yrtmi_image im = img.getParams().desc();
+	const yrtmi_lors l = proj.lors();
+	check(yrtmi_back(m_ctx.get(), m_proj, &im, img.getDevicePointer(), &l, proj.values()),
+	      "applyAH");
+}
+
+void OperatorProjectorMI::applyA(const ImageDevice& img, Histogram3DDevice& proj)
+{
+	needFrames(img, 1);
+	const yrtmi_image im = img.getParams().desc();
+	const yrtmi_lors l = proj.lors();
+	check(yrtmi_forward(m_ctx.get(), m_proj, &im, img.getDevicePointer(), &l,
+	                    proj.getDevicePointer()),
+	      "applyA");
+}
+
+void OperatorProjectorMI::applyAH(const Histogram3DDevice& proj, ImageDevice& img)
+{
+	needFrames(img, 1);
+	const yrtmi_image im = img.getParams().desc();
+	const yrtmi_lors l = proj.lors();
+	check(yrtmi_back(m_ctx.get(), m_proj, &im, img.getDevicePointer(), &l,
+	                 proj.getDevicePointer()),
+	      "applyAH");
+}
+
+// ---------------------------------------------------------------------------
+// OSEM_MI
+// ---------------------------------------------------------------------------
+namespace
+{
+// An owned plan handle.
+struct PlanHandle
+{
+	yrtmi_plan* p = nullptr;
+	PlanHandle() = default;
+	explicit PlanHandle(yrtmi_plan* q) : p(q) {}
+	~PlanHandle() { yrtmi_plan_destroy(p); }
+	PlanHandle(const PlanHandle&) = delete;
+	PlanHandle& operator=(const PlanHandle&) = delete;
+};
+
+std::unique_ptr<PlanHandle> makePlan(const Context& ctx, const OperatorProjectorMI& proj,
+                                     const yrtmi_image& im, const yrtmi_lors& l)
+{
+	yrtmi_plan* q = nullptr;
+	check(yrtmi_plan_create(ctx.get(), proj.get(), &im, &l, 0, &q), "tile plan");
+	return std::unique_ptr<PlanHandle>(new PlanHandle(q));
+}
+}  // namespace
+
+// One subset's device data, kept for the whole reconstruction.
+struct OSEM_MI::Subset
+{
+	yrtmi_lors lors{};
+	std::unique_ptr<DeviceArray<uint32_t>> d1, d2, bins;
+	std::unique_ptr<DeviceArray<float>> tof, meas, att, sens, randoms, scatter;
+	std::unique_ptr<DeviceArray<int32_t>> frame;
+	std::unique_ptr<DeviceArray<uint8_t>> motion;
+	std::unique_ptr<PlanHandle> plan;
+};
+
+OSEM_MI::OSEM_MI(const Scanner& scanner, int device, void* stream)
+    : projectorParams(scanner), m_ctx(new Context(scanner, device, stream))
+{
+}
+
+OSEM_MI::~OSEM_MI() = default;
+
+void OSEM_MI::setDataInput(const ListModeDevice* lm)
+{
+	m_lm = lm;
+	m_histo = nullptr;
+}
+
+void OSEM_MI::setDataInput(const Histogram3DDevice* h)
+{
+	m_histo = h;
+	m_lm = nullptr;
+}
+
+int OSEM_MI::getExpectedSensImagesAmount() const
+{
+	return isListModeEnabled() ? 1 : num_OSEM_subsets;
+}
+
+std::unique_ptr<OperatorProjectorMI> OSEM_MI::makeProjector(bool withTOF) const
+{
+	std::unique_ptr<OperatorProjectorMI> p(
+	    new OperatorProjectorMI(*m_ctx, projectorParams, withTOF));
+	p->setLaunch(lanesPerEvent);
+	return p;
+}
+
+bool OSEM_MI::planned() const
+{
+	return useTilePlans && projectorParams.projectorType == ProjectorType::SIDDON;
+}
+
+ImageParams OSEM_MI::singleFrame() const
+{
+	ImageParams p = m_imageParams;
+	p.nt = 1;
+	return p;
+}
+
+void OSEM_MI::setSensitivityImages(const std::vector<const ImageDevice*>& imgs)
+{
+	m_sens.clear();
+	for (const ImageDevice* s : imgs)
+	{
+		if (s == nullptr)
+			throw Error("null sensitivity image");
+		std::unique_ptr<ImageDevice> c(new ImageDevice(*m_ctx, s->getParams()));
+		c->copyFrom(*s);
+		m_sens.push_back(std::move(c));
+	}
+}
+
+const std::vector<std::unique_ptr<ImageDevice>>& OSEM_MI::generateSensitivityImages()
+{
+	if (!m_hasImageParams)
+		throw Error("Image parameters not set");
+	const ImageParams im1 = singleFrame();
+	const yrtmi_image ci = im1.desc(1);
+	const int S = getExpectedSensImagesAmount();
+	// sensitivity projector: TOF off (OSEM_CPU.cpp:94-108)
+	auto proj = makeProjector(false);
+	yrtmi_image ca{};
+	if (m_att != nullptr)
+		ca = m_att->getParams().desc(1);
+	m_sens.clear();
+	for (int s = 0; s < S; s++)
+	{
+		std::unique_ptr<ImageDevice> img(new ImageDevice(*m_ctx, im1));
+		yrtmi_lors l{};
+		l.kind = YRTMI_SRC_HISTO_SUBSET;
+		l.n = yrtmi_histo_subset_size(m_ctx->get(), S, s);
+		if (l.n < 0)
+			throw Error("sensitivity: the scanner has no histogram geometry");
+		l.num_subsets = S;
+		l.subset = s;
+		std::unique_ptr<DeviceArray<float>> sensv;
+		const float* sensPtr = nullptr;
+		if (m_sensHisto != nullptr && l.n > 0)
+		{
+			// per-bin sensitivity factors of every bin of the subset
+			// (Corrector::getSensitivity, Corrector.cpp:295-308)
+			sensv.reset(new DeviceArray<float>(*m_ctx, (size_t)l.n));
+			check(yrtmi_histo_subset_gather(m_ctx->get(), m_sensHisto->getDevicePointer(), S,
+			                                s, m_invertSens ? 1 : 0, sensv->data()),
+			      "sensitivity factors");
+			sensPtr = sensv->data();
+		}
+		if (planned())
+		{
+			auto plan = makePlan(*m_ctx, *proj, ci, l);
+			check(yrtmi_sensitivity_planned(m_ctx->get(), proj->get(), &ci,
+			                                img->getDevicePointer(), &l,
+			                                m_att ? &ca : nullptr,
+			                                m_att ? m_att->getDevicePointer() : nullptr,
+			                                sensPtr, plan->p),
+			      "sensitivity");
+		}
+		else
+		{
+			check(yrtmi_sensitivity(m_ctx->get(), proj->get(), &ci, img->getDevicePointer(),
+			                        &l, m_att ? &ca : nullptr,
+			                        m_att ? m_att->getDevicePointer() : nullptr, sensPtr),
+			      "sensitivity");
+		}
+		// hard threshold: <= thr -> 0, else * global scale
+		// (applyThreshold, OSEM_CPU.cpp:168-172); the mask is a copy
+		ImageDevice mask(*m_ctx, im1);
+		mask.copyFrom(*img);
+		check(yrtmi_threshold(m_ctx->get(), img->getDevicePointer(), mask.getDevicePointer(),
+		                      (int64_t)im1.numVoxels(), 1, 1, hardThreshold, 0.f, 0.f,
+		                      m_globalScale, 0.f),
+		      "threshold");
+		m_sens.push_back(std::move(img));
+	}
+	return m_sens;
+}
+
+void OSEM_MI::prepareSubsets(OperatorProjectorMI& proj, const ImageParams& imt)
+{
+	m_subsets.clear();
+	const int S = num_OSEM_subsets;
+	const yrtmi_image ci = imt.desc();
+	yrtmi_image ca{};
+	if (m_att != nullptr)
+		ca = m_att->getParams().desc(1);
+	yrtmi_ctx* c = m_ctx->get();
+	if (m_lm != nullptr)
+	{
+		if (!m_lm->motionTransforms.empty())
+			m_ctx->uploadMotion(m_lm->motionTransforms);
+		const int64_t N = (int64_t)m_lm->count();
+		for (int s = 0; s < S; s++)
+		{
+			// BinIteratorChronologicalInterleaved: events s, s + S, ...
+			std::unique_ptr<Subset> q(new Subset());
+			const int64_t n = N > s ? (N - s + S - 1) / S : 0;
+			auto take = [&](auto& dst, auto* src, int bytes) {
+				using A = typename std::remove_reference<decltype(*dst)>::type;
+				if (src == nullptr)
+					return;
+				dst.reset(new A(*m_ctx, (size_t)n));
+				check(yrtmi_subset_interleaved(c, src, bytes, N, S, s, dst->data(), nullptr),
+				      "list-mode subset");
+			};
+			take(q->d1, m_lm->d1(), 4);
+			take(q->d2, m_lm->d2(), 4);
+			take(q->tof, m_lm->tof(), 4);
+			take(q->frame, m_lm->frame(), 4);
+			take(q->motion, m_lm->motion(), 1);
+			take(q->randoms, m_lm->randoms(), 4);
+			q->lors.kind = YRTMI_SRC_PAIRS;
+			q->lors.n = n;
+			q->lors.d1 = q->d1 ? q->d1->data() : nullptr;
+			q->lors.d2 = q->d2 ? q->d2->data() : nullptr;
+			q->lors.num_subsets = 1;
+			q->lors.tof = q->tof ? q->tof->data() : nullptr;
+			q->lors.frame = q->frame ? q->frame->data() : nullptr;
+			q->lors.motion_frame = q->motion ? q->motion->data() : nullptr;
+			if (m_att != nullptr && n > 0)
+			{
+				// per-event attenuation factors, once (precomputeAttenuationFactors)
+				q->att.reset(new DeviceArray<float>(*m_ctx, (size_t)n));
+				yrtmi_lors la = q->lors;
+				la.tof = nullptr;
+				la.frame = nullptr;
+				check(yrtmi_attenuation_factors(c, &ca, m_att->getDevicePointer(), &la,
+				                                q->att->data()),
+				      "attenuation factors");
+			}
+			m_subsets.push_back(std::move(q));
+		}
+	}
+	else
+	{
+		for (int s = 0; s < S; s++)
+		{
+			std::unique_ptr<Subset> q(new Subset());
+			const int64_t cap = yrtmi_histo_subset_size(c, S, s);
+			if (cap < 0)
+				throw Error("histogram subset out of range");
+			q->bins.reset(new DeviceArray<uint32_t>(*m_ctx, (size_t)std::max<int64_t>(cap, 1)));
+			q->meas.reset(new DeviceArray<float>(*m_ctx, (size_t)std::max<int64_t>(cap, 1)));
+			int64_t n = 0;
+			// zero-count bins contribute nothing (OSEM_GPU.cu:1222-1225)
+			check(yrtmi_histo_subset_nonzero(c, m_histo->getDevicePointer(), S, s,
+			                                 q->bins->data(), q->meas->data(), &n),
+			      "nonzero bins");
+			q->lors.kind = YRTMI_SRC_BINS;
+			q->lors.n = n;
+			q->lors.bins = q->bins->data();
+			q->lors.num_subsets = 1;
+			auto gatherHisto = [&](std::unique_ptr<DeviceArray<float>>& dst,
+			                       const Histogram3DDevice* h, bool invert) {
+				if (h == nullptr || n == 0)
+					return;
+				dst.reset(new DeviceArray<float>(*m_ctx, (size_t)n));
+				check(yrtmi_gather(c, h->getDevicePointer(), (int64_t)h->count(),
+				                   q->bins->data(), n, invert ? 1 : 0, dst->data()),
+				      "per-bin correction");
+			};
+			gatherHisto(q->sens, m_sensHisto, m_invertSens);
+			gatherHisto(q->randoms, m_randomsHisto, false);
+			gatherHisto(q->scatter, m_scatterHisto, false);
+			if (m_att != nullptr && n > 0)
+			{
+				q->att.reset(new DeviceArray<float>(*m_ctx, (size_t)n));
+				check(yrtmi_attenuation_factors(c, &ca, m_att->getDevicePointer(), &q->lors,
+				                                q->att->data()),
+				      "attenuation factors");
+			}
+			m_subsets.push_back(std::move(q));
+		}
+	}
+	if (planned())
+		for (auto& q : m_subsets)
+			if (q->lors.n > 0)
+				q->plan = makePlan(*m_ctx, proj, ci, q->lors);
+}
+
+std::unique_ptr<ImageDevice> OSEM_MI::reconstruct()
+{
+	if (!m_hasImageParams)
+		throw Error("Image parameters not set");
+	if (m_lm == nullptr && m_histo == nullptr)
+		throw Error("Data input not set");
+	const int S = num_OSEM_subsets;
+	if (S <= 0 || num_MLEM_iterations < 0)
+		throw Error("bad iteration / subset counts");
+	if ((int)m_sens.size() != getExpectedSensImagesAmount())
+		throw Error("The number of sensitivity images provided does not match the number "
+		            "of subsets. Expected " +
+		            std::to_string(getExpectedSensImagesAmount()) + " but received " +
+		            std::to_string(m_sens.size()));
+	yrtmi_ctx* c = m_ctx->get();
+	// setupForDynamicRecon (OSEM.cpp:823-830)
+	ImageParams imt = m_imageParams;
+	imt.nt = m_lm != nullptr ? m_lm->numFrames : 1;
+	const int64_t nvox = (int64_t)imt.numVoxels();
+	const ImageParams im1 = singleFrame();
+	// list-mode sensitivity x 1/S on a copy (OSEM.cpp:1007-1035)
+	std::vector<std::unique_ptr<ImageDevice>> sens;
+	for (auto& s : m_sens)
+	{
+		std::unique_ptr<ImageDevice> cpy(new ImageDevice(*m_ctx, im1));
+		cpy->copyFrom(*s);
+		if (isListModeEnabled() && S != 1)
+			check(yrtmi_axpby(c, nvox, 0.f, nullptr, 1.0f / (float)S, cpy->getDevicePointer()),
+			      "sensitivity scale");
+		sens.push_back(std::move(cpy));
+	}
+	std::unique_ptr<ImageDevice> x(new ImageDevice(*m_ctx, imt));
+	if (m_initial != nullptr)
+		x->copyFrom(*m_initial);
+	else
+		x->setValue(INITIAL_VALUE_MLEM);
+	// mask (OSEM_CPU.cpp:241-269): the mask image, the sensitivity image, or
+	// the sum of the subsets' sensitivity images
+	ImageDevice mask(*m_ctx, im1);
+	if (m_mask != nullptr)
+		mask.copyFrom(*m_mask);
+	else if (S == 1 || isListModeEnabled())
+		mask.copyFrom(*sens[0]);
+	else
+	{
+		// the sum of the subsets' sensitivity images, in subset order
+		for (auto& si : sens)
+			check(yrtmi_axpby(c, nvox, 1.f, si->getDevicePointer(), 1.f,
+			                  mask.getDevicePointer()),
+			      "mask sum");
+	}
+	check(yrtmi_threshold(c, x->getDevicePointer(), mask.getDevicePointer(), nvox, imt.nt, 1,
+	                      0.f, 0.f, 0.f, 1.f, 0.f),
+	      "mask");
+	auto proj = makeProjector(true);
+	prepareSubsets(*proj, imt);
+	ImageDevice u(*m_ctx, imt);
+	const yrtmi_image ci = imt.desc();
+	for (int it = 0; it < num_MLEM_iterations; it++)
+	{
+		for (int s = 0; s < S; s++)
+		{
+			Subset& q = *m_subsets[s];
+			u.setValue(0.f);
+			yrtmi_corrections cr{};
+			cr.meas = q.meas ? q.meas->data() : nullptr;
+			cr.sens = q.sens ? q.sens->data() : nullptr;
+			cr.att = q.att ? q.att->data() : nullptr;
+			cr.randoms = q.randoms ? q.randoms->data() : nullptr;
+			cr.scatter = q.scatter ? q.scatter->data() : nullptr;
+			cr.global_scale = m_globalScale;
+			cr.denom_threshold = denomThreshold;
+			if (q.lors.n > 0)
+			{
+				if (q.plan)
+					check(yrtmi_em_subset_planned(c, proj->get(), &ci, x->getDevicePointer(),
+					                              u.getDevicePointer(), &q.lors, &cr, q.plan->p),
+					      "EM subset");
+				else
+					check(yrtmi_em_subset(c, proj->get(), &ci, x->getDevicePointer(),
+					                      u.getDevicePointer(), &q.lors, &cr),
+					      "EM subset");
+			}
+			const ImageDevice& sImg = *sens[isListModeEnabled() ? 0 : s];
+			check(yrtmi_em_update(c, x->getDevicePointer(), u.getDevicePointer(),
+			                      sImg.getDevicePointer(), nvox, imt.nt, 1, EPS_FLT),
+			      "EM update");
+		}
+	}
+	m_ctx->synchronize();
+	return x;
+}
+
+std::unique_ptr<OSEM_MI> createOSEM(const Scanner& scanner, bool useGPU, int device)
+{
+	if (!useGPU)
+		throw Error("yrtmi provides the GPU OSEM only");
+	return std::unique_ptr<OSEM_MI>(new OSEM_MI(scanner, device));
+}
+
+// ---------------------------------------------------------------------------
+// util::forwProject / backProject
+// ---------------------------------------------------------------------------
+void forwProject(const Context& ctx, const ProjectorParams& pp, const ImageDevice& img,
+                 ListModeDevice& proj)
+{
+	OperatorProjectorMI op(ctx, pp);
+	op.applyA(img, proj);
+	ctx.synchronize();
+}
+
+void backProject(const Context& ctx, const ProjectorParams& pp, ImageDevice& img,
+                 const ListModeDevice& proj)
+{
+	OperatorProjectorMI op(ctx, pp);
+	op.applyAH(proj, img);
+	ctx.synchronize();
+}
+
+void forwProject(const Context& ctx, const ProjectorParams& pp, const ImageDevice& img,
+                 Histogram3DDevice& proj)
+{
+	OperatorProjectorMI op(ctx, pp);
+	op.applyA(img, proj);
+	ctx.synchronize();
+}
+
+void backProject(const Context& ctx, const ProjectorParams& pp, ImageDevice& img,
+                 const Histogram3DDevice& proj)
+{
+	OperatorProjectorMI op(ctx, pp);
+	op.applyAH(proj, img);
+	ctx.synchronize();
+}
+
+}  // namespace yrtmi

@@ -1,9 +1,11 @@
-// yrtmi C ABI: contexts, resident scanner tables, projectors and the entry
-// points of include/yrtmi.h.  Host code only; kernels live in
-// yrtmi_kernels.hip.
+// yrtmi C ABI: contexts, resident scanner tables, projectors, tile plans and
+// the entry points of include/yrtmi.h.  Host code only; kernels live in
+// yrtmi_kernels.hip and yrtmi_tiles.hip.
 #include "yrtmi.h"
 #include "yrtmi_kernels.hpp"
+#include "yrtmi_tiles.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -828,6 +830,573 @@ int yrtmi_count_segments(yrtmi_ctx* c, const yrtmi_proj* p,
 	                       c->stream));
 	HIP_TRY(hipStreamSynchronize(c->stream));
 	*total = (int64_t)h;
+	return YRTMI_OK;
+}
+
+}  // extern "C"
+
+struct yrtmi_plan
+{
+	int device = 0;
+	const yrtmi_proj* proj = nullptr;
+	yrtmi_image im{};
+	yrtmi_lors lors{};
+	int useTof = 0;
+	DTiles T{};
+	int64_t records = 0, nChunks = 0, bytes = 0;
+	uint32_t* key = nullptr;
+	float* alpha = nullptr;
+	uint32_t* vox = nullptr;
+	TileChunk* chunks = nullptr;
+	float* v = nullptr;  // per-LOR values of the em / sensitivity calls
+};
+
+namespace
+{
+
+void freePlan(yrtmi_plan* q)
+{
+	hipFree(q->key);
+	hipFree(q->alpha);
+	hipFree(q->vox);
+	hipFree(q->chunks);
+	hipFree(q->v);
+	delete q;
+}
+
+bool sameImage(const yrtmi_image& a, const yrtmi_image& b)
+{
+	return a.nx == b.nx && a.ny == b.ny && a.nz == b.nz && a.nt == b.nt &&
+	       a.vx == b.vx && a.vy == b.vy && a.vz == b.vz && a.lx == b.lx &&
+	       a.ly == b.ly && a.lz == b.lz && a.ox == b.ox && a.oy == b.oy &&
+	       a.oz == b.oz && a.fov_radius == b.fov_radius;
+}
+
+bool sameLors(const yrtmi_lors& a, const yrtmi_lors& b)
+{
+	return a.kind == b.kind && a.n == b.n && a.d1 == b.d1 && a.d2 == b.d2 &&
+	       a.bins == b.bins && a.num_subsets == b.num_subsets &&
+	       a.subset == b.subset && a.tof == b.tof && a.frame == b.frame &&
+	       a.motion_frame == b.motion_frame;
+}
+
+// Common checks of the *_planned calls.
+int checkPlan(const yrtmi_plan* q, const yrtmi_proj* p, const yrtmi_image* im,
+              const yrtmi_lors* l)
+{
+	if (q == nullptr)
+		return fail(YRTMI_EINVAL, "null plan");
+	if (q->proj != p)
+		return fail(YRTMI_EINVAL, "plan was made for another projector");
+	if (!sameImage(q->im, *im))
+		return fail(YRTMI_EINVAL, "plan was made for another image geometry");
+	if (!sameLors(q->lors, *l))
+		return fail(YRTMI_EINVAL, "plan was made for another LOR set");
+	return YRTMI_OK;
+}
+
+DPlanRead planRead(const yrtmi_plan* q)
+{
+	DPlanRead r{};
+	r.R = q->proj->d.numRays;
+	r.chunks = q->chunks;
+	r.key = q->key;
+	r.alpha = q->alpha;
+	r.vox = q->vox;
+	return r;
+}
+
+}  // namespace
+
+extern "C" {
+
+int yrtmi_plan_create(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
+                      const yrtmi_lors* l, int chunk_records, yrtmi_plan** out)
+{
+	if (out == nullptr)
+		return fail(YRTMI_EINVAL, "plan: null output");
+	*out = nullptr;
+	KArgs a;
+	int r = prepare(c, p, im, l, a);
+	if (r)
+		return r;
+	if (p->d.type != YRTMI_PROJ_SIDDON)
+		return fail(YRTMI_EINVAL, "plan: tile plans need the Siddon projector");
+	if (im->nx > 1024 || im->ny > 1024 || im->nz > 1024)
+		return fail(YRTMI_EINVAL, "plan: image dimensions above 1024");
+	const int R = p->d.numRays;
+	if ((l->n * (int64_t)R) >= (int64_t(1) << 32))
+		return fail(YRTMI_EINVAL, "plan: n * num_rays must be below 2^32");
+	if (chunk_records < 0 || chunk_records > (1 << 24))
+		return fail(YRTMI_EINVAL, "plan: chunk_records out of range");
+	const int CH = chunk_records > 0 ? chunk_records : 4096;
+	DTiles T{};
+	T.ntx = (int32_t)((im->nx + kTileX - 1) / kTileX);
+	T.nty = (int32_t)((im->ny + kTileY - 1) / kTileY);
+	T.ntz = (int32_t)((im->nz + kTileZ - 1) / kTileZ);
+	const int64_t nT = (int64_t)T.ntx * T.nty * T.ntz * im->nt;
+	if (nT > kMaxTiles)
+		return fail(YRTMI_EINVAL, "plan: more than 8192 image tiles (frames included)");
+	T.nTiles = (int32_t)nT;
+
+	yrtmi_plan* q = new yrtmi_plan();
+	q->device = c->device;
+	q->proj = p;
+	q->im = *im;
+	q->lors = *l;
+	q->useTof = p->d.useTof;
+	q->T = T;
+	if (l->n > 0)
+		if (hipMalloc(&q->v, sizeof(float) * l->n) != hipSuccess)
+		{
+			freePlan(q);
+			return fail(YRTMI_ENOMEM, "plan: value buffer");
+		}
+	q->bytes = sizeof(float) * l->n;
+
+	// scratch of the build
+	unsigned long long* dCount = nullptr;
+	unsigned int* dCursor = nullptr;
+	int64_t* dOff = nullptr;
+	int* dErr = nullptr;
+	auto cleanup = [&](int rc) {
+		hipFree(dCount);
+		hipFree(dCursor);
+		hipFree(dOff);
+		hipFree(dErr);
+		if (rc != YRTMI_OK)
+			freePlan(q);
+		else
+			*out = q;
+		return rc;
+	};
+	if (l->n == 0)
+		return cleanup(YRTMI_OK);
+	if (hipMalloc(&dCount, sizeof(unsigned long long) * nT) != hipSuccess ||
+	    hipMalloc(&dCursor, sizeof(unsigned int) * nT) != hipSuccess ||
+	    hipMalloc(&dOff, sizeof(int64_t) * nT) != hipSuccess ||
+	    hipMalloc(&dErr, sizeof(int)) != hipSuccess)
+		return cleanup(fail(YRTMI_ENOMEM, "plan: build scratch"));
+	hipError_t e = hipMemsetAsync(dCount, 0, sizeof(unsigned long long) * nT, c->stream);
+	if (e == hipSuccess)
+		e = hipMemsetAsync(dCursor, 0, sizeof(unsigned int) * nT, c->stream);
+	if (e == hipSuccess)
+		e = hipMemsetAsync(dErr, 0, sizeof(int), c->stream);
+	if (e != hipSuccess)
+		return cleanup(hipFail(e, "plan: clear"));
+
+	DPlan pl{};
+	pl.R = R;
+	pl.count = dCount;
+	pl.cursor = dCursor;
+	pl.tileOff = dOff;
+	pl.err = dErr;
+	const int64_t rays = l->n * (int64_t)R;
+	const int grid = (int)std::min<int64_t>((rays + 255) / 256, 2048);
+	e = launchBin(0, a, T, pl, grid, c->stream);
+	std::vector<unsigned long long> cnt(nT);
+	if (e == hipSuccess)
+		e = hipMemcpyAsync(cnt.data(), dCount, sizeof(unsigned long long) * nT,
+		                   hipMemcpyDeviceToHost, c->stream);
+	if (e == hipSuccess)
+		e = hipStreamSynchronize(c->stream);
+	if (e != hipSuccess)
+		return cleanup(hipFail(e, "plan: count pass"));
+
+	std::vector<int64_t> off(nT);
+	std::vector<TileChunk> chunks;
+	int64_t total = 0;
+	for (int64_t t = 0; t < nT; t++)
+	{
+		if (cnt[t] >= (1ull << 31))
+			return cleanup(fail(YRTMI_EINVAL, "plan: more than 2^31 records in one tile"));
+		off[t] = total;
+		for (int64_t b = 0; b < (int64_t)cnt[t]; b += CH)
+		{
+			TileChunk k;
+			k.begin = total + b;
+			k.tile = (int32_t)t;
+			k.count = (int32_t)std::min<int64_t>(CH, (int64_t)cnt[t] - b);
+			chunks.push_back(k);
+		}
+		total += (int64_t)cnt[t];
+	}
+	if ((int64_t)chunks.size() >= 0x7fffffffLL)
+		return cleanup(fail(YRTMI_EINVAL, "plan: too many chunks"));
+	q->records = total;
+	q->nChunks = (int64_t)chunks.size();
+	if (total > 0)
+	{
+		if (hipMalloc(&q->key, sizeof(uint32_t) * total) != hipSuccess ||
+		    hipMalloc(&q->alpha, sizeof(float) * total) != hipSuccess ||
+		    hipMalloc(&q->vox, sizeof(uint32_t) * total) != hipSuccess ||
+		    hipMalloc(&q->chunks, sizeof(TileChunk) * chunks.size()) != hipSuccess)
+			return cleanup(fail(YRTMI_ENOMEM, "plan: records"));
+		q->bytes += total * 12 + (int64_t)(sizeof(TileChunk) * chunks.size());
+		e = hipMemcpyAsync(dOff, off.data(), sizeof(int64_t) * nT,
+		                   hipMemcpyHostToDevice, c->stream);
+		if (e == hipSuccess)
+			e = hipMemcpyAsync(q->chunks, chunks.data(), sizeof(TileChunk) * chunks.size(),
+			                   hipMemcpyHostToDevice, c->stream);
+		pl.key = q->key;
+		pl.alpha = q->alpha;
+		pl.vox = q->vox;
+		if (e == hipSuccess)
+			e = launchBin(1, a, T, pl, grid, c->stream);
+		std::vector<unsigned int> cur(nT);
+		int herr = 0;
+		if (e == hipSuccess)
+			e = hipMemcpyAsync(cur.data(), dCursor, sizeof(unsigned int) * nT,
+			                   hipMemcpyDeviceToHost, c->stream);
+		if (e == hipSuccess)
+			e = hipMemcpyAsync(&herr, dErr, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+		if (e == hipSuccess)
+			e = hipStreamSynchronize(c->stream);
+		if (e != hipSuccess)
+			return cleanup(hipFail(e, "plan: fill pass"));
+		if (herr & 2)
+			return cleanup(fail(YRTMI_ESTATE, "plan: a ray's plane shift is out of range"));
+		bool same = herr == 0;
+		for (int64_t t = 0; t < nT && same; t++)
+			same = (unsigned long long)cur[t] == cnt[t];
+		if (!same)
+			return cleanup(fail(YRTMI_ESTATE, "plan: fill pass disagrees with count pass"));
+	}
+	return cleanup(YRTMI_OK);
+}
+
+int yrtmi_plan_destroy(yrtmi_plan* q)
+{
+	if (q == nullptr)
+		return YRTMI_OK;
+	hipSetDevice(q->device);
+	freePlan(q);
+	return YRTMI_OK;
+}
+
+int yrtmi_plan_info(const yrtmi_plan* q, int64_t* records, int64_t* chunks,
+                    int64_t* bytes)
+{
+	if (q == nullptr)
+		return fail(YRTMI_EINVAL, "null plan");
+	if (records)
+		*records = q->records;
+	if (chunks)
+		*chunks = q->nChunks;
+	if (bytes)
+		*bytes = q->bytes;
+	return YRTMI_OK;
+}
+
+int yrtmi_back_planned(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
+                       float* img, const yrtmi_lors* l, const float* vals,
+                       const yrtmi_plan* q)
+{
+	KArgs a;
+	int r = prepare(c, p, im, l, a);
+	if (r)
+		return r;
+	r = checkPlan(q, p, im, l);
+	if (r)
+		return r;
+	if (l->n == 0)
+		return YRTMI_OK;
+	if (img == nullptr || vals == nullptr)
+		return fail(YRTMI_EINVAL, "back: null image or values");
+	HIP_TRY(launchTileBack(a, q->T, planRead(q), q->nChunks, vals, img, c->stream));
+	return YRTMI_OK;
+}
+
+static void setCorr(KArgs& a, const yrtmi_corrections* corr)
+{
+	a.c.g = 1.0f;
+	a.c.denomThr = 1e-35f;
+	if (corr != nullptr)
+	{
+		a.c.meas = corr->meas;
+		a.c.sens = corr->sens;
+		a.c.att = corr->att;
+		a.c.randoms = corr->randoms;
+		a.c.scatter = corr->scatter;
+		a.c.g = corr->global_scale;
+		a.c.denomThr = corr->denom_threshold;
+	}
+}
+
+int yrtmi_em_ratio(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
+                   const float* x, const yrtmi_lors* l,
+                   const yrtmi_corrections* corr, float* v)
+{
+	KArgs a;
+	int r = prepare(c, p, im, l, a);
+	if (r)
+		return r;
+	if (p->d.type != YRTMI_PROJ_SIDDON)
+		return fail(YRTMI_EINVAL, "em_ratio: Siddon projector only");
+	if (l->n == 0)
+		return YRTMI_OK;
+	if (x == nullptr || v == nullptr)
+		return fail(YRTMI_EINVAL, "em_ratio: null image or output");
+	a.x = x;
+	setCorr(a, corr);
+	HIP_TRY(launchEmRatio(a, p->lpe, v, c->stream));
+	return YRTMI_OK;
+}
+
+int yrtmi_em_subset_planned(yrtmi_ctx* c, const yrtmi_proj* p,
+                            const yrtmi_image* im, const float* x, float* u,
+                            const yrtmi_lors* l, const yrtmi_corrections* corr,
+                            const yrtmi_plan* q)
+{
+	KArgs a;
+	int r = prepare(c, p, im, l, a);
+	if (r)
+		return r;
+	r = checkPlan(q, p, im, l);
+	if (r)
+		return r;
+	if (l->n == 0)
+		return YRTMI_OK;
+	if (x == nullptr || u == nullptr)
+		return fail(YRTMI_EINVAL, "em_subset: null image");
+	a.x = x;
+	setCorr(a, corr);
+	HIP_TRY(launchEmRatio(a, p->lpe, q->v, c->stream));
+	HIP_TRY(launchTileBack(a, q->T, planRead(q), q->nChunks, q->v, u, c->stream));
+	return YRTMI_OK;
+}
+
+int yrtmi_sensitivity_planned(yrtmi_ctx* c, const yrtmi_proj* p,
+                              const yrtmi_image* im, float* img,
+                              const yrtmi_lors* l, const yrtmi_image* att_im,
+                              const float* att_img, const float* sens,
+                              const yrtmi_plan* q)
+{
+	KArgs a;
+	int r = prepare(c, p, im, l, a);
+	if (r)
+		return r;
+	r = checkPlan(q, p, im, l);
+	if (r)
+		return r;
+	if (q->useTof)
+		return fail(YRTMI_EINVAL, "sensitivity: the plan's projector must have TOF off");
+	if (l->n == 0)
+		return YRTMI_OK;
+	if (img == nullptr)
+		return fail(YRTMI_EINVAL, "sensitivity: null image");
+	if (l->frame != nullptr || im->nt != 1)
+		return fail(YRTMI_EINVAL, "sensitivity: single-frame image and LORs only");
+	if (att_img != nullptr)
+	{
+		r = checkImage(att_im, "attenuation image");
+		if (r)
+			return r;
+		a.attIm = toDImage(att_im);
+		a.att = att_img;
+	}
+	a.vals = sens;
+	a.L.tof = nullptr;
+	HIP_TRY(launchSensValue(a, q->v, c->stream));
+	HIP_TRY(launchTileBack(a, q->T, planRead(q), q->nChunks, q->v, img, c->stream));
+	return YRTMI_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int yrtmi_device_alloc(yrtmi_ctx* c, int64_t bytes, void** out)
+{
+	int r = setDev(c);
+	if (r)
+		return r;
+	if (out == nullptr || bytes < 0)
+		return fail(YRTMI_EINVAL, "device_alloc: bad arguments");
+	*out = nullptr;
+	if (bytes == 0)
+		return YRTMI_OK;
+	if (hipMalloc(out, (size_t)bytes) != hipSuccess)
+	{
+		*out = nullptr;
+		return fail(YRTMI_ENOMEM, "device_alloc: out of device memory");
+	}
+	return YRTMI_OK;
+}
+
+int yrtmi_device_free(yrtmi_ctx* c, void* p)
+{
+	int r = setDev(c);
+	if (r)
+		return r;
+	if (p == nullptr)
+		return YRTMI_OK;
+	HIP_TRY(hipStreamSynchronize(c->stream));
+	HIP_TRY(hipFree(p));
+	return YRTMI_OK;
+}
+
+int yrtmi_copy(yrtmi_ctx* c, void* dst, const void* src, int64_t bytes, int kind)
+{
+	int r = setDev(c);
+	if (r)
+		return r;
+	if (bytes < 0 || (bytes > 0 && (dst == nullptr || src == nullptr)))
+		return fail(YRTMI_EINVAL, "copy: bad arguments");
+	if (bytes == 0)
+		return YRTMI_OK;
+	hipMemcpyKind k;
+	switch (kind)
+	{
+	case 0:
+		k = hipMemcpyHostToDevice;
+		break;
+	case 1:
+		k = hipMemcpyDeviceToHost;
+		break;
+	case 2:
+		k = hipMemcpyDeviceToDevice;
+		break;
+	default:
+		return fail(YRTMI_EINVAL, "copy: kind must be 0, 1 or 2");
+	}
+	HIP_TRY(hipMemcpyAsync(dst, src, (size_t)bytes, k, c->stream));
+	HIP_TRY(hipStreamSynchronize(c->stream));
+	return YRTMI_OK;
+}
+
+int yrtmi_histo_subset_nonzero(yrtmi_ctx* c, const float* values, int S, int s,
+                               uint32_t* bins, float* meas, int64_t* count)
+{
+	int r = setDev(c);
+	if (r)
+		return r;
+	if (!c->histoReady)
+		return fail(YRTMI_ESTATE, "histo_subset_nonzero: no yrtmi_histo_setup");
+	if (count == nullptr || values == nullptr || bins == nullptr || meas == nullptr)
+		return fail(YRTMI_EINVAL, "histo_subset_nonzero: null argument");
+	const int64_t n = yrtmi_histo_subset_size(c, S, s);
+	if (n < 0)
+		return fail(YRTMI_EINVAL, "histo_subset_nonzero: bad subset index");
+	*count = 0;
+	if (n == 0)
+		return YRTMI_OK;
+	SubsetGeom g{};
+	g.numPhi = c->numPhi;
+	g.numR = c->numR;
+	g.S = (uint32_t)S;
+	g.s = (uint32_t)s;
+	g.nps = c->numPhi / (uint32_t)S;
+	if ((uint32_t)s + g.nps * (uint32_t)S < c->numPhi)
+		g.nps += 1;
+	const int64_t nb = (n + kCompactPerBlock - 1) / kCompactPerBlock;
+	int* dCnt = nullptr;
+	int64_t* dOff = nullptr;
+	auto done = [&](int rc) {
+		hipFree(dCnt);
+		hipFree(dOff);
+		return rc;
+	};
+	if (hipMalloc(&dCnt, sizeof(int) * nb) != hipSuccess ||
+	    hipMalloc(&dOff, sizeof(int64_t) * nb) != hipSuccess)
+		return done(fail(YRTMI_ENOMEM, "histo_subset_nonzero: scratch"));
+	std::vector<int> hc(nb);
+	std::vector<int64_t> ho(nb);
+	hipError_t e = launchCountNonzero(values, g, n, dCnt, nb, c->stream);
+	if (e == hipSuccess)
+		e = hipMemcpyAsync(hc.data(), dCnt, sizeof(int) * nb, hipMemcpyDeviceToHost, c->stream);
+	if (e == hipSuccess)
+		e = hipStreamSynchronize(c->stream);
+	if (e != hipSuccess)
+		return done(hipFail(e, "histo_subset_nonzero: count"));
+	int64_t tot = 0;
+	for (int64_t b = 0; b < nb; b++)
+	{
+		ho[b] = tot;
+		tot += hc[b];
+	}
+	e = hipMemcpyAsync(dOff, ho.data(), sizeof(int64_t) * nb, hipMemcpyHostToDevice, c->stream);
+	if (e == hipSuccess)
+		e = launchWriteNonzero(values, g, n, dOff, nb, bins, meas, c->stream);
+	if (e == hipSuccess)
+		e = hipStreamSynchronize(c->stream);
+	if (e != hipSuccess)
+		return done(hipFail(e, "histo_subset_nonzero: write"));
+	*count = tot;
+	return done(YRTMI_OK);
+}
+
+int yrtmi_subset_interleaved(yrtmi_ctx* c, const void* src, int elem_bytes,
+                             int64_t n_total, int S, int s, void* dst, int64_t* count)
+{
+	int r = setDev(c);
+	if (r)
+		return r;
+	if (S <= 0 || s < 0 || s >= S || n_total < 0 ||
+	    (elem_bytes != 1 && elem_bytes != 2 && elem_bytes != 4 && elem_bytes != 8))
+		return fail(YRTMI_EINVAL, "subset_interleaved: bad arguments");
+	const int64_t n = (n_total > s) ? (n_total - s + S - 1) / S : 0;
+	if (count != nullptr)
+		*count = n;
+	if (n == 0)
+		return YRTMI_OK;
+	if (src == nullptr || dst == nullptr)
+		return fail(YRTMI_EINVAL, "subset_interleaved: null buffer");
+	HIP_TRY(launchInterleaved(src, elem_bytes, n, S, s, dst, c->stream));
+	return YRTMI_OK;
+}
+
+int yrtmi_gather(yrtmi_ctx* c, const float* src, int64_t src_n, const uint32_t* idx,
+                 int64_t n, int invert, float* out)
+{
+	int r = setDev(c);
+	if (r)
+		return r;
+	if (n < 0 || src_n < 0 || (n > 0 && (src == nullptr || idx == nullptr || out == nullptr)))
+		return fail(YRTMI_EINVAL, "gather: bad arguments");
+	if (n == 0)
+		return YRTMI_OK;
+	HIP_TRY(launchGather(src, src_n, idx, n, invert, out, c->stream));
+	return YRTMI_OK;
+}
+
+int yrtmi_histo_subset_gather(yrtmi_ctx* c, const float* h, int S, int s, int invert,
+                              float* out)
+{
+	int r = setDev(c);
+	if (r)
+		return r;
+	if (!c->histoReady)
+		return fail(YRTMI_ESTATE, "histo_subset_gather: no yrtmi_histo_setup");
+	const int64_t n = yrtmi_histo_subset_size(c, S, s);
+	if (n < 0)
+		return fail(YRTMI_EINVAL, "histo_subset_gather: bad subset index");
+	if (n == 0)
+		return YRTMI_OK;
+	if (h == nullptr || out == nullptr)
+		return fail(YRTMI_EINVAL, "histo_subset_gather: null buffer");
+	SubsetGeom g{};
+	g.numPhi = c->numPhi;
+	g.numR = c->numR;
+	g.S = (uint32_t)S;
+	g.s = (uint32_t)s;
+	g.nps = c->numPhi / (uint32_t)S;
+	if ((uint32_t)s + g.nps * (uint32_t)S < c->numPhi)
+		g.nps += 1;
+	HIP_TRY(launchSubsetGather(h, g, n, invert, out, c->stream));
+	return YRTMI_OK;
+}
+
+int yrtmi_axpby(yrtmi_ctx* c, int64_t n, float a, const float* x, float b, float* y)
+{
+	int r = setDev(c);
+	if (r)
+		return r;
+	if (n < 0 || (n > 0 && y == nullptr))
+		return fail(YRTMI_EINVAL, "axpby: bad arguments");
+	if (n == 0)
+		return YRTMI_OK;
+	HIP_TRY(launchAxpby(n, a, x, b, y, c->stream));
 	return YRTMI_OK;
 }
 

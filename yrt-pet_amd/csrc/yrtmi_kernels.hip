@@ -421,6 +421,193 @@ hipError_t launchFoldPartials(float* y, float* part, int64_t stride, int nk,
 	return hipGetLastError();
 }
 
+// Element i of histogram subset s of S (BinIteratorRangeHistogram3D::get,
+// BinIterator.cpp:191-197): r fastest, then the subset's phis, then z bins.
+__device__ __forceinline__ uint32_t subsetBin(uint32_t i, const SubsetGeom& g)
+{
+	const uint32_t per = g.nps * g.numR;
+	const uint32_t zb = i / per;
+	const uint32_t rem = i - zb * per;
+	const uint32_t ps = rem / g.numR;
+	const uint32_t r = rem - ps * g.numR;
+	return (zb * g.numPhi + g.S * ps + g.s) * g.numR + r;
+}
+
+__global__ void __launch_bounds__(256) kCountNonzero(const float* __restrict__ v,
+                                                     SubsetGeom g, int64_t n,
+                                                     int* __restrict__ blockCount)
+{
+	__shared__ int wc[4];
+	int c = 0;
+	for (int j = 0; j < kCompactRounds; j++)
+	{
+		const int64_t i = (int64_t)blockIdx.x * kCompactPerBlock + j * 256 + threadIdx.x;
+		if (i < n && v[subsetBin((uint32_t)i, g)] != 0.f)
+			c++;
+	}
+#pragma unroll
+	for (int o = 32; o >= 1; o >>= 1)
+		c += __shfl_xor(c, o);
+	if ((threadIdx.x & 63) == 0)
+		wc[threadIdx.x >> 6] = c;
+	__syncthreads();
+	if (threadIdx.x == 0)
+		blockCount[blockIdx.x] = wc[0] + wc[1] + wc[2] + wc[3];
+}
+
+__global__ void __launch_bounds__(256) kWriteNonzero(const float* __restrict__ v,
+                                                     SubsetGeom g, int64_t n,
+                                                     const int64_t* __restrict__ blockOff,
+                                                     uint32_t* __restrict__ bins,
+                                                     float* __restrict__ meas)
+{
+	__shared__ int wc[4];
+	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+	int64_t run = blockOff[blockIdx.x];
+	for (int j = 0; j < kCompactRounds; j++)
+	{
+		const int64_t i = (int64_t)blockIdx.x * kCompactPerBlock + j * 256 + threadIdx.x;
+		uint32_t bin = 0;
+		float val = 0.f;
+		if (i < n)
+		{
+			bin = subsetBin((uint32_t)i, g);
+			val = v[bin];
+		}
+		const bool keep = val != 0.f;
+		const unsigned long long m = __ballot(keep);
+		if (lane == 0)
+			wc[w] = __popcll(m);
+		__syncthreads();
+		int before = 0, total = 0;
+#pragma unroll
+		for (int k = 0; k < 4; k++)
+		{
+			before += (k < w) ? wc[k] : 0;
+			total += wc[k];
+		}
+		if (keep)
+		{
+			const int64_t o = run + before + __popcll(m & ((1ull << lane) - 1ull));
+			bins[o] = bin;
+			meas[o] = val;
+		}
+		run += total;
+		__syncthreads();
+	}
+}
+
+hipError_t launchCountNonzero(const float* v, const SubsetGeom& g, int64_t n,
+                              int* blockCount, int64_t nBlocks, hipStream_t st)
+{
+	kCountNonzero<<<(unsigned)nBlocks, 256, 0, st>>>(v, g, n, blockCount);
+	return hipGetLastError();
+}
+
+hipError_t launchWriteNonzero(const float* v, const SubsetGeom& g, int64_t n,
+                              const int64_t* blockOff, int64_t nBlocks,
+                              uint32_t* bins, float* meas, hipStream_t st)
+{
+	kWriteNonzero<<<(unsigned)nBlocks, 256, 0, st>>>(v, g, n, blockOff, bins, meas);
+	return hipGetLastError();
+}
+
+__device__ __forceinline__ float maybeInvert(float v, int invert)
+{
+	return (invert && v != 0.0f) ? 1.0f / v : v;
+}
+
+__global__ void __launch_bounds__(256) kGather(const float* __restrict__ src,
+                                               int64_t srcN,
+                                               const uint32_t* __restrict__ idx,
+                                               int64_t n, int invert,
+                                               float* __restrict__ out)
+{
+	const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+	for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+	     i += stride)
+	{
+		const uint32_t k = idx[i];
+		out[i] = maybeInvert(((int64_t)k < srcN) ? src[k] : 0.f, invert);
+	}
+}
+
+hipError_t launchGather(const float* src, int64_t srcN, const uint32_t* idx,
+                        int64_t n, int invert, float* out, hipStream_t st)
+{
+	kGather<<<gridFor(n), 256, 0, st>>>(src, srcN, idx, n, invert, out);
+	return hipGetLastError();
+}
+
+__global__ void __launch_bounds__(256) kSubsetGather(const float* __restrict__ h,
+                                                     SubsetGeom g, int64_t n,
+                                                     int invert,
+                                                     float* __restrict__ out)
+{
+	const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+	for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+	     i += stride)
+		out[i] = maybeInvert(h[subsetBin((uint32_t)i, g)], invert);
+}
+
+hipError_t launchSubsetGather(const float* h, const SubsetGeom& g, int64_t n,
+                              int invert, float* out, hipStream_t st)
+{
+	kSubsetGather<<<gridFor(n), 256, 0, st>>>(h, g, n, invert, out);
+	return hipGetLastError();
+}
+
+// x and y may alias.
+__global__ void __launch_bounds__(256) kAxpby(int64_t n, float a, const float* x,
+                                              float b, float* y)
+{
+	const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+	for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+	     i += stride)
+		y[i] = (x != nullptr) ? a * x[i] + b * y[i] : b * y[i];
+}
+
+hipError_t launchAxpby(int64_t n, float a, const float* x, float b, float* y,
+                       hipStream_t st)
+{
+	kAxpby<<<gridFor(n), 256, 0, st>>>(n, a, x, b, y);
+	return hipGetLastError();
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) kInterleaved(const T* __restrict__ src,
+                                                    int64_t n, int64_t S, int64_t s,
+                                                    T* __restrict__ dst)
+{
+	const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+	for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+	     i += stride)
+		dst[i] = src[s + i * S];
+}
+
+hipError_t launchInterleaved(const void* src, int elemBytes, int64_t n, int64_t S,
+                             int64_t s, void* dst, hipStream_t st)
+{
+	switch (elemBytes)
+	{
+	case 1:
+		kInterleaved<<<gridFor(n), 256, 0, st>>>((const uint8_t*)src, n, S, s, (uint8_t*)dst);
+		break;
+	case 2:
+		kInterleaved<<<gridFor(n), 256, 0, st>>>((const uint16_t*)src, n, S, s, (uint16_t*)dst);
+		break;
+	case 4:
+		kInterleaved<<<gridFor(n), 256, 0, st>>>((const uint32_t*)src, n, S, s, (uint32_t*)dst);
+		break;
+	case 8:
+		kInterleaved<<<gridFor(n), 256, 0, st>>>((const uint64_t*)src, n, S, s, (uint64_t*)dst);
+		break;
+	default:
+		return hipErrorInvalidValue;
+	}
+	return hipGetLastError();
+}
+
 hipError_t launchFill(float* x, int64_t n, float v, hipStream_t st)
 {
 	kFill<<<gridFor(n), 256, 0, st>>>(x, n, v);

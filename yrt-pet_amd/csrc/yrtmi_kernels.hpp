@@ -19,6 +19,30 @@ hipError_t launchThreshold(float* x, const float* m, int64_t nvox, int64_t nt,
                            int64_t mFrames, float thr, float les, float leo,
                            float gts, float gto, hipStream_t st);
 hipError_t launchFill(float* x, int64_t n, float v, hipStream_t st);
+
+// Histogram subset geometry for the bin-list kernels.
+struct SubsetGeom
+{
+	uint32_t numPhi, numR, nps, S, s;
+};
+constexpr int kCompactRounds = 8;
+constexpr int kCompactPerBlock = 256 * kCompactRounds;
+// Nonzero bins of a histogram subset, in subset order: per-block counts,
+// then (after an exclusive scan of the counts) the writes.
+hipError_t launchCountNonzero(const float* v, const SubsetGeom& g, int64_t n,
+                              int* blockCount, int64_t nBlocks, hipStream_t st);
+hipError_t launchWriteNonzero(const float* v, const SubsetGeom& g, int64_t n,
+                              const int64_t* blockOff, int64_t nBlocks,
+                              uint32_t* bins, float* meas, hipStream_t st);
+// dst[i] = src[s + i*S], i < n, elements of elemBytes bytes.
+hipError_t launchInterleaved(const void* src, int elemBytes, int64_t n, int64_t S,
+                             int64_t s, void* dst, hipStream_t st);
+hipError_t launchGather(const float* src, int64_t srcN, const uint32_t* idx,
+                        int64_t n, int invert, float* out, hipStream_t st);
+hipError_t launchSubsetGather(const float* h, const SubsetGeom& g, int64_t n,
+                              int invert, float* out, hipStream_t st);
+hipError_t launchAxpby(int64_t n, float a, const float* x, float b, float* y,
+                       hipStream_t st);
 // y += sum of the nk per-XCD partial images (row stride `stride`), which are
 // left zeroed.
 hipError_t launchFoldPartials(float* y, float* part, int64_t stride, int nk,

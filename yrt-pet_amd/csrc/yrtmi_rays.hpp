@@ -48,37 +48,59 @@ __device__ __forceinline__ void getAlpha(T r0, T r1, T p1, T p2, T inv, T& amin,
 	}
 }
 
-// Siddon ray (ProjectorSiddon::project_helper<IS_FWD, FLAG_INCR=true,
-// FLAG_TOF>, ProjectorSiddon.cpp:478-767).  `img` points at the frame's
-// voxels.  MODE_FWD returns the line integral; MODE_BACK adds val*w
-// atomically; MODE_COUNT counts applied segments.
-template <int MODE, bool TOF, bool S64>
-__device__ __forceinline__ float siddonRay(const DImage& P,
-                                           float* __restrict__ img, float3 p1,
-                                           float3 p2, float val,
-                                           const DTof& tof, float tofv,
-                                           uint32_t& cnt)
+// Siddon ray after its setup: the clipped alpha range, the next plane
+// crossing of each axis and what the walk needs to advance them
+// (ProjectorSiddon::project_helper<IS_FWD, FLAG_INCR=true, FLAG_TOF>,
+// ProjectorSiddon.cpp:478-767).
+struct SidRay
+{
+	float3 p1, p2;
+	float invx, invy, invz;
+	float a_cur, amax;
+	float axn, ayn, azn;  // alpha of the next plane per axis (FMAX: flat)
+	float xc, yc, zc;     // coordinate of that plane
+	float dnorm;
+	float tA, tB;  // TOF: S64 x_c = a_mid*tA - tB; S32 pc = tB
+	int dirx, diry, dirz;
+	// index of the first plane per axis, counted from the image face the
+	// ray enters through (x_start + k * dir * v)
+	int kx, ky, kz;
+	// first voxel (S64 only)
+	double x1d, y1d, z1d, ddx, ddy, ddz;
+};
+
+// Ray-plane setup: FOV-cylinder clip, image slabs, TOF window, first plane
+// crossing of each axis.  Returns false when the ray misses the image.
+template <bool TOF, bool S64>
+__device__ __forceinline__ bool sidSetup(const DImage& P, float3 p1, float3 p2,
+                                         const DTof& tof, float tofv, SidRay& r)
 {
 #pragma clang fp contract(off)
 	const float FMAX = 3.402823466e+38f;
 	const bool flat_x = (p1.x == p2.x);
 	const bool flat_y = (p1.y == p2.y);
 	const bool flat_z = (p1.z == p2.z);
-	int dir_x, dir_y, dir_z;
-	float invxf, invyf, invzf;
-	float a_cur, amaxf, axn = FMAX, ayn = FMAX, azn = FMAX;
-	float x_cur, y_cur, z_cur, dnorm;
-	float tA = 0.f, tB = 0.f;  // TOF: S64 x_c = a_mid*tA - tB; S32 pc = tB
-	// first-voxel data (S64 only)
-	double x1d = 0, y1d = 0, z1d = 0, ddx = 0, ddy = 0, ddz = 0;
+	r.p1 = p1;
+	r.p2 = p2;
+	r.axn = FMAX;
+	r.ayn = FMAX;
+	r.azn = FMAX;
+	r.tA = 0.f;
+	r.tB = 0.f;
+	r.kx = r.ky = r.kz = 0;
+	r.x1d = r.y1d = r.z1d = r.ddx = r.ddy = r.ddz = 0.0;
 	if constexpr (S64)
 	{
-		x1d = p1.x;
-		y1d = p1.y;
-		z1d = p1.z;
-		ddx = (double)p2.x - x1d;
-		ddy = (double)p2.y - y1d;
-		ddz = (double)p2.z - z1d;
+		const double x1d = p1.x, y1d = p1.y, z1d = p1.z;
+		const double ddx = (double)p2.x - x1d;
+		const double ddy = (double)p2.y - y1d;
+		const double ddz = (double)p2.z - z1d;
+		r.x1d = x1d;
+		r.y1d = y1d;
+		r.z1d = z1d;
+		r.ddx = ddx;
+		r.ddy = ddy;
+		r.ddz = ddz;
 		double amin = 0.0, amax = 1.0;
 		const double A = ddx * ddx + ddy * ddy;
 		if (A != 0.0)
@@ -88,7 +110,7 @@ __device__ __forceinline__ float siddonRay(const DImage& P,
 			const double Cq = x1d * x1d + y1d * y1d - R * R;
 			const double Delta = B * B - 4.0 * A * Cq;
 			if (Delta <= 0.0)
-				return 0.f;
+				return false;
 			const double sq = sqrt(Delta);
 			amin = fmax(amin, (-B - sq) / (2.0 * A));
 			amax = fmin(amax, (-B + sq) / (2.0 * A));
@@ -97,9 +119,9 @@ __device__ __forceinline__ float siddonRay(const DImage& P,
 		const double invx = flat_x ? 0.0 : 1.0 / ddx;
 		const double invy = flat_y ? 0.0 : 1.0 / ddy;
 		const double invz = flat_z ? 0.0 : 1.0 / ddz;
-		dir_x = (invx >= 0.0) ? 1 : -1;
-		dir_y = (invy >= 0.0) ? 1 : -1;
-		dir_z = (invz >= 0.0) ? 1 : -1;
+		r.dirx = (invx >= 0.0) ? 1 : -1;
+		r.diry = (invy >= 0.0) ? 1 : -1;
+		r.dirz = (invz >= 0.0) ? 1 : -1;
 		const double hx = 0.5 * (double)P.lx, hy = 0.5 * (double)P.ly,
 		             hz = 0.5 * (double)P.lz;
 		// util::get_alpha (ProjectorUtils.hpp:15-41)
@@ -110,7 +132,7 @@ __device__ __forceinline__ float siddonRay(const DImage& P,
 			amax = fmin(amax, fmax(a0, a1));
 		}
 		else if (x1d < -hx || x1d > hx)
-			return 0.f;
+			return false;
 		if (!flat_y)
 		{
 			const double a0 = (-hy - y1d) * invy, a1 = (hy - y1d) * invy;
@@ -118,7 +140,7 @@ __device__ __forceinline__ float siddonRay(const DImage& P,
 			amax = fmin(amax, fmax(a0, a1));
 		}
 		else if (y1d < -hy || y1d > hy)
-			return 0.f;
+			return false;
 		if (!flat_z)
 		{
 			const double a0 = (-hz - z1d) * invz, a1 = (hz - z1d) * invz;
@@ -126,7 +148,7 @@ __device__ __forceinline__ float siddonRay(const DImage& P,
 			amax = fmin(amax, fmax(a0, a1));
 		}
 		else if (z1d < -hz || z1d > hz)
-			return 0.f;
+			return false;
 		if (TOF)
 		{
 			const double tmm = (double)tofMm(tofv);
@@ -135,45 +157,48 @@ __device__ __forceinline__ float siddonRay(const DImage& P,
 				amin = fmax(amin, 0.5 + (tmm - (double)tof.trunc) / dnormd);
 				amax = fmin(amax, 0.5 + (tmm + (double)tof.trunc) / dnormd);
 			}
-			tB = (float)((0.5 * dnormd + tmm) / (double)tof.sigma);
-			tA = (float)(dnormd / (double)tof.sigma);
+			r.tB = (float)((0.5 * dnormd + tmm) / (double)tof.sigma);
+			r.tA = (float)(dnormd / (double)tof.sigma);
 		}
 		if ((invx >= 0.0 && x1d > hx) || (invx < 0.0 && x1d < -hx) ||
 		    (invy >= 0.0 && y1d > hy) || (invy < 0.0 && y1d < -hy) ||
 		    (invz >= 0.0 && z1d > hz) || (invz < 0.0 && z1d < -hz))
-			return 0.f;
+			return false;
 		if (!(amin < amax))
-			return 0.f;
+			return false;
 		double xc = (invx > 0.0) ? -hx : hx;
 		double yc = (invy > 0.0) ? -hy : hy;
 		double zc = (invz > 0.0) ? -hz : hz;
 		if (!flat_x)
 		{
-			xc += ceil((double)dir_x * (amin * ddx - xc + x1d) / (double)P.vx) *
-			      dir_x * (double)P.vx;
-			axn = (float)((xc - x1d) * invx);
+			const double k = ceil((double)r.dirx * (amin * ddx - xc + x1d) / (double)P.vx);
+			xc += k * r.dirx * (double)P.vx;
+			r.axn = (float)((xc - x1d) * invx);
+			r.kx = (int)k;
 		}
 		if (!flat_y)
 		{
-			yc += ceil((double)dir_y * (amin * ddy - yc + y1d) / (double)P.vy) *
-			      dir_y * (double)P.vy;
-			ayn = (float)((yc - y1d) * invy);
+			const double k = ceil((double)r.diry * (amin * ddy - yc + y1d) / (double)P.vy);
+			yc += k * r.diry * (double)P.vy;
+			r.ayn = (float)((yc - y1d) * invy);
+			r.ky = (int)k;
 		}
 		if (!flat_z)
 		{
-			zc += ceil((double)dir_z * (amin * ddz - zc + z1d) / (double)P.vz) *
-			      dir_z * (double)P.vz;
-			azn = (float)((zc - z1d) * invz);
+			const double k = ceil((double)r.dirz * (amin * ddz - zc + z1d) / (double)P.vz);
+			zc += k * r.dirz * (double)P.vz;
+			r.azn = (float)((zc - z1d) * invz);
+			r.kz = (int)k;
 		}
-		x_cur = (float)xc;
-		y_cur = (float)yc;
-		z_cur = (float)zc;
-		invxf = (float)invx;
-		invyf = (float)invy;
-		invzf = (float)invz;
-		dnorm = (float)dnormd;
-		amaxf = (float)amax;
-		a_cur = (float)amin;
+		r.xc = (float)xc;
+		r.yc = (float)yc;
+		r.zc = (float)zc;
+		r.invx = (float)invx;
+		r.invy = (float)invy;
+		r.invz = (float)invz;
+		r.dnorm = (float)dnormd;
+		r.amax = (float)amax;
+		r.a_cur = (float)amin;
 	}
 	else
 	{
@@ -186,125 +211,163 @@ __device__ __forceinline__ float siddonRay(const DImage& P,
 		if (A != 0.f)
 		{
 			if (Delta <= 0.f)
-				return 0.f;
+				return false;
 			t0 = (-B - sqrtf(Delta)) / (2.0f * A);
 			t1 = (-B + sqrtf(Delta)) / (2.0f * A);
 		}
 		const float ex = p1.x - p2.x, ey = p1.y - p2.y, ez = p1.z - p2.z;
-		dnorm = sqrtf(ex * ex + ey * ey + ez * ez);
-		invxf = flat_x ? 0.0f : 1.0f / (p2.x - p1.x);
-		invyf = flat_y ? 0.0f : 1.0f / (p2.y - p1.y);
-		invzf = flat_z ? 0.0f : 1.0f / (p2.z - p1.z);
-		dir_x = (invxf >= 0.0f) ? 1 : -1;
-		dir_y = (invyf >= 0.0f) ? 1 : -1;
-		dir_z = (invzf >= 0.0f) ? 1 : -1;
+		r.dnorm = sqrtf(ex * ex + ey * ey + ez * ez);
+		r.invx = flat_x ? 0.0f : 1.0f / (p2.x - p1.x);
+		r.invy = flat_y ? 0.0f : 1.0f / (p2.y - p1.y);
+		r.invz = flat_z ? 0.0f : 1.0f / (p2.z - p1.z);
+		r.dirx = (r.invx >= 0.0f) ? 1 : -1;
+		r.diry = (r.invy >= 0.0f) ? 1 : -1;
+		r.dirz = (r.invz >= 0.0f) ? 1 : -1;
 		const float x0 = -P.lx * 0.5f, x1 = P.lx * 0.5f;
 		const float y0 = -P.ly * 0.5f, y1 = P.ly * 0.5f;
 		const float z0 = -P.lz * 0.5f, z1 = P.lz * 0.5f;
 		float amin = fmaxf(0.0f, t0), amax = fminf(1.0f, t1);
 		if (!flat_x)
 		{
-			const float a0 = (-0.5f * P.lx - p1.x) * invxf, a1 = (0.5f * P.lx - p1.x) * invxf;
+			const float a0 = (-0.5f * P.lx - p1.x) * r.invx, a1 = (0.5f * P.lx - p1.x) * r.invx;
 			amin = fmaxf(amin, (a0 < a1) ? a0 : a1);
 			amax = fminf(amax, (a0 < a1) ? a1 : a0);
 		}
 		else if (p1.x < -0.5f * P.lx || p1.x > 0.5f * P.lx)
-			return 0.f;
+			return false;
 		if (!flat_y)
 		{
-			const float a0 = (-0.5f * P.ly - p1.y) * invyf, a1 = (0.5f * P.ly - p1.y) * invyf;
+			const float a0 = (-0.5f * P.ly - p1.y) * r.invy, a1 = (0.5f * P.ly - p1.y) * r.invy;
 			amin = fmaxf(amin, (a0 < a1) ? a0 : a1);
 			amax = fminf(amax, (a0 < a1) ? a1 : a0);
 		}
 		else if (p1.y < -0.5f * P.ly || p1.y > 0.5f * P.ly)
-			return 0.f;
+			return false;
 		if (!flat_z)
 		{
-			const float a0 = (-0.5f * P.lz - p1.z) * invzf, a1 = (0.5f * P.lz - p1.z) * invzf;
+			const float a0 = (-0.5f * P.lz - p1.z) * r.invz, a1 = (0.5f * P.lz - p1.z) * r.invz;
 			amin = fmaxf(amin, (a0 < a1) ? a0 : a1);
 			amax = fminf(amax, (a0 < a1) ? a1 : a0);
 		}
 		else if (p1.z < -0.5f * P.lz || p1.z > 0.5f * P.lz)
-			return 0.f;
+			return false;
 		if (TOF)
 		{
 			// TimeOfFlightHelper::getAlphaRange (TimeOfFlight.hpp:153-173)
 			const float tmm = tofMm(tofv);
 			if (tof.trunc > 0.f)
 			{
-				const float lo = (float)fmax(0.0, 0.5 + (double)((tmm - tof.trunc) / dnorm));
-				const float hi = (float)fmin(1.0, 0.5 + (double)((tmm + tof.trunc) / dnorm));
+				const float lo = (float)fmax(0.0, 0.5 + (double)((tmm - tof.trunc) / r.dnorm));
+				const float hi = (float)fmin(1.0, 0.5 + (double)((tmm + tof.trunc) / r.dnorm));
 				amin = fmaxf(amin, lo);
 				amax = fminf(amax, hi);
 			}
-			tB = (float)(0.5 * (double)dnorm + (double)tmm);  // pc
+			r.tB = (float)(0.5 * (double)r.dnorm + (double)tmm);  // pc
 		}
-		x_cur = (invxf > 0.0f) ? x0 : x1;
-		y_cur = (invyf > 0.0f) ? y0 : y1;
-		z_cur = (invzf > 0.0f) ? z0 : z1;
-		if ((invxf >= 0.0f && p1.x > x1) || (invxf < 0.0f && p1.x < x0) ||
-		    (invyf >= 0.0f && p1.y > y1) || (invyf < 0.0f && p1.y < y0) ||
-		    (invzf >= 0.0f && p1.z > z1) || (invzf < 0.0f && p1.z < z0))
-			return 0.f;
+		r.xc = (r.invx > 0.0f) ? x0 : x1;
+		r.yc = (r.invy > 0.0f) ? y0 : y1;
+		r.zc = (r.invz > 0.0f) ? z0 : z1;
+		if ((r.invx >= 0.0f && p1.x > x1) || (r.invx < 0.0f && p1.x < x0) ||
+		    (r.invy >= 0.0f && p1.y > y1) || (r.invy < 0.0f && p1.y < y0) ||
+		    (r.invz >= 0.0f && p1.z > z1) || (r.invz < 0.0f && p1.z < z0))
+			return false;
 		if (!(amin < amax))
-			return 0.f;
-		a_cur = amin;
-		amaxf = amax;
+			return false;
+		r.a_cur = amin;
+		r.amax = amax;
 		if (!flat_x)
 		{
-			const float kx = ceilf((float)dir_x * (a_cur * (p2.x - p1.x) - x_cur + p1.x) / P.vx);
-			x_cur += (float)((int)kx * dir_x) * P.vx;
-			axn = (x_cur - p1.x) * invxf;
+			const float kx = ceilf((float)r.dirx * (amin * (p2.x - p1.x) - r.xc + p1.x) / P.vx);
+			r.xc += (float)((int)kx * r.dirx) * P.vx;
+			r.axn = (r.xc - p1.x) * r.invx;
+			r.kx = (int)kx;
 		}
 		if (!flat_y)
 		{
-			const float ky = ceilf((float)dir_y * (a_cur * (p2.y - p1.y) - y_cur + p1.y) / P.vy);
-			y_cur += (float)((int)ky * dir_y) * P.vy;
-			ayn = (y_cur - p1.y) * invyf;
+			const float ky = ceilf((float)r.diry * (amin * (p2.y - p1.y) - r.yc + p1.y) / P.vy);
+			r.yc += (float)((int)ky * r.diry) * P.vy;
+			r.ayn = (r.yc - p1.y) * r.invy;
+			r.ky = (int)ky;
 		}
 		if (!flat_z)
 		{
-			const float kz = ceilf((float)dir_z * (a_cur * (p2.z - p1.z) - z_cur + p1.z) / P.vz);
-			z_cur += (float)((int)kz * dir_z) * P.vz;
-			azn = (z_cur - p1.z) * invzf;
+			const float kz = ceilf((float)r.dirz * (amin * (p2.z - p1.z) - r.zc + p1.z) / P.vz);
+			r.zc += (float)((int)kz * r.dirz) * P.vz;
+			r.azn = (r.zc - p1.z) * r.invz;
+			r.kz = (int)kz;
 		}
 	}
-	// ---- FP32 traversal (622-766) ----
-	const float sdx = (float)dir_x * P.vx, sdy = (float)dir_y * P.vy,
-	            sdz = (float)dir_z * P.vz;
-	float axp = axn, ayp = ayn, azp = azn;
-	int vx = 0, vy = 0, vz = 0;
-	int off = 0;
-	int dprev = 0;
-	bool first = true;
-	float acc = 0.f;
-	while (a_cur < amaxf)
+	return true;
+}
+
+// The voxel holding the midpoint of segment [a0, a1] (the walk's rule for
+// its first voxel, ProjectorSiddon.cpp:669-683).
+template <bool S64>
+__device__ __forceinline__ void sidMidVoxel(const DImage& P, const SidRay& r, float a0,
+                                            float a1, int& vx, int& vy, int& vz)
+{
+#pragma clang fp contract(off)
+	if constexpr (S64)
 	{
-		int dn = 0;
+		const double am = 0.5 * ((double)a0 + (double)a1);
+		vx = (int)((r.x1d + am * r.ddx + 0.5 * (double)P.lx) / (double)P.vx);
+		vy = (int)((r.y1d + am * r.ddy + 0.5 * (double)P.ly) / (double)P.vy);
+		vz = (int)((r.z1d + am * r.ddz + 0.5 * (double)P.lz) / (double)P.vz);
+	}
+	else
+	{
+		const float am = 0.5f * (a0 + a1);
+		vx = (int)((r.p1.x + am * (r.p2.x - r.p1.x) + P.lx / 2.0f) * (1.0f / P.vx));
+		vy = (int)((r.p1.y + am * (r.p2.y - r.p1.y) + P.ly / 2.0f) * (1.0f / P.vy));
+		vz = (int)((r.p1.z + am * (r.p2.z - r.p1.z) + P.lz / 2.0f) * (1.0f / P.vz));
+	}
+}
+
+// Plane-index shift of a ray: the walk numbers planes from its first plane
+// and moves its voxel index by one per plane crossed, so along the whole ray
+// (walk's next plane index) - (index of the voxel's far face) is a constant
+// per axis.  It is 0 except where the first voxel, taken at the midpoint of
+// a first segment an ulp long at a plane corner, lands one voxel over; the
+// walk then carries that offset to the end of the ray.  Computed by running
+// the walk's selection to its first nonempty segment.  Returns false when
+// the ray has no segment.
+template <bool S64>
+__device__ __forceinline__ bool sidPlaneShift(const DImage& P, const SidRay& r, int& dx,
+                                              int& dy, int& dz)
+{
+#pragma clang fp contract(off)
+	const float sdx = (float)r.dirx * P.vx, sdy = (float)r.diry * P.vy,
+	            sdz = (float)r.dirz * P.vz;
+	float axn = r.axn, ayn = r.ayn, azn = r.azn, xc = r.xc, yc = r.yc, zc = r.zc;
+	float axp = axn, ayp = ayn, azp = azn;
+	int sx = 0, sy = 0, sz = 0;  // planes consumed before the first segment
+	const float a_cur = r.a_cur;
+	while (a_cur < r.amax)
+	{
 		float a_next = -1.f;
 		if (axp <= ayp && axp <= azp)
 		{
 			a_next = axn;
-			x_cur += sdx;
-			axn = (x_cur - p1.x) * invxf;
-			dn |= 1;
+			xc += sdx;
+			axn = (xc - r.p1.x) * r.invx;
+			sx++;
 		}
 		if (ayp <= axp && ayp <= azp)
 		{
 			a_next = ayn;
-			y_cur += sdy;
-			ayn = (y_cur - p1.y) * invyf;
-			dn |= 2;
+			yc += sdy;
+			ayn = (yc - r.p1.y) * r.invy;
+			sy++;
 		}
 		if (azp <= axp && azp <= ayp)
 		{
 			a_next = azn;
-			z_cur += sdz;
-			azn = (z_cur - p1.z) * invzf;
-			dn |= 4;
+			zc += sdz;
+			azn = (zc - r.p1.z) * r.invz;
+			sz++;
 		}
-		if (a_next > amaxf)
-			a_next = amaxf;
+		if (a_next > r.amax)
+			a_next = r.amax;
 		if (a_cur >= a_next)
 		{
 			axp = axn;
@@ -312,74 +375,242 @@ __device__ __forceinline__ float siddonRay(const DImage& P,
 			azp = azn;
 			continue;
 		}
+		// planes advanced in this (first nonempty) iteration belong to its
+		// end, not to its voxel
+		if (axp <= ayp && axp <= azp)
+			sx--;
+		if (ayp <= axp && ayp <= azp)
+			sy--;
+		if (azp <= axp && azp <= ayp)
+			sz--;
+		int vx, vy, vz;
+		sidMidVoxel<S64>(P, r, a_cur, a_next, vx, vy, vz);
+		// far face of voxel v: k = v + 1 (dir > 0) or n - v (dir < 0);
+		// a flat axis has no planes
+		dx = (r.p1.x == r.p2.x) ? 0 : r.kx + sx - ((r.dirx > 0) ? vx + 1 : P.nx - vx);
+		dy = (r.p1.y == r.p2.y) ? 0 : r.ky + sy - ((r.diry > 0) ? vy + 1 : P.ny - vy);
+		dz = (r.p1.z == r.p2.z) ? 0 : r.kz + sz - ((r.dirz > 0) ? vz + 1 : P.nz - vz);
+		return true;
+	}
+	return false;
+}
+
+// Re-enters the walk of `r` (fresh from sidSetup) at alpha `a0` inside voxel
+// (vx, vy, vz), whose segment starts there.  The next plane of each axis is
+// that voxel's far face in the direction of travel plus the ray's plane
+// shift (sidPlaneShift), counted from the entry face of the image as the
+// walk counts it (x_start + k * dir * v).  On voxel sizes that are sums of
+// few powers of two this reproduces the walk's accumulated plane positions
+// exactly; otherwise to an ulp.
+template <bool S64>
+__device__ __forceinline__ void sidRestart(const DImage& P, SidRay& r, float a0,
+                                           int vx, int vy, int vz, int dx, int dy,
+                                           int dz)
+{
+#pragma clang fp contract(off)
+	r.a_cur = a0;
+	if (r.p1.x != r.p2.x)
+	{
+		const int k = ((r.dirx > 0) ? vx + 1 : P.nx - vx) + dx;
+		float c;
+		if constexpr (S64)
+			c = (float)(((r.dirx > 0) ? -0.5 : 0.5) * (double)P.lx +
+			            (double)(k * r.dirx) * (double)P.vx);
+		else
+			c = ((r.dirx > 0) ? -P.lx * 0.5f : P.lx * 0.5f) +
+			    (float)(k * r.dirx) * P.vx;
+		if (k != r.kx)  // else: the setup's first plane, kept as set up
+		{
+			r.xc = c;
+			r.axn = (c - r.p1.x) * r.invx;
+		}
+	}
+	if (r.p1.y != r.p2.y)
+	{
+		const int k = ((r.diry > 0) ? vy + 1 : P.ny - vy) + dy;
+		float c;
+		if constexpr (S64)
+			c = (float)(((r.diry > 0) ? -0.5 : 0.5) * (double)P.ly +
+			            (double)(k * r.diry) * (double)P.vy);
+		else
+			c = ((r.diry > 0) ? -P.ly * 0.5f : P.ly * 0.5f) +
+			    (float)(k * r.diry) * P.vy;
+		if (k != r.ky)
+		{
+			r.yc = c;
+			r.ayn = (c - r.p1.y) * r.invy;
+		}
+	}
+	if (r.p1.z != r.p2.z)
+	{
+		const int k = ((r.dirz > 0) ? vz + 1 : P.nz - vz) + dz;
+		float c;
+		if constexpr (S64)
+			c = (float)(((r.dirz > 0) ? -0.5 : 0.5) * (double)P.lz +
+			            (double)(k * r.dirz) * (double)P.vz);
+		else
+			c = ((r.dirz > 0) ? -P.lz * 0.5f : P.lz * 0.5f) +
+			    (float)(k * r.dirz) * P.vz;
+		if (k != r.kz)
+		{
+			r.zc = c;
+			r.azn = (c - r.p1.z) * r.invz;
+		}
+	}
+}
+
+// Voxel box a walk stays in: [lo, hi) per axis and the linear index
+// base + vz*sz + vy*sy + vx of a voxel (the image itself: base 0, sy = nx,
+// sz = nx*ny; an LDS tile: its own strides and origin).
+struct WalkBox
+{
+	int x0, x1, y0, y1, z0, z1;
+	int sy, sz, base;
+};
+
+__device__ __forceinline__ WalkBox imageBox(const DImage& P)
+{
+	return WalkBox{0, P.nx, 0, P.ny, 0, P.nz, P.nx, P.nxy, 0};
+}
+
+// FP32 walk (ProjectorSiddon.cpp:622-766) from r.a_cur to r.amax.  With
+// `first` the walk starts at the ray's entry and its first voxel is the one
+// holding the midpoint of the first nonempty segment; otherwise it starts in
+// (vx, vy, vz) (after sidRestart).  It ends when the voxel leaves `B`.
+// fn(vx, vy, vz, idx, a_start, w) is called for every nonempty segment.
+template <bool TOF, bool S64, class Fn>
+__device__ __forceinline__ void sidWalk(const DImage& P, SidRay& r,
+                                        const DTof& tof, const WalkBox& B,
+                                        bool first, int vx, int vy, int vz,
+                                        Fn&& fn)
+{
+#pragma clang fp contract(off)
+	const float sdx = (float)r.dirx * P.vx, sdy = (float)r.diry * P.vy,
+	            sdz = (float)r.dirz * P.vz;
+	float axp = r.axn, ayp = r.ayn, azp = r.azn;
+	float a_cur = r.a_cur;
+	const float amaxf = r.amax;
+	int off = first ? 0 : B.base + vz * B.sz + vy * B.sy;
+	int dprev = 0;
+	while (a_cur < amaxf)
+	{
+		int dn = 0;
+		float a_next = -1.f;
+		if (axp <= ayp && axp <= azp)
+		{
+			a_next = r.axn;
+			r.xc += sdx;
+			r.axn = (r.xc - r.p1.x) * r.invx;
+			dn |= 1;
+		}
+		if (ayp <= axp && ayp <= azp)
+		{
+			a_next = r.ayn;
+			r.yc += sdy;
+			r.ayn = (r.yc - r.p1.y) * r.invy;
+			dn |= 2;
+		}
+		if (azp <= axp && azp <= ayp)
+		{
+			a_next = r.azn;
+			r.zc += sdz;
+			r.azn = (r.zc - r.p1.z) * r.invz;
+			dn |= 4;
+		}
+		if (a_next > amaxf)
+			a_next = amaxf;
+		if (a_cur >= a_next)
+		{
+			if (first)
+			{
+				// planes at or before the ray's start: consumed without a voxel
+				axp = r.axn;
+				ayp = r.ayn;
+				azp = r.azn;
+				continue;
+			}
+			// After the first voxel the walk's planes only come after a_cur
+			// (they advance monotonically and ties are taken together); a walk
+			// re-entered at a tile boundary can meet one recomputed an ulp
+			// early, which is still a crossing: an empty segment, then the move.
+			if (a_cur >= amaxf)
+				break;
+			a_next = a_cur;
+		}
 		if (first)
 		{
-			if constexpr (S64)
-			{
-				const double am = 0.5 * ((double)a_cur + (double)a_next);
-				vx = (int)((x1d + am * ddx + 0.5 * (double)P.lx) / (double)P.vx);
-				vy = (int)((y1d + am * ddy + 0.5 * (double)P.ly) / (double)P.vy);
-				vz = (int)((z1d + am * ddz + 0.5 * (double)P.lz) / (double)P.vz);
-			}
-			else
-			{
-				const float am = 0.5f * (a_cur + a_next);
-				vx = (int)((p1.x + am * (p2.x - p1.x) + P.lx / 2.0f) * (1.0f / P.vx));
-				vy = (int)((p1.y + am * (p2.y - p1.y) + P.ly / 2.0f) * (1.0f / P.vy));
-				vz = (int)((p1.z + am * (p2.z - p1.z) + P.lz / 2.0f) * (1.0f / P.vz));
-			}
-			if (vx < 0 || vx >= P.nx || vy < 0 || vy >= P.ny || vz < 0 ||
-			    vz >= P.nz)
+			sidMidVoxel<S64>(P, r, a_cur, a_next, vx, vy, vz);
+			if (vx < B.x0 || vx >= B.x1 || vy < B.y0 || vy >= B.y1 || vz < B.z0 ||
+			    vz >= B.z1)
 				break;
-			off = vz * P.nxy + vy * P.nx;
+			off = B.base + vz * B.sz + vy * B.sy;
 			first = false;
 		}
 		else
 		{
 			if (dprev & 1)
 			{
-				vx += dir_x;
-				if (vx < 0 || vx >= P.nx)
+				vx += r.dirx;
+				if (vx < B.x0 || vx >= B.x1)
 					break;
 			}
 			if (dprev & 2)
 			{
-				vy += dir_y;
-				if (vy < 0 || vy >= P.ny)
+				vy += r.diry;
+				if (vy < B.y0 || vy >= B.y1)
 					break;
-				off += dir_y * P.nx;
+				off += r.diry * B.sy;
 			}
 			if (dprev & 4)
 			{
-				vz += dir_z;
-				if (vz < 0 || vz >= P.nz)
+				vz += r.dirz;
+				if (vz < B.z0 || vz >= B.z1)
 					break;
-				off += dir_z * P.nxy;
+				off += r.dirz * B.sz;
 			}
 		}
 		dprev = dn;
-		float w = (a_next - a_cur) * dnorm;
+		float w = (a_next - a_cur) * r.dnorm;
 		if (TOF)
 		{
 			float xcn;
 			if constexpr (S64)
-				xcn = fmaf(0.5f * (a_cur + a_next), tA, -tB);
+				xcn = fmaf(0.5f * (a_cur + a_next), r.tA, -r.tB);
 			else  // TimeOfFlightHelper::getWeight (TimeOfFlight.hpp:175-185)
-				xcn = (0.5f * (a_cur * dnorm + a_next * dnorm) - tB) / tof.sigma;
+				xcn = (0.5f * (a_cur * r.dnorm + a_next * r.dnorm) - r.tB) / tof.sigma;
 			w *= gaussW(xcn, tof.norm);
 		}
-		const int idx = off + vx;
-		if (MODE == MODE_FWD)
-			acc = fmaf(w, img[idx], acc);
-		else if (MODE == MODE_BACK)
-			unsafeAtomicAdd(img + idx, val * w);
-		else
-			cnt++;
+		fn(vx, vy, vz, off + vx, a_cur, w);
 		a_cur = a_next;
-		axp = axn;
-		ayp = ayn;
-		azp = azn;
+		axp = r.axn;
+		ayp = r.ayn;
+		azp = r.azn;
 	}
+}
+
+// Siddon ray over the whole image.  `img` points at the frame's voxels.
+// MODE_FWD returns the line integral; MODE_BACK adds val*w atomically;
+// MODE_COUNT counts applied segments.
+template <int MODE, bool TOF, bool S64>
+__device__ __forceinline__ float siddonRay(const DImage& P,
+                                           float* __restrict__ img, float3 p1,
+                                           float3 p2, float val,
+                                           const DTof& tof, float tofv,
+                                           uint32_t& cnt)
+{
+	SidRay r;
+	if (!sidSetup<TOF, S64>(P, p1, p2, tof, tofv, r))
+		return 0.f;
+	float acc = 0.f;
+	sidWalk<TOF, S64>(P, r, tof, imageBox(P), true, 0, 0, 0,
+	                  [&](int, int, int, int idx, float, float w) {
+		                  if (MODE == MODE_FWD)
+			                  acc = fmaf(w, img[idx], acc);
+		                  else if (MODE == MODE_BACK)
+			                  unsafeAtomicAdd(img + idx, val * w);
+		                  else
+			                  cnt++;
+	                  });
 	return acc;
 }
 

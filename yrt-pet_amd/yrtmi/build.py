@@ -2,8 +2,10 @@
 
     python -m yrtmi.build          (from yrt-pet_amd/)
 
-Produces ``yrtmi/libyrtmi.so`` (HIP kernels + C ABI).  The library is always
-rebuilt when this runs; loaders never build on their own.
+Produces ``yrtmi/libyrtmi.so`` (HIP kernels + C ABI) and the C++ host layer
+with its pybind11 module ``pyyrtmi`` (``yrt-pet_amd/pyyrtmi*.so``, host code
+only, linked against libyrtmi.so).  Everything is rebuilt when this runs;
+loaders never build on their own.
 """
 from __future__ import annotations
 
@@ -27,7 +29,7 @@ HIP_FLAGS = [
     "-Wno-unused-command-line-argument",
 ]
 
-LIB_SOURCES = ["yrtmi_kernels.hip", "yrtmi_capi.hip"]
+LIB_SOURCES = ["yrtmi_kernels.hip", "yrtmi_tiles.hip", "yrtmi_capi.hip"]
 
 
 def hipcc():
@@ -58,8 +60,33 @@ def build_library(verbose=False):
     return out
 
 
+HOST_SOURCES = ["host/yrtmi_host.cpp", "host/pyyrtmi.cpp"]
+
+
+def build_host_module(verbose=False):
+    """The C++ host layer + pybind11 bindings (g++, no device code)."""
+    import sysconfig
+    import pybind11
+    suffix = sysconfig.get_config_var("EXT_SUFFIX") or ".so"
+    out = os.path.join(PKG_ROOT, "pyyrtmi" + suffix)
+    tmp = out + ".tmp"
+    py_inc = sysconfig.get_paths()["include"]
+    cmd = [shutil.which("g++") or "g++", "-O2", "-std=c++17", "-shared", "-fPIC",
+           "-fvisibility=hidden", "-Wall", "-Wno-unused-function",
+           f"-I{INCLUDE}", f"-I{os.path.join(CSRC, 'host')}",
+           f"-I{pybind11.get_include()}", f"-I{py_inc}",
+           *HOST_SOURCES, f"-L{HERE}", "-lyrtmi", "-Wl,-rpath,$ORIGIN/yrtmi",
+           "-o", tmp]
+    log = _run(cmd)
+    os.replace(tmp, out)
+    if verbose and log.strip():
+        print(log)
+    return out
+
+
 def main():
     print(build_library(verbose=True))
+    print(build_host_module(verbose=True))
 
 
 if __name__ == "__main__":

@@ -106,6 +106,18 @@ _SIGS = {
     "yrtmi_fill": (C.c_int, [_P, _P, C.c_int64, C.c_float]),
     "yrtmi_count_segments": (C.c_int, [_P, _P, C.POINTER(Image), C.POINTER(Lors),
                                        C.POINTER(C.c_int64)]),
+    "yrtmi_plan_create": (C.c_int, [_P, _P, C.POINTER(Image), C.POINTER(Lors), C.c_int,
+                                    C.POINTER(_P)]),
+    "yrtmi_plan_destroy": (C.c_int, [_P]),
+    "yrtmi_plan_info": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                  C.POINTER(C.c_int64)]),
+    "yrtmi_back_planned": (C.c_int, [_P, _P, C.POINTER(Image), _P, C.POINTER(Lors), _P, _P]),
+    "yrtmi_em_subset_planned": (C.c_int, [_P, _P, C.POINTER(Image), _P, _P,
+                                          C.POINTER(Lors), C.POINTER(Corrections), _P]),
+    "yrtmi_em_ratio": (C.c_int, [_P, _P, C.POINTER(Image), _P, C.POINTER(Lors),
+                                 C.POINTER(Corrections), _P]),
+    "yrtmi_sensitivity_planned": (C.c_int, [_P, _P, C.POINTER(Image), _P, C.POINTER(Lors),
+                                            C.POINTER(Image), _P, _P, _P]),
 }
 
 

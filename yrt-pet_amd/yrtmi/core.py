@@ -278,6 +278,34 @@ class Projector:
             pass
 
 
+class Plan:
+    """Tile plan of a LOR set (yrtmi_plan_create): the Siddon walks of its
+    rays binned by 32x32x16 image tiles.  Geometry only; holds device
+    memory until dropped.  Keeps the LOR set alive (the plan refers to its
+    arrays)."""
+
+    def __init__(self, ctx, proj, im, lors, chunk_records=0):
+        self.ctx, self.proj, self.im, self.lors = ctx, proj, im, lors
+        h = _P()
+        ci, cl = im.c(), lors.c()
+        capi.check(capi.lib().yrtmi_plan_create(ctx.h, proj.h, C.byref(ci), C.byref(cl),
+                                                int(chunk_records), C.byref(h)))
+        self.h = h
+
+    def info(self):
+        r, c, b = C.c_int64(), C.c_int64(), C.c_int64()
+        capi.check(capi.lib().yrtmi_plan_info(self.h, C.byref(r), C.byref(c), C.byref(b)))
+        return dict(records=r.value, chunks=c.value, bytes=b.value)
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                capi.lib().yrtmi_plan_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
 class Context:
     """A yrtmi context bound to one GPU and a HIP stream (torch's current
     stream by default), holding the resident detector LUT and histogram /
@@ -329,19 +357,26 @@ class Context:
                                             C.byref(cl), _ptr(out)))
         return out
 
-    def back(self, proj: Projector, im: ImageParams, img, lors: LorSet, vals):
+    def plan(self, proj: Projector, im: ImageParams, lors: LorSet, chunk_records=0):
+        return Plan(self, proj, im, lors, chunk_records)
+
+    def back(self, proj: Projector, im: ImageParams, img, lors: LorSet, vals, plan=None):
         _check_dtype(img, torch.float32, "img")
         _check_dtype(vals, torch.float32, "vals")
         if img.numel() != im.nvox * im.nt or vals.numel() < lors.n:
             raise ValueError("back: size mismatch")
         ci, cl = im.c(), lors.c()
-        capi.check(capi.lib().yrtmi_back(self.h, proj.h, C.byref(ci), _ptr(img),
-                                         C.byref(cl), _ptr(vals)))
+        if plan is not None:
+            capi.check(capi.lib().yrtmi_back_planned(self.h, proj.h, C.byref(ci), _ptr(img),
+                                                     C.byref(cl), _ptr(vals), plan.h))
+        else:
+            capi.check(capi.lib().yrtmi_back(self.h, proj.h, C.byref(ci), _ptr(img),
+                                             C.byref(cl), _ptr(vals)))
         return img
 
     def em_subset(self, proj, im, x, u, lors, meas=None, sens=None, att=None,
                   randoms=None, scatter=None, global_scale=1.0,
-                  denom_threshold=1e-35):
+                  denom_threshold=1e-35, plan=None):
         for t, nm in ((x, "x"), (u, "u"), (meas, "meas"), (sens, "sens"),
                       (att, "att"), (randoms, "randoms"), (scatter, "scatter")):
             _check_dtype(t, torch.float32, nm)
@@ -353,11 +388,35 @@ class Context:
         corr = capi.Corrections(_ptr(meas), _ptr(sens), _ptr(att), _ptr(randoms),
                                 _ptr(scatter), global_scale, denom_threshold)
         ci, cl = im.c(), lors.c()
-        capi.check(capi.lib().yrtmi_em_subset(self.h, proj.h, C.byref(ci), _ptr(x),
-                                              _ptr(u), C.byref(cl), C.byref(corr)))
+        if plan is not None:
+            capi.check(capi.lib().yrtmi_em_subset_planned(
+                self.h, proj.h, C.byref(ci), _ptr(x), _ptr(u), C.byref(cl), C.byref(corr),
+                plan.h))
+        else:
+            capi.check(capi.lib().yrtmi_em_subset(self.h, proj.h, C.byref(ci), _ptr(x),
+                                                  _ptr(u), C.byref(cl), C.byref(corr)))
         return u
 
-    def sensitivity(self, proj, im, img, lors, att_im=None, att_img=None, sens=None):
+    def em_ratio(self, proj, im, x, lors, v, meas=None, sens=None, att=None,
+                 randoms=None, scatter=None, global_scale=1.0, denom_threshold=1e-35):
+        """v[i] = EM ratio of LOR i (the forward half of em_subset)."""
+        for t, nm in ((x, "x"), (v, "v"), (meas, "meas"), (sens, "sens"),
+                      (att, "att"), (randoms, "randoms"), (scatter, "scatter")):
+            _check_dtype(t, torch.float32, nm)
+        if x.numel() != im.nvox * im.nt or v.numel() < lors.n:
+            raise ValueError("em_ratio: size mismatch")
+        for t in (meas, sens, att, randoms, scatter):
+            if t is not None and t.numel() < lors.n:
+                raise ValueError("em_ratio: correction array shorter than the LOR set")
+        corr = capi.Corrections(_ptr(meas), _ptr(sens), _ptr(att), _ptr(randoms),
+                                _ptr(scatter), global_scale, denom_threshold)
+        ci, cl = im.c(), lors.c()
+        capi.check(capi.lib().yrtmi_em_ratio(self.h, proj.h, C.byref(ci), _ptr(x),
+                                             C.byref(cl), C.byref(corr), _ptr(v)))
+        return v
+
+    def sensitivity(self, proj, im, img, lors, att_im=None, att_img=None, sens=None,
+                    plan=None):
         _check_dtype(img, torch.float32, "img")
         if img.numel() != im.nvox:
             raise ValueError("sensitivity: image must be single-frame")
@@ -365,9 +424,14 @@ class Context:
         ca = att_im.c(nt=1) if att_img is not None else None
         if att_img is not None and att_img.numel() != att_im.nvox:
             raise ValueError("sensitivity: attenuation image size mismatch")
-        capi.check(capi.lib().yrtmi_sensitivity(
-            self.h, proj.h, C.byref(ci), _ptr(img), C.byref(cl),
-            C.byref(ca) if ca is not None else None, _ptr(att_img), _ptr(sens)))
+        if plan is not None:
+            capi.check(capi.lib().yrtmi_sensitivity_planned(
+                self.h, proj.h, C.byref(ci), _ptr(img), C.byref(cl),
+                C.byref(ca) if ca is not None else None, _ptr(att_img), _ptr(sens), plan.h))
+        else:
+            capi.check(capi.lib().yrtmi_sensitivity(
+                self.h, proj.h, C.byref(ci), _ptr(img), C.byref(cl),
+                C.byref(ca) if ca is not None else None, _ptr(att_img), _ptr(sens)))
         return img
 
     def attenuation_factors(self, att_im, att_img, lors, out):

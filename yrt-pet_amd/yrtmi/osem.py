@@ -4,8 +4,13 @@
 Everything lives in HBM for the whole reconstruction: events / nonzero
 histogram bins (SoA), per-LOR correction factors (computed once), the
 sensitivity image(s), the image and the EM update image.  Per subset the
-host issues: a fill of the update image, one fused EM kernel, one EM-update
+host issues: a fill of the update image, the EM kernels, one EM-update
 kernel; there is no host-side property packing or per-batch copy.
+
+With the Siddon projector (and `use_tile_plans`, the default) every subset
+gets a tile plan once, before the first iteration: its rays' walks binned
+by image tile, so that backprojection accumulates in LDS instead of issuing
+one global atomic per voxel segment.
 
 Data inputs:
   ListModeData   - d1, d2 (int32), optional tof (ps), dynamic frame,
@@ -85,6 +90,7 @@ class OSEM:
         self.sensImages: list[torch.Tensor] = []
         self.lanes_per_event = None
         self.per_xcd = True
+        self.use_tile_plans = True
 
     # ---- configuration (OSEM.hpp:57-134) ----
     def setImageParams(self, p):
@@ -152,6 +158,9 @@ class OSEM:
         p.set_accumulation(self.per_xcd)
         return p
 
+    def _planned(self):
+        return self.use_tile_plans and self.projectorParams.projectorType == "Siddon"
+
     def _sens_factor_per_bin(self, bins_flat):
         """Corrector::getSensitivity (Corrector.cpp:295-308)."""
         if self.sensHisto is None:
@@ -180,8 +189,10 @@ class OSEM:
                 bins = self._subset_bins(S, s)
                 sens = self._sens_factor_per_bin(bins)
                 L = core.LorSet.bins_(bins)
+            plan = self.ctx.plan(proj, im1, L) if self._planned() else None
             self.ctx.sensitivity(proj, im1, img, L, att_im=self.attImageParams,
-                                 att_img=self.attImage, sens=sens)
+                                 att_img=self.attImage, sens=sens, plan=plan)
+            del plan
             # hard threshold: <= thr -> 0, else * global scale
             # (applyThreshold, OSEM_CPU.cpp:168-172)
             self.ctx.threshold(img, img.clone(), im1.nvox, self.hardThreshold, 0.0, 0.0,
@@ -205,7 +216,7 @@ class OSEM:
         return (z * ph * nr + phi * nr + r).to(torch.int32)
 
     # ---- reconstruction (OSEM.cpp:315-365,788-811,1000-1035) ----
-    def _prepare(self):
+    def _prepare(self, proj, imt):
         d = self.data
         im = self.imageParams
         S = self.num_OSEM_subsets
@@ -243,6 +254,8 @@ class OSEM:
                 gb = lambda h: None if h is None else h.reshape(-1)[bins.long()].float().contiguous()
                 self.subsets.append(dict(L=L, meas=m, att=att, sens=self._sens_factor_per_bin(bins),
                                          randoms=gb(self.randomsHisto), scatter=gb(self.scatterHisto)))
+        for c in self.subsets:
+            c["plan"] = self.ctx.plan(proj, imt, c["L"]) if self._planned() else None
 
     def reconstruct(self):
         im = self.imageParams
@@ -273,8 +286,8 @@ class OSEM:
             for s in sens:
                 mask += s
         self.ctx.threshold(x, mask.contiguous(), imt.nvox, 0.0, 0.0, 0.0, 1.0, 0.0)
-        self._prepare()
         proj = self._projector(with_tof=True)
+        self._prepare(proj, imt)
         u = torch.empty_like(x)
         for it in range(self.num_MLEM_iterations):
             for s in range(S):
@@ -283,7 +296,7 @@ class OSEM:
                 self.ctx.em_subset(proj, imt, x, u, c["L"], meas=c["meas"], sens=c["sens"],
                                    att=c["att"], randoms=c["randoms"], scatter=c["scatter"],
                                    global_scale=self.globalScale,
-                                   denom_threshold=self.denomThreshold)
+                                   denom_threshold=self.denomThreshold, plan=c["plan"])
                 sidx = 0 if self.isListModeEnabled() else s
                 self.ctx.em_update(x, u, sens[sidx].contiguous(), imt.nvox, EPS_FLT)
         return x
