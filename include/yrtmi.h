@@ -234,6 +234,16 @@ int yrtmi_threshold(yrtmi_ctx* ctx, float* x, const float* mask, int64_t nvox,
                     float le_off, float gt_scale, float gt_off);
 int yrtmi_fill(yrtmi_ctx* ctx, float* x, int64_t n, float value);
 
+/* FP64 Siddon projector (no counterpart in the reference, which computes in
+ * FP32): the same LORs and rays with every ray quantity in FP64 (detector
+ * positions are the FP32 LUT values).  For double-precision gradient checks
+ * of the differentiable projector and as an accuracy reference.  Siddon
+ * projectors only; DEVICE double images and values. */
+int yrtmi_forward_f64(yrtmi_ctx* ctx, const yrtmi_proj* p, const yrtmi_image* im,
+                      const double* img, const yrtmi_lors* lors, double* out);
+int yrtmi_back_f64(yrtmi_ctx* ctx, const yrtmi_proj* p, const yrtmi_image* im,
+                   double* img, const yrtmi_lors* lors, const double* vals);
+
 /* ---- instrumentation ---------------------------------------------------- */
 /* Total voxel segments (Siddon, summed over rays) or footprint voxels (DD)
  * touched by `lors`: the algorithmic work unit of the roofline figures.

@@ -534,14 +534,17 @@ def sensitivity_images(sc: ScannerDef, im: ImageDef, proj: ProjDef,
                        num_subsets: int, listmode: bool, att_im=None,
                        att_img=None, hard_threshold=DEFAULT_HARD_THRESHOLD,
                        global_scale=1.0, dtype=np.float32, nthreads=1,
-                       pos_ori=None):
+                       pos_ori=None, sens_histo=None, invert_sens=False):
     """OSEM::generateSensitivityImagesCore (src/recon/OSEM.cpp:926-986) with
     OSEM_CPU::generateSensitivityImageForCurrentSubset (OSEM_CPU.cpp:127-188):
     backproject the multiplicative correction (acf * sens, 1.0 without
     corrections; Corrector_CPU.cpp:17-45) of every bin of the scanner's
     uniform histogram (Corrector.cpp:152-157,228-241), one image per subset
     (one only for list-mode), projector without TOF (OSEM_CPU.cpp:94-108),
-    then the hard threshold (voxels <= thr -> 0, others * global scale)."""
+    then the hard threshold (voxels <= thr -> 0, others * global scale).
+    sens_histo: per-bin normalisation (histogram order), inverted where
+    nonzero when invert_sens (Corrector::getSensitivity, Corrector.cpp:
+    295-308); the bin's factor is acf * sens (Corrector_CPU.cpp:17-45)."""
     pos, ori = pos_ori if pos_ori is not None else lut(sc)
     S = 1 if listmode else num_subsets
     p = proj.without_tof()
@@ -556,6 +559,9 @@ def sensitivity_images(sc: ScannerDef, im: ImageDef, proj: ProjDef,
             vals = attenuation_factors(att_im, att_img, L, nthreads)
         else:
             vals = np.ones(L.n, np.float32)
+        if sens_histo is not None:
+            sf = sensitivity_factors(sens_histo, bins, invert_sens)
+            vals = (vals * sf).astype(np.float32)
         img = back(im1, p, L, vals, dtype=dtype, nthreads=nthreads)
         img = img.astype(np.float32) if dtype == np.float32 else img
         thr_img = img.copy()
@@ -564,11 +570,22 @@ def sensitivity_images(sc: ScannerDef, im: ImageDef, proj: ProjDef,
     return out
 
 
+def sensitivity_factors(sens_histo, bins, invert=False):
+    """Corrector::getSensitivity (Corrector.cpp:295-308) for bins: the
+    histogram value, 1/value where nonzero when inverted."""
+    v = np.asarray(sens_histo, np.float32).reshape(-1)[np.asarray(bins, np.int64)]
+    if invert:
+        nz = v != 0
+        v = v.copy()
+        v[nz] = np.float32(1.0) / v[nz]
+    return v.astype(np.float32)
+
+
 def reconstruct(sc: ScannerDef, im: ImageDef, proj: ProjDef, lors: Lors,
                 sens_images, num_iterations: int, num_subsets: int,
                 listmode: bool, meas=None, att=None, rnd=None, scat=None,
                 global_scale=1.0, subset_indices=None, dtype=np.float32,
-                nthreads=1, initial=None):
+                nthreads=1, initial=None, norm=None):
     """OSEM::reconstruct / iterate (src/recon/OSEM.cpp:315-365,788-811) with
     the CPU hooks of OSEM_CPU.cpp:217-403:
       - list-mode sensitivity scaled by 1/S (OSEM.cpp:1007-1035);
@@ -577,7 +594,8 @@ def reconstruct(sc: ScannerDef, im: ImageDef, proj: ProjDef, lors: Lors,
       - per subset: u = A^T(m / (A x .* corr)) .* corr; x *= u/S where S > EPS.
     `lors` holds every event/bin; `subset_indices[s]` selects the subset
     (default: interleaved for list-mode, the caller supplies histogram
-    subsets).  `att` is the per-LOR ACF (precomputeAttenuationFactors)."""
+    subsets).  `att` is the per-LOR ACF (precomputeAttenuationFactors);
+    `norm` the per-LOR normalisation factor (sensitivity_factors)."""
     dt = _dt(dtype)
     S = num_subsets
     sens = [np.asarray(s, dt).reshape(im.nz, im.ny, im.nx) for s in sens_images]
@@ -600,7 +618,7 @@ def reconstruct(sc: ScannerDef, im: ImageDef, proj: ProjDef, lors: Lors,
         for s in range(S):
             idx = subset_indices[s]
             u = np.zeros(im.shape, dt)
-            em_subset(im, x, u, proj, sub_lors[s], take(meas, idx), None,
+            em_subset(im, x, u, proj, sub_lors[s], take(meas, idx), take(norm, idx),
                       take(att, idx), take(rnd, idx), take(scat, idx),
                       global_scale, DEFAULT_DENOM_THRESHOLD, dt, nthreads)
             sidx = 0 if listmode else s

@@ -1,0 +1,31 @@
+// Test harness only: lets tests/hostwalk/walk_host.cpp compile the device
+// ray-walk header (yrt-pet_amd/csrc/yrtmi_rays.hpp) with the host compiler,
+// so the walk's logic is checked on the CPU.  It defines the HIP function
+// qualifiers away and gives host versions of the few device intrinsics the
+// header uses (correctly rounded FP32 ops, exp, atomic add).  Nothing in the
+// library includes it.
+#pragma once
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#define __device__
+#define __host__
+#define __global__
+#define __forceinline__ inline
+#define __launch_bounds__(x)
+struct float3
+{
+	float x, y, z;
+};
+inline float3 make_float3(float a, float b, float c) { return {a, b, c}; }
+inline float __fadd_rn(float a, float b) { return a + b; }
+inline float __fsub_rn(float a, float b) { return a - b; }
+inline float __fmul_rn(float a, float b) { return a * b; }
+inline float __fdiv_rn(float a, float b) { return a / b; }
+inline float __fsqrt_rn(float a) { return std::sqrt(a); }
+inline float __expf(float a) { return std::exp(a); }
+inline void unsafeAtomicAdd(float* p, float v) { *p += v; }
+using std::max;
+using std::min;
+typedef int hipError_t;
+typedef void* hipStream_t;

@@ -4,6 +4,7 @@
 #include "yrtmi.h"
 #include "yrtmi_kernels.hpp"
 #include "yrtmi_tiles.hpp"
+#include "yrtmi_f64.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -520,6 +521,9 @@ int yrtmi_proj_create(yrtmi_ctx* c, const yrtmi_projector_desc* d,
 		q.tof.sigma = (float)(width_mm / (2 * sqrtf(2 * logf(2.0f))));
 		q.tof.trunc = (d->tof_n_std <= 0) ? -1.f : d->tof_n_std * q.tof.sigma;
 		q.tof.norm = (float)(1 / (std::sqrt(2 * 3.14159265358979) * q.tof.sigma));
+		q.tof64.sigma = width_mm / (2 * std::sqrt(2 * std::log(2.0)));
+		q.tof64.trunc = (d->tof_n_std <= 0) ? -1.0 : d->tof_n_std * q.tof64.sigma;
+		q.tof64.norm = 1 / (std::sqrt(2 * M_PI) * q.tof64.sigma);
 	}
 	// multi-ray offsets: rand_r(seed = 13), 4 per extra ray
 	// (MultiRayGenerator.cpp:78-92; the seed is reset for every LOR,
@@ -665,6 +669,40 @@ int yrtmi_back(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
 	r = foldAccum(c, a, img, ny);
 	if (r)
 		return r;
+	return YRTMI_OK;
+}
+
+int yrtmi_forward_f64(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
+                      const double* img, const yrtmi_lors* l, double* out)
+{
+	KArgs a;
+	int r = prepare(c, p, im, l, a);
+	if (r)
+		return r;
+	if (p->d.type != YRTMI_PROJ_SIDDON)
+		return fail(YRTMI_EINVAL, "forward_f64: Siddon projector only");
+	if (l->n == 0)
+		return YRTMI_OK;
+	if (img == nullptr || out == nullptr)
+		return fail(YRTMI_EINVAL, "forward_f64: null image or output");
+	HIP_TRY(launchProj64(true, a, img, nullptr, nullptr, out, c->stream));
+	return YRTMI_OK;
+}
+
+int yrtmi_back_f64(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
+                   double* img, const yrtmi_lors* l, const double* vals)
+{
+	KArgs a;
+	int r = prepare(c, p, im, l, a);
+	if (r)
+		return r;
+	if (p->d.type != YRTMI_PROJ_SIDDON)
+		return fail(YRTMI_EINVAL, "back_f64: Siddon projector only");
+	if (l->n == 0)
+		return YRTMI_OK;
+	if (img == nullptr || vals == nullptr)
+		return fail(YRTMI_EINVAL, "back_f64: null image or values");
+	HIP_TRY(launchProj64(false, a, nullptr, img, vals, nullptr, c->stream));
 	return YRTMI_OK;
 }
 

@@ -40,6 +40,12 @@ struct DTof
 	float sigma, trunc, norm;
 };
 
+// The same constants in FP64 (the FP64 projector).
+struct DTof64
+{
+	double sigma, trunc, norm;
+};
+
 // Projection-space PSF tables (ProjectionPsfManager.cpp:27-103).
 struct DPsf
 {
@@ -56,6 +62,7 @@ struct DProj
 	int32_t useTof;
 	int32_t setup64;  // FP64 ray-plane setup instead of the reference FP32 order
 	DTof tof;
+	DTof64 tof64;
 	float tz, tt;  // crystal size z / transaxial
 	float rayOff[4 * (kMaxRays - 1)];
 	DPsf psf;

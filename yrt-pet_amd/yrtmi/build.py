@@ -29,7 +29,7 @@ HIP_FLAGS = [
     "-Wno-unused-command-line-argument",
 ]
 
-LIB_SOURCES = ["yrtmi_kernels.hip", "yrtmi_tiles.hip", "yrtmi_capi.hip"]
+LIB_SOURCES = ["yrtmi_kernels.hip", "yrtmi_tiles.hip", "yrtmi_f64.hip", "yrtmi_capi.hip"]
 
 
 def hipcc():

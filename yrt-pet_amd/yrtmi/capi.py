@@ -106,6 +106,18 @@ _SIGS = {
     "yrtmi_fill": (C.c_int, [_P, _P, C.c_int64, C.c_float]),
     "yrtmi_count_segments": (C.c_int, [_P, _P, C.POINTER(Image), C.POINTER(Lors),
                                        C.POINTER(C.c_int64)]),
+    "yrtmi_forward_f64": (C.c_int, [_P, _P, C.POINTER(Image), _P, C.POINTER(Lors), _P]),
+    "yrtmi_back_f64": (C.c_int, [_P, _P, C.POINTER(Image), _P, C.POINTER(Lors), _P]),
+    "yrtmi_device_alloc": (C.c_int, [_P, C.c_int64, C.POINTER(_P)]),
+    "yrtmi_device_free": (C.c_int, [_P, _P]),
+    "yrtmi_copy": (C.c_int, [_P, _P, _P, C.c_int64, C.c_int]),
+    "yrtmi_histo_subset_nonzero": (C.c_int, [_P, _P, C.c_int, C.c_int, _P, _P,
+                                             C.POINTER(C.c_int64)]),
+    "yrtmi_subset_interleaved": (C.c_int, [_P, _P, C.c_int, C.c_int64, C.c_int, C.c_int, _P,
+                                           C.POINTER(C.c_int64)]),
+    "yrtmi_gather": (C.c_int, [_P, _P, C.c_int64, _P, C.c_int64, C.c_int, _P]),
+    "yrtmi_histo_subset_gather": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, _P]),
+    "yrtmi_axpby": (C.c_int, [_P, C.c_int64, C.c_float, _P, C.c_float, _P]),
     "yrtmi_plan_create": (C.c_int, [_P, _P, C.POINTER(Image), C.POINTER(Lors), C.c_int,
                                     C.POINTER(_P)]),
     "yrtmi_plan_destroy": (C.c_int, [_P]),
