@@ -1,24 +1,44 @@
 #!/usr/bin/env python3
 """Benchmark: list-mode TOF OSEM with 4-ray Siddon on one MI355X.
 
-Workload (the 64-ring x 512-crystal scanner configuration): scanner radius
-400 mm, 3.2 x 3.2 x 20 mm crystals, 100 M list-mode events (uint32 d1, d2 +
-float32 TOF) drawn from a seeded phantom (water cylinder r = 150 mm, length
-180 mm, plus 6 spheres of radius 4-14 mm at 5:1 contrast), TOF 250 ps FWHM
-truncated at 5 sigma, image 200 x 200 x 128 at 2 mm, 10 subsets.
+Default workload (the 64-ring x 512-crystal scanner configuration): scanner
+radius 400 mm, 3.2 x 3.2 x 20 mm crystals, 100 M list-mode events (uint32 d1,
+d2 + float32 TOF) drawn from a seeded phantom (water cylinder r = 150 mm,
+length 180 mm, plus 6 spheres of radius 4-14 mm at 5:1 contrast), TOF 250 ps
+FWHM truncated at 5 sigma, image 200 x 200 x 128 at 2 mm, 10 subsets.
 
 One step = one full OSEM iteration: for each of the 10 subsets, the EM
 update image of that subset's events (forward projection and ratio per
 event, then backprojection) followed by the multiplicative image update.
 Default (--accum tiled): the backprojection goes through per-subset tile
-plans (the Siddon walks binned by 32x32x16 image tiles, accumulated in LDS),
-built once in setup; --accum xcd / direct run the fused one-kernel EM with
-global atomics instead.  Events, plans, the detector LUT and all images are
-resident in HBM before timing starts; the sensitivity image
-(backprojection of all 536.9 M scanner LORs x 4 rays) and the plans are
-made once in setup and timed separately.
+plans (the Siddon walks binned by image tiles, accumulated in LDS), built
+once in setup, with the launch shapes picked by the autotuner on subset 0
+(--no-tune: defaults), and the iteration replayed as one recorded graph
+(--no-graph: launched call by call).  --accum xcd / direct run the fused
+one-kernel EM with global atomics instead.  Events, plans, the detector LUT
+and all images are resident in HBM before timing starts; the sensitivity
+image (backprojection of all 536.9 M scanner LORs x 4 rays), the plans and
+the tuning are made once in setup and timed separately.  Per-kernel times
+(roofline) come from one more iteration run call by call with HIP events on
+the context's stream.
 
-Prints one JSON line (see the `out` dict at the end).
+--workload histogram: the CPU-runnable histogram configuration (24 rings x
+256 crystals, R 300 mm, max ring difference 23; image 96 x 96 x 48 at 2 mm;
+water cylinder r 100 mm x 90 mm with 4 hot spheres (r 5/8/11/15 mm, 4:1),
+mu 0.0096/mm inside; measured histogram = Poisson(attenuated forward
+projection scaled to 50 M counts); Siddon, no TOF, 5 iterations x 8
+subsets), reconstructed end to end through the C++ host layer (pyyrtmi.OSEM:
+sensitivity images, then reconstruct) and by the oracle's restatement of
+the reference CPU path on the same data, timed beside it; prints the final
+images' relative L2 difference.
+
+--workload frames: 16 dynamic frames of uneven event counts (log-uniform in
+[--frame-min, --frame-max], seeded) reconstructed (a) the plain way, frame
+after frame with the fused EM kernel, (b) frame after frame through tile
+plans, and (c) batched: all frames in one multi-frame image and one event
+set (frame per event), tile plans, autotuned, graph-replayed.
+
+Prints one JSON line (see the dict each workload returns).
 """
 from __future__ import annotations
 
@@ -42,10 +62,12 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=("listmode", "frames", "histogram"),
+                    default="listmode")
     ap.add_argument("--events", type=int, default=100_000_000)
     ap.add_argument("--subsets", type=int, default=10)
     ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--lpe", type=int, default=4, help="lanes per event")
+    ap.add_argument("--lpe", type=int, default=4, help="lanes per event (untuned runs)")
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--grid", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -54,6 +76,13 @@ def parse():
                     help="backprojection: tile plans with LDS accumulation, or global "
                          "atomics into per-XCD images / the target directly")
     ap.add_argument("--chunk", type=int, default=0, help="tile plan records per workgroup")
+    ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
+    ap.add_argument("--order", choices=("lor", "none"), default="lor",
+                    help="project each subset's events in locality order (yrtmi_lor_order)")
+    ap.add_argument("--no-graph", action="store_true", help="launch the iteration call by call")
+    ap.add_argument("--frames", type=int, default=16)
+    ap.add_argument("--frame-min", type=float, default=0.25e6)
+    ap.add_argument("--frame-max", type=float, default=10e6)
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one 256 MiB device copy first (PMC calibration)")
     ap.add_argument("--quiet", action="store_true")
@@ -65,6 +94,32 @@ def log(args, *a):
         print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+def scanner_and_image():
+    import yrtmi
+    sc = yrtmi.Scanner("cfg2", 64 * 3.2, 3.2, 3.2, 20.0, 400.0, 512, 64)
+    im = yrtmi.ImageParams(200, 200, 128, 2.0, 2.0, 2.0)
+    return sc, im
+
+
+def phantom(args, im):
+    from yrtmi import synth
+    rng = np.random.default_rng(args.seed)
+    spheres = synth.random_spheres_in_cylinder(rng, 150.0, 180.0,
+                                               list(np.linspace(4.0, 14.0, 6)), 5.0)
+    return synth.cylinder_spheres(im, 150.0, 180.0, 1.0, spheres, supersample=2)
+
+
+def subset_major(torch, ev, n, S, dev):
+    """Event i belongs to subset i mod S (BinIteratorChronologicalInterleaved);
+    permute once so each subset is contiguous."""
+    perm = torch.cat([torch.arange(s, n, S, device=dev) for s in range(S)])
+    out = {k: ev[k][perm].contiguous() for k in ("d1", "d2", "tof")}
+    bounds = [0]
+    for s in range(S):
+        bounds.append(bounds[-1] + len(range(s, n, S)))
+    return out, bounds
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -72,9 +127,6 @@ def main():
                                    "(multi-GPU sharding is described in DESIGN.md, not built)"}))
         return 2
     import torch
-    import yrtmi
-    from yrtmi import synth
-
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     if args.calib_copy:
@@ -84,32 +136,36 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
+    stream = torch.cuda.Stream()  # default priority; graphs need a non-null stream
+    with torch.cuda.stream(stream):
+        if args.workload == "frames":
+            out = bench_frames(args, torch, dev, stream)
+        elif args.workload == "histogram":
+            out = bench_histogram(args, torch, dev, stream)
+        else:
+            out = bench_listmode(args, torch, dev, stream)
+    print(json.dumps(out))
+    return 0
+
+
+def bench_listmode(args, torch, dev, stream):
+    import yrtmi
+    from yrtmi import synth
+
     t_setup = time.time()
-    sc = yrtmi.Scanner("cfg2", 64 * 3.2, 3.2, 3.2, 20.0, 400.0, 512, 64)
-    im = yrtmi.ImageParams(200, 200, 128, 2.0, 2.0, 2.0)
-    rng = np.random.default_rng(args.seed)
-    spheres = synth.random_spheres_in_cylinder(rng, 150.0, 180.0,
-                                               list(np.linspace(4.0, 14.0, 6)), 5.0)
-    act = synth.cylinder_spheres(im, 150.0, 180.0, 1.0, spheres, supersample=2)
+    sc, im = scanner_and_image()
+    act = phantom(args, im)
     log(args, f"phantom ready ({time.time() - t_setup:.1f}s)")
     ev = synth.sample_events(sc, im, act, args.events, seed=args.seed,
                              tof_fwhm_ps=250.0, device=dev)
+    S, n = args.subsets, args.events
+    ev, bounds = subset_major(torch, ev, n, S, dev)
+    d1, d2, tof = ev["d1"], ev["d2"], ev["tof"]
+    del ev
     torch.cuda.synchronize()
-    log(args, f"{args.events} events drawn ({time.time() - t_setup:.1f}s)")
-    # subset-major permutation: event i belongs to subset i mod S
-    # (BinIteratorChronologicalInterleaved); each subset becomes contiguous
-    S = args.subsets
-    n = args.events
-    perm = torch.cat([torch.arange(s, n, S, device=dev) for s in range(S)])
-    d1 = ev["d1"][perm].contiguous()
-    d2 = ev["d2"][perm].contiguous()
-    tof = ev["tof"][perm].contiguous()
-    del ev, perm
-    bounds = [0]
-    for s in range(S):
-        bounds.append(bounds[-1] + len(range(s, n, S)))
+    log(args, f"{n} events drawn ({time.time() - t_setup:.1f}s)")
 
-    ctx = yrtmi.Context(sc)
+    ctx = yrtmi.Context(sc, stream=stream)
     pp = yrtmi.ProjectorParams(sc, numRays=4)
     pp.addTOF(250.0, 5)
     proj = ctx.projector(pp)
@@ -132,7 +188,7 @@ def main():
             del pl
         else:
             ctx.sensitivity(psens, im, sens, Ls)
-    torch.cuda.synchronize()
+    ctx.synchronize()
     sens_s = time.time() - t0
     n_lors = sum(yrtmi.LorSet.histo_subset(ctx, SS, s).n for s in range(SS))
     log(args, f"sensitivity: {n_lors} LORs x 4 rays in {sens_s:.2f}s")
@@ -144,16 +200,34 @@ def main():
     u = torch.zeros(im.shape, device=dev)
     sub = [yrtmi.LorSet.pairs(d1[bounds[s]:bounds[s + 1]], d2[bounds[s]:bounds[s + 1]],
                               tof=tof[bounds[s]:bounds[s + 1]]) for s in range(S)]
+    order_s = 0.0
+    if args.order == "lor":
+        # each subset's events in locality order (the EM update does not
+        # depend on it); a setup step like the plans
+        ctx.synchronize()
+        t0 = time.time()
+        sub = [L.permuted(ctx, ctx.lor_order(im, L)) for L in sub]
+        ctx.synchronize()
+        order_s = time.time() - t0
+        log(args, f"locality order of the subsets: {order_s:.2f}s")
     # algorithmic work: voxel segments per subset (instrumented count)
     segs = [ctx.count_segments(proj, im, L) for L in sub]
     log(args, f"segments/event {sum(segs) / n:.1f}; setup {time.time() - t_setup:.1f}s")
-    plans, plan_s, plan_bytes, plan_records = [], 0.0, 0, 0
+    plans, plan_s, plan_bytes, plan_records, tune, tune_s = [], 0.0, 0, 0, None, 0.0
     vbuf = None
     if tiled:
-        torch.cuda.synchronize()
+        shape, pers = 0, 0
+        if not args.no_tune:
+            t0 = time.time()
+            tune = ctx.autotune(proj, im, sub[0], x)
+            tune_s = time.time() - t0
+            shape, pers = tune["tile_shape"], tune["persistent"]
+            log(args, f"autotune ({tune_s:.2f}s): {tune}")
+        ctx.synchronize()
         t0 = time.time()
-        plans = [ctx.plan(proj, im, L, args.chunk) for L in sub]
-        torch.cuda.synchronize()
+        plans = [ctx.plan(proj, im, L, args.chunk, tile_shape=shape, persistent=pers)
+                 for L in sub]
+        ctx.synchronize()
         plan_s = time.time() - t0
         infos = [p_.info() for p_ in plans]
         plan_bytes = sum(i["bytes"] for i in infos)
@@ -180,16 +254,27 @@ def main():
 
     for _ in range(args.warmup):
         iteration()
+    graph = None
+    if not args.no_graph:
+        ctx.synchronize()
+        with ctx.graph() as graph:
+            iteration()
+    ctx.synchronize()
     torch.cuda.synchronize()
-    kernel_ms = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        evs = [tuple(torch.cuda.Event(enable_timing=True) for _ in range(3))
-               for _ in range(S)]
-        iteration(evs)
-        kernel_ms.append(evs)
+        if graph is not None:
+            graph.launch()
+        else:
+            iteration()
+    ctx.synchronize()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    # per-kernel times: one more iteration call by call, HIP events on the
+    # context's stream
+    evs = [tuple(torch.cuda.Event(enable_timing=True) for _ in range(3)) for _ in range(S)]
+    iteration(evs)
+    ctx.synchronize()
     s_per_iter = wall / args.steps
     ev_proj_per_s = 2.0 * n * args.steps / wall
     # Algorithmic bytes per launch (R_ev = 12 B/event: d1, d2, tof; an atomic
@@ -201,29 +286,30 @@ def main():
     kernels = []
 
     def roof(name, idx, bytes_per_subset):
-        ms = [evs[s][idx[0]].elapsed_time(evs[s][idx[1]]) for evs in kernel_ms for s in range(S)]
-        by = [bytes_per_subset[s] for _ in kernel_ms for s in range(S)]
-        avg_ms, avg_b = float(np.mean(ms)), float(np.mean(by))
+        ms = [evs[s][idx[0]].elapsed_time(evs[s][idx[1]]) for s in range(S)]
+        avg_ms, avg_b = float(np.mean(ms)), float(np.mean(bytes_per_subset))
         ach = avg_b / (avg_ms * 1e-3) / 1e9
         kernels.append({"kernel": name, "avg_launch_ms": avg_ms, "alg_bytes_per_launch": avg_b,
                         "achieved": ach, "frac": ach / HBM_PEAK_GBS})
 
+    lpe = tune["lanes_per_event"] if tune else args.lpe
     if tiled:
-        roof("kEmRatio<TOF,LPE=4> (forward + ratio)", (0, 1),
+        roof(f"kEmRatio<TOF,LPE={lpe}> (forward + ratio)", (0, 1),
              [12.0 * nev[s] + 4.0 * segs[s] + 4.0 * nev[s] for s in range(S)])
-        roof("kTileBack<TOF> (tile-binned backprojection)", (1, 2),
+        shape = tune["tile_shape"] if tune else 0
+        roof(f"kTileBack<TOF,shape={shape}> (tile-binned backprojection)", (1, 2),
              [12.0 * nev[s] + 4.0 * nev[s] + 8.0 * segs[s] for s in range(S)])
     else:
-        roof("kEvents<MODE_EM,Siddon,TOF,LPE=4> (fused EM)", (0, 2),
+        roof(f"kEvents<MODE_EM,Siddon,TOF,LPE={lpe}> (fused EM)", (0, 2),
              [12.0 * nev[s] + 12.0 * segs[s] for s in range(S)])
     dom = max(kernels, key=lambda k: k["avg_launch_ms"])
-    log(args, f"{s_per_iter:.3f} s/iter; " + "; ".join(
+    log(args, f"{s_per_iter:.3f} s/iter ({'graph' if graph else 'calls'}); " + "; ".join(
         f"{k['kernel']}: {k['avg_launch_ms']:.2f} ms, {k['achieved']:.0f} GB/s alg" for k in kernels))
 
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args, sc, im, pp, d1, d2, tof, bounds, x)
-    out = {
+    return {
         "metric": "fwd+back event-projections/s, s/OSEM iter",
         "value": ev_proj_per_s,
         "unit": "event-projections/s",
@@ -241,24 +327,244 @@ def main():
                                "100M events, 200x200x128 @2mm, 10 subsets",
                    "events": n, "subsets": S, "rays": 4, "tof_fwhm_ps": 250.0,
                    "tof_n_std": 5, "image": [200, 200, 128], "voxel_mm": 2.0,
-                   "lanes_per_event": args.lpe, "block": args.block,
-                   "parallelism": "single GPU"},
+                   "lanes_per_event": lpe, "accumulation": args.accum,
+                   "event_order": args.order,
+                   "graph": graph is not None, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": dom["achieved"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": dom["frac"], "traffic": None,
                      "kernel": dom["kernel"], "avg_launch_ms": dom["avg_launch_ms"],
                      "alg_bytes_per_launch": dom["alg_bytes_per_launch"],
                      "segments_per_event": sum(segs) / n},
         "kernels": kernels,
-        "accumulation": args.accum,
+        "tuning": tune,
+        "tuning_s": tune_s,
         "sensitivity_s": sens_s,
         "sensitivity_lors": n_lors,
+        "order_s": order_s,
         "plan_s": plan_s,
         "plan_bytes": plan_bytes,
         "plan_records": plan_records,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out))
-    return 0
+
+
+def bench_frames(args, torch, dev, stream):
+    """Batched dynamic frames vs the plain per-frame path (see docstring)."""
+    import yrtmi
+    from yrtmi import synth
+
+    sc, im1 = scanner_and_image()
+    F, S = args.frames, args.subsets
+    rng = np.random.default_rng(args.seed + 1)
+    counts = np.exp(rng.uniform(math.log(args.frame_min), math.log(args.frame_max), F))
+    counts = [int(c) for c in counts]
+    act = phantom(args, im1)
+    t0 = time.time()
+    evs = [synth.sample_events(sc, im1, act, c, seed=args.seed + 7 * f, tof_fwhm_ps=250.0,
+                               device=dev) for f, c in enumerate(counts)]
+    n = sum(counts)
+    log(args, f"{F} frames, {n} events ({min(counts)}..{max(counts)}) in {time.time() - t0:.1f}s")
+    ctx = yrtmi.Context(sc, stream=stream)
+    pp = yrtmi.ProjectorParams(sc, numRays=4)
+    pp.addTOF(250.0, 5)
+    proj = ctx.projector(pp)
+    proj.set_launch(args.lpe)
+    sens = torch.full(im1.shape, 1.0, device=dev)  # a sensitivity of ones: EM-loop timing
+
+    # per-frame subsets, contiguous
+    per = []
+    for f in range(F):
+        e, b = subset_major(torch, evs[f], counts[f], S, dev)
+        per.append([yrtmi.LorSet.pairs(e["d1"][b[s]:b[s + 1]], e["d2"][b[s]:b[s + 1]],
+                                       tof=e["tof"][b[s]:b[s + 1]]) for s in range(S)])
+    imF = yrtmi.ImageParams(im1.nx, im1.ny, im1.nz, im1.vx, im1.vy, im1.vz, nt=F)
+    xF = torch.full(imF.shape, 0.125, device=dev)
+    uF = torch.zeros(imF.shape, device=dev)
+    x1 = [xF[f:f + 1] for f in range(F)]
+    u1 = [uF[f:f + 1] for f in range(F)]
+
+    def plain(planned=None):
+        for f in range(F):
+            for s in range(S):
+                ctx.fill(u1[f], 0.0)
+                ctx.em_subset(proj, im1, x1[f], u1[f], per[f][s],
+                              plan=None if planned is None else planned[f][s])
+                ctx.em_update(x1[f], u1[f], sens, im1.nvox)
+
+    def timed(fn, steps):
+        fn()
+        ctx.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        ctx.synchronize()
+        return (time.perf_counter() - t) / steps
+
+    res = {}
+    proj.set_accumulation(True)
+    res["plain_fused_s"] = timed(plain, args.steps)
+    log(args, f"plain (per frame, fused EM): {res['plain_fused_s']:.3f} s/iter")
+    planned = [[ctx.plan(proj, im1, per[f][s]) for s in range(S)] for f in range(F)]
+    res["plain_tiled_s"] = timed(lambda: plain(planned), args.steps)
+    log(args, f"per frame, tile plans: {res['plain_tiled_s']:.3f} s/iter")
+    del planned
+    # batched: one event set per subset over all frames, frame per event
+    cat = lambda key, s: torch.cat([getattr(per[f][s], key) for f in range(F)]).contiguous()
+    bsub = []
+    for s in range(S):
+        fr = torch.cat([torch.full((per[f][s].n,), f, dtype=torch.int32, device=dev)
+                        for f in range(F)])
+        bsub.append(yrtmi.LorSet.pairs(cat("d1", s), cat("d2", s), tof=cat("tof", s), frame=fr))
+    tune = ctx.autotune(proj, imF, bsub[0], xF)
+    log(args, f"autotune: {tune}")
+    bplans = [ctx.plan(proj, imF, L, tile_shape=tune["tile_shape"],
+                       persistent=tune["persistent"]) for L in bsub]
+
+    def batched():
+        for s in range(S):
+            ctx.fill(uF, 0.0)
+            ctx.em_subset(proj, imF, xF, uF, bsub[s], plan=bplans[s])
+            ctx.em_update(xF, uF, sens, im1.nvox)
+
+    batched()
+    ctx.synchronize()
+    with ctx.graph() as g:
+        batched()
+    res["batched_s"] = timed(g.launch, args.steps)
+    log(args, f"batched (one launch per subset for all frames, graph): {res['batched_s']:.3f} s/iter")
+    value = 2.0 * n / res["batched_s"]
+    return {
+        "metric": "fwd+back event-projections/s, batched dynamic frames vs plain per-frame",
+        "value": value,
+        "unit": "event-projections/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": 1,
+        "ms_per_step": res["batched_s"] * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded phantom, physically sampled TOF list-mode events)",
+        "config": {"workload": f"{F} dynamic frames, log-uniform [{args.frame_min:g}, "
+                               f"{args.frame_max:g}] events, 4-ray Siddon TOF 250 ps, "
+                               f"200x200x128 @2mm, {S} subsets",
+                   "frame_events": counts, "events": n},
+        "plain_fused_value": 2.0 * n / res["plain_fused_s"],
+        "plain_tiled_value": 2.0 * n / res["plain_tiled_s"],
+        "speedup_vs_plain_fused": res["plain_fused_s"] / res["batched_s"],
+        "speedup_vs_per_frame_tiled": res["plain_tiled_s"] / res["batched_s"],
+        "seconds": res, "tuning": tune,
+    }
+
+
+def bench_histogram(args, torch, dev, stream):
+    """Histogram OSEM, full size, GPU (C++ host layer) and CPU oracle."""
+    import oracle as O
+    import pyyrtmi as Y
+    import yrtmi
+    from yrtmi import synth
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    from helpers import oracle_image, oracle_scanner, rel_l2
+
+    a = ("cfg1", 96.0, 4.0, 4.0, 20.0, 300.0, 256, 24)
+    sc, ysc = yrtmi.Scanner(*a, maxRingDiff=23), Y.Scanner(*a, 1, 23, 1, 1)
+    im = yrtmi.ImageParams(96, 96, 48, 2.0, 2.0, 2.0)
+    yim = Y.ImageParams.fromParams(96, 96, 48, 2.0, 2.0, 2.0)
+    S, iters = 8, 5
+    rng = np.random.default_rng(args.seed)
+    spheres = synth.random_spheres_in_cylinder(rng, 100.0, 90.0, [5.0, 8.0, 11.0, 15.0], 4.0)
+    act = synth.cylinder_spheres(im, 100.0, 90.0, 1.0, spheres, supersample=2)
+    body = synth.cylinder_spheres(im, 100.0, 90.0, 1.0, [], supersample=1)
+    mu = np.where(body > 0, np.float32(0.096), np.float32(0.0)).astype(np.float32)  # cm^-1
+    # measured histogram: Poisson(attenuated forward projection, 50 M counts)
+    ctx = yrtmi.Context(sc, stream=stream)
+    pp = yrtmi.ProjectorParams(sc)
+    proj = ctx.projector(pp)
+    L = yrtmi.LorSet.histo_subset(ctx, 1, 0)
+    act_t = torch.from_numpy(act.astype(np.float32)).to(dev)
+    mu_t = torch.from_numpy(mu).to(dev)
+    y = torch.empty(L.n, device=dev)
+    ctx.forward(proj, im, act_t, L, y)
+    acf = torch.empty(L.n, device=dev)
+    ctx.attenuation_factors(im, mu_t, L, acf)
+    y = y * acf
+    y = y * (50e6 / float(y.sum()))
+    g = torch.Generator(device=dev)
+    g.manual_seed(args.seed)
+    y = torch.poisson(y, generator=g).contiguous()
+    ctx.synchronize()
+    counts = float(y.sum())
+    log(args, f"histogram: {L.n} bins, {counts:.3e} counts, {int((y > 0).sum())} nonzero")
+    # GPU: the C++ host layer's OSEM
+    osem = Y.createOSEM(ysc)
+    yctx = osem.getContext()
+    h = Y.Histogram3DDeviceAlias(yctx, y)
+    mu_img = Y.ImageDeviceAlias(yctx, yim, mu_t)
+    osem.setImageParams(yim)
+    osem.setAttenuationImage(mu_img)
+    osem.num_OSEM_subsets = S
+    osem.num_MLEM_iterations = iters
+    osem.setDataInput(h)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    osem.generateSensitivityImages()
+    yctx.synchronize()
+    t_sens = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    x = osem.reconstruct()
+    t_rec = time.perf_counter() - t0
+    xg = x.transferToHostMemory()
+    log(args, f"GPU: sensitivity {t_sens:.3f} s, reconstruct ({iters} x {S}) {t_rec:.3f} s")
+    # CPU: the oracle's restatement of the reference CPU OSEM on the same data
+    nthreads = max(1, min(16, os.cpu_count() or 1))
+    osc = oracle_scanner(sc)
+    oi = oracle_image(im)
+    po = O.proj_for_scanner(osc)
+    pos, ori = O.lut(osc)
+    zb, ph, nr = O.histo_dims(osc)
+    assert zb * ph * nr == L.n, "histogram sizes differ"
+    d1, d2 = O.histo_pairs(osc, np.arange(L.n))
+    OL = O.lors_from_pairs(pos, ori, d1, d2)
+    yh = y.cpu().numpy()
+    if args.no_cpu_baseline:
+        return _histo_result(L.n, counts, iters, S, t_rec, t_sens, None)
+    t0 = time.perf_counter()
+    osens = O.sensitivity_images(osc, oi, po, S, False, att_im=oi, att_img=mu, nthreads=nthreads,
+                                 pos_ori=(pos, ori))
+    c_sens = time.perf_counter() - t0
+    acf_o = O.attenuation_factors(oi, mu, OL, nthreads)
+    subsets = [O.histo_subset_bins(osc, S, s) for s in range(S)]
+    t0 = time.perf_counter()
+    xo = O.reconstruct(osc, oi, po, OL, osens, iters, S, False, meas=yh, att=acf_o,
+                       subset_indices=subsets, nthreads=nthreads)
+    c_rec = time.perf_counter() - t0
+    r = rel_l2(xg, xo)
+    log(args, f"CPU ({nthreads} threads): sensitivity {c_sens:.1f} s, reconstruct {c_rec:.1f} s; "
+              f"final images rel L2 {r:.2e}")
+    cpu = {"value": L.n * iters / c_rec, "unit": "bins/s", "cores": nthreads, "kind": "port",
+           "sample": "the full reconstruction (5 iterations x 8 subsets) by the oracle's FP32 "
+                     "restatement of the reference CPU OSEM",
+           "seconds": c_rec, "s_per_osem_iter": c_rec / iters, "sensitivity_s": c_sens}
+    out = _histo_result(L.n, counts, iters, S, t_rec, t_sens, cpu)
+    out["parity_rel_l2_vs_cpu"] = r
+    return out
+
+
+def _histo_result(nbins, counts, iters, S, t_rec, t_sens, cpu):
+    return {
+        "metric": "histogram bins reconstructed/s, s/OSEM iter (histogram OSEM)",
+        "value": nbins * iters / t_rec,
+        "unit": "bins/s",
+        "n_gpus": 1, "steps": iters, "warmup": 0,
+        "ms_per_step": t_rec / iters * 1e3,
+        "s_per_osem_iter": t_rec / iters,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded phantom; Poisson histogram of 50 M counts)",
+        "config": {"workload": "histogram OSEM, Siddon, no TOF, 24x256 scanner R300, "
+                               f"96x96x48 @2mm, {iters} iterations x {S} subsets, attenuation",
+                   "bins": nbins, "counts": counts,
+                   "bins_counted": "every histogram bin of an iteration; the GPU path skips "
+                                   "zero-count bins, the CPU path projects them"},
+        "sensitivity_s": t_sens,
+        "cpu_baseline": cpu,
+    }
 
 
 def cpu_baseline(args, sc, im, pp, d1, d2, tof, bounds, x):

@@ -133,6 +133,30 @@ int yrtmi_ctx_create(int device, void* stream, yrtmi_ctx** out);
 int yrtmi_ctx_destroy(yrtmi_ctx* ctx);
 int yrtmi_ctx_set_stream(yrtmi_ctx* ctx, void* stream);
 int yrtmi_ctx_synchronize(yrtmi_ctx* ctx);
+/* Give the context a stream of its own (created here, default priority,
+ * destroyed with the context) and make it the context's stream; *stream
+ * (HOST, nullable) receives it.  Graph capture needs a stream other than the
+ * null stream. */
+int yrtmi_ctx_own_stream(yrtmi_ctx* ctx, void** stream);
+
+/* ---- graphs: a recorded sequence of calls, replayed as one launch -------
+ * No counterpart in the reference (which launches every kernel from the
+ * host each time).  yrtmi_graph_begin starts recording the context's stream
+ * (not the null stream): the calls made on the context until
+ * yrtmi_graph_end are recorded, not run; yrtmi_graph_launch replays them
+ * with the same buffers.  Recordable: forward / back / em_subset /
+ * em_ratio / the *_planned calls / sensitivity / em_update / threshold /
+ * fill / axpby / gather / subset_interleaved / permute / forward_f64 /
+ * back_f64.
+ * Calls that allocate, copy or synchronise (plan_create, count_segments,
+ * histo_subset_nonzero, lor_order, copy, device_alloc/free, lut / histo /
+ * motion uploads, and a first per-XCD backprojection that would allocate its
+ * partial images) fail with YRTMI_ESTATE while recording. */
+typedef struct yrtmi_graph yrtmi_graph;
+int yrtmi_graph_begin(yrtmi_ctx* ctx);
+int yrtmi_graph_end(yrtmi_ctx* ctx, yrtmi_graph** out);
+int yrtmi_graph_launch(yrtmi_ctx* ctx, const yrtmi_graph* g);
+int yrtmi_graph_destroy(yrtmi_graph* g);
 
 /* Detector LUT (host arrays n_det x 3 each), kept resident on the device as
  * SoA.  Replaces the per-batch host LOR precompute (LORsDevice.cu:28-55,
@@ -296,14 +320,33 @@ int yrtmi_histo_subset_gather(yrtmi_ctx* ctx, const float* histo, int S, int s,
 int yrtmi_axpby(yrtmi_ctx* ctx, int64_t n, float a, const float* x, float b,
                 float* y);
 
+/* ---- locality order ------------------------------------------------------
+ * No counterpart in the reference.  Its EM update does not depend on the
+ * order in which a subset's LORs are projected (OSEM_CPU.cpp:329-374 sums
+ * their backprojections; the order changes FP32 rounding only), so a caller
+ * that owns a subset's arrays (OSEM over a list-mode subset) may reorder them.
+ * yrtmi_lor_order writes perm (DEVICE uint32[n]) = 0..n-1 sorted stably by a
+ * key of each LOR's line: dynamic frame, axial midpoint (as a voxel slice of
+ * im), axial tilt, transaxial angle, transaxial offset.  LORs close in that
+ * order cross nearly the same voxels, so the waves that project them
+ * concurrently read a slab of the image that stays in L2.  Skipped LORs
+ * (frame out of range, bad motion index) go last.  n < 2^32, nz <= 1024.
+ * Synchronises; allocates temporaries (about 24 bytes per LOR).
+ * yrtmi_permute: dst[i] = src[perm[i]] for i < n, elements of elem_bytes
+ * (1, 2, 4 or 8) bytes, DEVICE arrays, dst distinct from src. */
+int yrtmi_lor_order(yrtmi_ctx* ctx, const yrtmi_image* im,
+                    const yrtmi_lors* lors, uint32_t* perm);
+int yrtmi_permute(yrtmi_ctx* ctx, const void* src, int elem_bytes,
+                  const uint32_t* perm, int64_t n, void* dst);
+
 /* ---- tile plans (Siddon backprojection through LDS image tiles) --------- */
 /* No counterpart in the reference: a performance extension of the
  * backprojection behind yrtmi_back / yrtmi_em_subset / yrtmi_sensitivity
  * (OperatorProjectorSiddon_GPU::applyAH, OSEM_GPU::computeEMUpdateImage,
  * OSEM_GPU::generateSensitivityImageForCurrentSubset).
  *
- * A plan bins the Siddon walk of every (LOR, ray) of `lors` by the
- * 32 x 32 x 16-voxel image tiles it crosses, recording where it enters each.
+ * A plan bins the Siddon walk of every (LOR, ray) of `lors` by the image
+ * tiles it crosses, recording where it enters each.
  * It depends on geometry only (detector pairs or bins, TOF values, frames,
  * motion frames, the projector, the image geometry), not on image or LOR
  * values: a list-mode or histogram subset is planned once per
@@ -313,18 +356,53 @@ int yrtmi_axpby(yrtmi_ctx* ctx, int64_t n, float a, const float* x, float b,
  * binary fractions, ulp-level shifts of segment ends at tile boundaries).
  *
  * Limits: Siddon only; nx, ny, nz <= 1024; n * num_rays < 2^32; at most 8192
- * tiles (frames included).  chunk_records: records per workgroup
- * (0: 4096).  The call synchronises the context's stream and allocates
- * device memory (yrtmi_plan_info's bytes); the plan owns it. */
+ * tiles (frames included).  The call synchronises the context's stream and
+ * allocates device memory (yrtmi_plan_info's bytes); the plan owns it.
+ *
+ * Options (NULL: defaults): chunk_records, records per workgroup (0: 4096);
+ * tile_shape, YRTMI_TILE_32x32x16 (64 KB of LDS, default),
+ * YRTMI_TILE_16x16x16 (16 KB) or YRTMI_TILE_32x32x32 (128 KB); persistent,
+ * 0 for one workgroup per chunk, else that many workgroups taking chunks from
+ * a device queue (changeable with yrtmi_plan_set_persistent). */
+#define YRTMI_TILE_32x32x16 0
+#define YRTMI_TILE_16x16x16 1
+#define YRTMI_TILE_32x32x32 2
+typedef struct
+{
+	int chunk_records;
+	int tile_shape;
+	int persistent;
+} yrtmi_plan_options;
 typedef struct yrtmi_plan yrtmi_plan;
 int yrtmi_plan_create(yrtmi_ctx* ctx, const yrtmi_proj* p,
                       const yrtmi_image* im, const yrtmi_lors* lors,
-                      int chunk_records, yrtmi_plan** out);
+                      const yrtmi_plan_options* options, yrtmi_plan** out);
 int yrtmi_plan_destroy(yrtmi_plan* plan);
+int yrtmi_plan_set_persistent(yrtmi_plan* plan, int workgroups);
 /* records: (LOR, ray, tile) entries; chunks: workgroups per use; bytes:
  * device memory held.  Host pointers, any may be NULL. */
 int yrtmi_plan_info(const yrtmi_plan* plan, int64_t* records, int64_t* chunks,
                     int64_t* bytes);
+/* Launch-shape autotuner (no reference counterpart).  On `lors` and image x
+ * it times the EM ratio kernel for lanes per event 1, 2, 4 and 8 and the
+ * planned backprojection for every tile shape, each with one workgroup per
+ * chunk and as a persistent launch (one plan per shape is built and freed),
+ * then sets the projector's lanes per event to the fastest and returns the
+ * fastest settings; plans made afterwards with best.tile_shape /
+ * best.persistent use them.  Synchronises and allocates temporaries. */
+typedef struct
+{
+	int lanes_per_event;
+	int tile_shape;
+	int persistent;
+	float ratio_ms;          /* EM ratio kernel, best lanes per event */
+	float back_ms;           /* planned backprojection, best settings */
+	float ratio_ms_lpe[4];   /* lanes per event 1, 2, 4, 8 */
+	float back_ms_shape[3][2]; /* [tile shape][one per chunk, persistent] */
+} yrtmi_tuning;
+int yrtmi_autotune(yrtmi_ctx* ctx, yrtmi_proj* p, const yrtmi_image* im,
+                   const yrtmi_lors* lors, const float* x, yrtmi_tuning* best);
+
 /* As yrtmi_back / yrtmi_em_subset / yrtmi_sensitivity through `plan`, which
  * must come from the same projector, image geometry and LOR set (checked:
  * projector, geometry and LOR descriptor fields; for the sensitivity call

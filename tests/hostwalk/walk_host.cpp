@@ -97,7 +97,7 @@ float hw_forward(const float* imp, const float* img, const float* p1, const floa
                  int tof, int s64, const float* tofc, float tofv, int* nseg)
 {
 	const DImage P = image(imp);
-	const DTof T{tofc[0], tofc[1], tofc[2]};
+	const DTof T{tofc[0], tofc[1], tofc[2], rcpNearest(tofc[0])};
 	const float3 a = make_float3(p1[0], p1[1], p1[2]), b = make_float3(p2[0], p2[1], p2[2]);
 	uint32_t c = 0, c2 = 0;
 	float v;
@@ -133,7 +133,7 @@ double hw_tiles(const float* imp, const float* p1, const float* p2, int tof, int
                 const float* tofc, float tofv, int TX, int TY, int TZ, int* out)
 {
 	const DImage P = image(imp);
-	const DTof T{tofc[0], tofc[1], tofc[2]};
+	const DTof T{tofc[0], tofc[1], tofc[2], rcpNearest(tofc[0])};
 	const float3 a = make_float3(p1[0], p1[1], p1[2]), b = make_float3(p2[0], p2[1], p2[2]);
 	if (tof && s64)
 		return tiles<true, true>(P, a, b, T, tofv, TX, TY, TZ, out);

@@ -326,15 +326,15 @@ PYBIND11_MODULE(pyyrtmi, m)
 	         [](OSEM_MI& o, const std::vector<const ImageDevice*>& imgs) {
 		         o.setSensitivityImages(imgs);
 	         })
-	    .def(
-	        "generateSensitivityImages",
-	        [](OSEM_MI& o) {
-		        py::list out;
-		        for (auto& s : o.generateSensitivityImages())
-			        out.append(py::cast(s.get(), py::return_value_policy::reference));
-		        return out;
-	        },
-	        py::keep_alive<0, 1>())
+	    .def("generateSensitivityImages",
+	         [](py::object self) {
+		         // each image is owned by the OSEM object, which it keeps alive
+		         py::list out;
+		         for (auto& s : self.cast<OSEM_MI&>().generateSensitivityImages())
+			         out.append(
+			             py::cast(s.get(), py::return_value_policy::reference_internal, self));
+		         return out;
+	         })
 	    .def("reconstruct", &OSEM_MI::reconstruct, py::keep_alive<0, 1>())
 	    .def("getProjectorParams", [](OSEM_MI& o) -> ProjectorParams& { return o.projectorParams; },
 	         py::return_value_policy::reference_internal)
@@ -344,6 +344,7 @@ PYBIND11_MODULE(pyyrtmi, m)
 	    .def_readwrite("hardThreshold", &OSEM_MI::hardThreshold)
 	    .def_readwrite("denomThreshold", &OSEM_MI::denomThreshold)
 	    .def_readwrite("useTilePlans", &OSEM_MI::useTilePlans)
+	    .def_readwrite("useLorOrder", &OSEM_MI::useLorOrder)
 	    .def_readwrite("lanesPerEvent", &OSEM_MI::lanesPerEvent);
 
 	m.def(

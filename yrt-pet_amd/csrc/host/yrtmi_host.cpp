@@ -551,7 +551,7 @@ std::unique_ptr<PlanHandle> makePlan(const Context& ctx, const OperatorProjector
                                      const yrtmi_image& im, const yrtmi_lors& l)
 {
 	yrtmi_plan* q = nullptr;
-	check(yrtmi_plan_create(ctx.get(), proj.get(), &im, &l, 0, &q), "tile plan");
+	check(yrtmi_plan_create(ctx.get(), proj.get(), &im, &l, nullptr, &q), "tile plan");
 	return std::unique_ptr<PlanHandle>(new PlanHandle(q));
 }
 }  // namespace
@@ -730,6 +730,33 @@ void OSEM_MI::prepareSubsets(OperatorProjectorMI& proj, const ImageParams& imt)
 			q->lors.tof = q->tof ? q->tof->data() : nullptr;
 			q->lors.frame = q->frame ? q->frame->data() : nullptr;
 			q->lors.motion_frame = q->motion ? q->motion->data() : nullptr;
+			if (useLorOrder && n > 1)
+			{
+				// the subset's events in locality order (LORs projected together
+				// read the same voxels); every per-event array follows
+				DeviceArray<uint32_t> perm(*m_ctx, (size_t)n);
+				check(yrtmi_lor_order(c, &ci, &q->lors, perm.data()), "locality order");
+				auto reorder = [&](auto& arr, int bytes) {
+					using A = typename std::remove_reference<decltype(*arr)>::type;
+					if (!arr)
+						return;
+					std::unique_ptr<A> out(new A(*m_ctx, (size_t)n));
+					check(yrtmi_permute(c, arr->data(), bytes, perm.data(), n, out->data()),
+					      "permute");
+					arr = std::move(out);
+				};
+				reorder(q->d1, 4);
+				reorder(q->d2, 4);
+				reorder(q->tof, 4);
+				reorder(q->frame, 4);
+				reorder(q->motion, 1);
+				reorder(q->randoms, 4);
+				q->lors.d1 = q->d1 ? q->d1->data() : nullptr;
+				q->lors.d2 = q->d2 ? q->d2->data() : nullptr;
+				q->lors.tof = q->tof ? q->tof->data() : nullptr;
+				q->lors.frame = q->frame ? q->frame->data() : nullptr;
+				q->lors.motion_frame = q->motion ? q->motion->data() : nullptr;
+			}
 			if (m_att != nullptr && n > 0)
 			{
 				// per-event attenuation factors, once (precomputeAttenuationFactors)

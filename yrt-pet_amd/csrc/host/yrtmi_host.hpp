@@ -339,6 +339,9 @@ public:
 	ProjectorParams projectorParams;
 	// Backprojection through tile plans (Siddon): built once per subset.
 	bool useTilePlans = true;
+	// Project each list-mode subset's events in locality order
+	// (yrtmi_lor_order); the EM update does not depend on the order.
+	bool useLorOrder = true;
 	int lanesPerEvent = 0;  // 0: chosen from the number of rays
 
 	const Context& getContext() const { return *m_ctx; }

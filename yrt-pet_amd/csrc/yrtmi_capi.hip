@@ -5,6 +5,7 @@
 #include "yrtmi_kernels.hpp"
 #include "yrtmi_tiles.hpp"
 #include "yrtmi_f64.hpp"
+#include "yrtmi_order.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -59,6 +60,15 @@ struct yrtmi_ctx
 	float* part = nullptr;
 	int64_t partStride = 0;
 	int nxcc = 1;
+	hipStream_t owned = nullptr;  // stream created by yrtmi_ctx_own_stream
+	bool capturing = false;       // between yrtmi_graph_begin and _end
+};
+
+struct yrtmi_graph
+{
+	int device = 0;
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t exec = nullptr;
 };
 
 struct yrtmi_proj
@@ -83,6 +93,14 @@ int setDev(yrtmi_ctx* c)
 	if (c == nullptr)
 		return fail(YRTMI_EINVAL, "null context");
 	HIP_TRY(hipSetDevice(c->device));
+	return YRTMI_OK;
+}
+
+// For calls that allocate, copy or synchronise: refused while recording.
+int notCapturing(yrtmi_ctx* c, const char* what)
+{
+	if (c != nullptr && c->capturing)
+		return fail(YRTMI_ESTATE, std::string(what) + ": not allowed while recording a graph");
 	return YRTMI_OK;
 }
 
@@ -196,6 +214,9 @@ int ensurePartials(yrtmi_ctx* c, int64_t n)
 	const int64_t stride = (n + 63) / 64 * 64;
 	if (c->part != nullptr && c->partStride >= stride)
 		return YRTMI_OK;
+	int r = notCapturing(c, "per-XCD partial images (run one backprojection before recording)");
+	if (r)
+		return r;
 	HIP_TRY(hipStreamSynchronize(c->stream));
 	HIP_TRY(hipFree(c->part));
 	c->part = nullptr;
@@ -281,7 +302,95 @@ int yrtmi_ctx_destroy(yrtmi_ctx* c)
 	(void)hipFree(c->motion);
 	(void)hipFree(c->counter);
 	(void)hipFree(c->part);
+	if (c->owned != nullptr)
+		(void)hipStreamDestroy(c->owned);
 	delete c;
+	return YRTMI_OK;
+}
+
+int yrtmi_ctx_own_stream(yrtmi_ctx* c, void** stream)
+{
+	int r = setDev(c);
+	if (r)
+		return r;
+	r = notCapturing(c, "own_stream");
+	if (r)
+		return r;
+	if (c->owned == nullptr)
+		HIP_TRY(hipStreamCreateWithFlags(&c->owned, hipStreamNonBlocking));
+	c->stream = c->owned;
+	if (stream != nullptr)
+		*stream = (void*)c->owned;
+	return YRTMI_OK;
+}
+
+int yrtmi_graph_begin(yrtmi_ctx* c)
+{
+	int r = setDev(c);
+	if (r)
+		return r;
+	if (c->capturing)
+		return fail(YRTMI_ESTATE, "graph_begin: already recording");
+	if (c->stream == nullptr)
+		return fail(YRTMI_ESTATE,
+		            "graph_begin: the context uses the null stream (yrtmi_ctx_own_stream)");
+	HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+	c->capturing = true;
+	return YRTMI_OK;
+}
+
+int yrtmi_graph_end(yrtmi_ctx* c, yrtmi_graph** out)
+{
+	int r = setDev(c);
+	if (r)
+		return r;
+	if (!c->capturing)
+		return fail(YRTMI_ESTATE, "graph_end: not recording");
+	c->capturing = false;
+	hipGraph_t g = nullptr;
+	HIP_TRY(hipStreamEndCapture(c->stream, &g));
+	if (out == nullptr)
+	{
+		(void)hipGraphDestroy(g);
+		return fail(YRTMI_EINVAL, "graph_end: null output");
+	}
+	hipGraphExec_t x = nullptr;
+	hipError_t e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+	if (e != hipSuccess)
+	{
+		(void)hipGraphDestroy(g);
+		return hipFail(e, "graph_end: instantiate");
+	}
+	auto* q = new yrtmi_graph();
+	q->device = c->device;
+	q->graph = g;
+	q->exec = x;
+	*out = q;
+	return YRTMI_OK;
+}
+
+int yrtmi_graph_launch(yrtmi_ctx* c, const yrtmi_graph* g)
+{
+	int r = setDev(c);
+	if (r)
+		return r;
+	if (g == nullptr || g->device != c->device)
+		return fail(YRTMI_EINVAL, "graph_launch: null graph or another device");
+	r = notCapturing(c, "graph_launch");
+	if (r)
+		return r;
+	HIP_TRY(hipGraphLaunch(g->exec, c->stream));
+	return YRTMI_OK;
+}
+
+int yrtmi_graph_destroy(yrtmi_graph* g)
+{
+	if (g == nullptr)
+		return YRTMI_OK;
+	(void)hipSetDevice(g->device);
+	(void)hipGraphExecDestroy(g->exec);
+	(void)hipGraphDestroy(g->graph);
+	delete g;
 	return YRTMI_OK;
 }
 
@@ -289,6 +398,9 @@ int yrtmi_ctx_set_stream(yrtmi_ctx* c, void* stream)
 {
 	if (c == nullptr)
 		return fail(YRTMI_EINVAL, "null context");
+	int r = notCapturing(c, "ctx_set_stream");
+	if (r)
+		return r;
 	c->stream = (hipStream_t)stream;
 	return YRTMI_OK;
 }
@@ -296,6 +408,9 @@ int yrtmi_ctx_set_stream(yrtmi_ctx* c, void* stream)
 int yrtmi_ctx_synchronize(yrtmi_ctx* c)
 {
 	int r = setDev(c);
+	if (r)
+		return r;
+	r = notCapturing(c, "ctx_synchronize");
 	if (r)
 		return r;
 	HIP_TRY(hipStreamSynchronize(c->stream));
@@ -306,6 +421,9 @@ int yrtmi_lut_upload(yrtmi_ctx* c, const float* pos, const float* orient,
                      int64_t n)
 {
 	int r = setDev(c);
+	if (r)
+		return r;
+	r = notCapturing(c, "lut_upload");
 	if (r)
 		return r;
 	if (pos == nullptr || orient == nullptr || n <= 0 || n >= (int64_t(1) << 31))
@@ -383,6 +501,9 @@ int yrtmi_histo_setup(yrtmi_ctx* c, const yrtmi_histo_desc* h,
                       int64_t* num_z_bin, int64_t* num_phi, int64_t* num_r)
 {
 	int r = setDev(c);
+	if (r)
+		return r;
+	r = notCapturing(c, "histo_setup");
 	if (r)
 		return r;
 	if (h == nullptr || h->dets_per_ring <= 0 || h->num_rings <= 0 ||
@@ -478,6 +599,9 @@ int yrtmi_motion_upload(yrtmi_ctx* c, const float* xf, int k)
 	int r = setDev(c);
 	if (r)
 		return r;
+	r = notCapturing(c, "motion_upload");
+	if (r)
+		return r;
 	if (xf == nullptr || k <= 0 || k > 256)
 		return fail(YRTMI_EINVAL, "motion: need 1..256 transforms");
 	HIP_TRY(hipStreamSynchronize(c->stream));
@@ -521,6 +645,7 @@ int yrtmi_proj_create(yrtmi_ctx* c, const yrtmi_projector_desc* d,
 		q.tof.sigma = (float)(width_mm / (2 * sqrtf(2 * logf(2.0f))));
 		q.tof.trunc = (d->tof_n_std <= 0) ? -1.f : d->tof_n_std * q.tof.sigma;
 		q.tof.norm = (float)(1 / (std::sqrt(2 * 3.14159265358979) * q.tof.sigma));
+		q.tof.rsigma = rcpNearest(q.tof.sigma);
 		q.tof64.sigma = width_mm / (2 * std::sqrt(2 * std::log(2.0)));
 		q.tof64.trunc = (d->tof_n_std <= 0) ? -1.0 : d->tof_n_std * q.tof64.sigma;
 		q.tof64.norm = 1 / (std::sqrt(2 * M_PI) * q.tof64.sigma);
@@ -853,6 +978,9 @@ int yrtmi_count_segments(yrtmi_ctx* c, const yrtmi_proj* p,
 	int r = prepare(c, p, im, l, a);
 	if (r)
 		return r;
+	r = notCapturing(c, "count_segments");
+	if (r)
+		return r;
 	if (total == nullptr)
 		return fail(YRTMI_EINVAL, "count: null output");
 	*total = 0;
@@ -887,6 +1015,8 @@ struct yrtmi_plan
 	uint32_t* vox = nullptr;
 	TileChunk* chunks = nullptr;
 	float* v = nullptr;  // per-LOR values of the em / sensitivity calls
+	unsigned int* queue = nullptr;  // chunk queue of persistent launches
+	int persistent = 0;
 };
 
 namespace
@@ -899,6 +1029,7 @@ void freePlan(yrtmi_plan* q)
 	hipFree(q->vox);
 	hipFree(q->chunks);
 	hipFree(q->v);
+	hipFree(q->queue);
 	delete q;
 }
 
@@ -938,9 +1069,11 @@ DPlanRead planRead(const yrtmi_plan* q)
 	DPlanRead r{};
 	r.R = q->proj->d.numRays;
 	r.chunks = q->chunks;
+	r.nChunks = q->nChunks;
 	r.key = q->key;
 	r.alpha = q->alpha;
 	r.vox = q->vox;
+	r.queue = q->queue;
 	return r;
 }
 
@@ -949,13 +1082,17 @@ DPlanRead planRead(const yrtmi_plan* q)
 extern "C" {
 
 int yrtmi_plan_create(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
-                      const yrtmi_lors* l, int chunk_records, yrtmi_plan** out)
+                      const yrtmi_lors* l, const yrtmi_plan_options* opt,
+                      yrtmi_plan** out)
 {
 	if (out == nullptr)
 		return fail(YRTMI_EINVAL, "plan: null output");
 	*out = nullptr;
 	KArgs a;
 	int r = prepare(c, p, im, l, a);
+	if (r)
+		return r;
+	r = notCapturing(c, "plan_create");
 	if (r)
 		return r;
 	if (p->d.type != YRTMI_PROJ_SIDDON)
@@ -965,13 +1102,22 @@ int yrtmi_plan_create(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
 	const int R = p->d.numRays;
 	if ((l->n * (int64_t)R) >= (int64_t(1) << 32))
 		return fail(YRTMI_EINVAL, "plan: n * num_rays must be below 2^32");
-	if (chunk_records < 0 || chunk_records > (1 << 24))
+	const yrtmi_plan_options o = opt ? *opt : yrtmi_plan_options{0, 0, 0};
+	if (o.chunk_records < 0 || o.chunk_records > (1 << 24))
 		return fail(YRTMI_EINVAL, "plan: chunk_records out of range");
-	const int CH = chunk_records > 0 ? chunk_records : 4096;
+	if (o.tile_shape < 0 || o.tile_shape >= kNumTileShapes)
+		return fail(YRTMI_EINVAL, "plan: unknown tile shape");
+	if (o.persistent < 0)
+		return fail(YRTMI_EINVAL, "plan: persistent workgroups must be >= 0");
+	const int CH = o.chunk_records > 0 ? o.chunk_records : 4096;
 	DTiles T{};
-	T.ntx = (int32_t)((im->nx + kTileX - 1) / kTileX);
-	T.nty = (int32_t)((im->ny + kTileY - 1) / kTileY);
-	T.ntz = (int32_t)((im->nz + kTileZ - 1) / kTileZ);
+	T.shape = o.tile_shape;
+	T.tx = tileDim(T.shape, 0);
+	T.ty = tileDim(T.shape, 1);
+	T.tz = tileDim(T.shape, 2);
+	T.ntx = (int32_t)((im->nx + T.tx - 1) / T.tx);
+	T.nty = (int32_t)((im->ny + T.ty - 1) / T.ty);
+	T.ntz = (int32_t)((im->nz + T.tz - 1) / T.tz);
 	const int64_t nT = (int64_t)T.ntx * T.nty * T.ntz * im->nt;
 	if (nT > kMaxTiles)
 		return fail(YRTMI_EINVAL, "plan: more than 8192 image tiles (frames included)");
@@ -984,6 +1130,13 @@ int yrtmi_plan_create(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
 	q->lors = *l;
 	q->useTof = p->d.useTof;
 	q->T = T;
+	q->persistent = o.persistent;
+	if (hipMalloc(&q->queue, 2 * sizeof(unsigned int)) != hipSuccess ||
+	    hipMemset(q->queue, 0, 2 * sizeof(unsigned int)) != hipSuccess)
+	{
+		freePlan(q);
+		return fail(YRTMI_ENOMEM, "plan: chunk queue");
+	}
 	if (l->n > 0)
 		if (hipMalloc(&q->v, sizeof(float) * l->n) != hipSuccess)
 		{
@@ -1010,21 +1163,35 @@ int yrtmi_plan_create(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
 	};
 	if (l->n == 0)
 		return cleanup(YRTMI_OK);
-	if (hipMalloc(&dCount, sizeof(unsigned long long) * nT) != hipSuccess ||
-	    hipMalloc(&dCursor, sizeof(unsigned int) * nT) != hipSuccess ||
-	    hipMalloc(&dOff, sizeof(int64_t) * nT) != hipSuccess ||
+	// length classes: as many as the LDS counters allow (tiles x classes <=
+	// kMaxBins), cut at fractions of the longest piece a tile can hold
+	// (tx + ty + tz segments)
+	DPlan pl{};
+	pl.R = R;
+	pl.C = 1;
+	while (pl.C < kMaxClasses && nT * pl.C * 2 <= kMaxBins)
+		pl.C *= 2;
+	{
+		const int L = T.tx + T.ty + T.tz;
+		static const int f8[7] = {3, 6, 9, 12, 16, 21, 28}, f4[3] = {6, 12, 21}, f2[1] = {12};
+		const int* f = pl.C == 8 ? f8 : (pl.C == 4 ? f4 : f2);
+		for (int k = 0; k < pl.C - 1; k++)
+			pl.cut[k] = std::max(1, L * f[k] / 40);
+	}
+	const int64_t nB = nT * pl.C;
+	if (hipMalloc(&dCount, sizeof(unsigned long long) * nB) != hipSuccess ||
+	    hipMalloc(&dCursor, sizeof(unsigned int) * nB) != hipSuccess ||
+	    hipMalloc(&dOff, sizeof(int64_t) * nB) != hipSuccess ||
 	    hipMalloc(&dErr, sizeof(int)) != hipSuccess)
 		return cleanup(fail(YRTMI_ENOMEM, "plan: build scratch"));
-	hipError_t e = hipMemsetAsync(dCount, 0, sizeof(unsigned long long) * nT, c->stream);
+	hipError_t e = hipMemsetAsync(dCount, 0, sizeof(unsigned long long) * nB, c->stream);
 	if (e == hipSuccess)
-		e = hipMemsetAsync(dCursor, 0, sizeof(unsigned int) * nT, c->stream);
+		e = hipMemsetAsync(dCursor, 0, sizeof(unsigned int) * nB, c->stream);
 	if (e == hipSuccess)
 		e = hipMemsetAsync(dErr, 0, sizeof(int), c->stream);
 	if (e != hipSuccess)
 		return cleanup(hipFail(e, "plan: clear"));
 
-	DPlan pl{};
-	pl.R = R;
 	pl.count = dCount;
 	pl.cursor = dCursor;
 	pl.tileOff = dOff;
@@ -1032,35 +1199,46 @@ int yrtmi_plan_create(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
 	const int64_t rays = l->n * (int64_t)R;
 	const int grid = (int)std::min<int64_t>((rays + 255) / 256, 2048);
 	e = launchBin(0, a, T, pl, grid, c->stream);
-	std::vector<unsigned long long> cnt(nT);
+	std::vector<unsigned long long> cnt(nB);
 	if (e == hipSuccess)
-		e = hipMemcpyAsync(cnt.data(), dCount, sizeof(unsigned long long) * nT,
+		e = hipMemcpyAsync(cnt.data(), dCount, sizeof(unsigned long long) * nB,
 		                   hipMemcpyDeviceToHost, c->stream);
 	if (e == hipSuccess)
 		e = hipStreamSynchronize(c->stream);
 	if (e != hipSuccess)
 		return cleanup(hipFail(e, "plan: count pass"));
 
-	std::vector<int64_t> off(nT);
+	// bins tile-major, class-minor: a tile's records are one contiguous
+	// range, ordered by class, cut into chunks of CH records
+	std::vector<int64_t> off(nB);
 	std::vector<TileChunk> chunks;
 	int64_t total = 0;
 	for (int64_t t = 0; t < nT; t++)
 	{
-		if (cnt[t] >= (1ull << 31))
+		const int64_t begin = total;
+		for (int k = 0; k < pl.C; k++)
+		{
+			off[t * pl.C + k] = total;
+			total += (int64_t)cnt[t * pl.C + k];
+		}
+		const int64_t n = total - begin;
+		if (n >= (int64_t(1) << 31))
 			return cleanup(fail(YRTMI_EINVAL, "plan: more than 2^31 records in one tile"));
-		off[t] = total;
-		for (int64_t b = 0; b < (int64_t)cnt[t]; b += CH)
+		for (int64_t b = 0; b < n; b += CH)
 		{
 			TileChunk k;
-			k.begin = total + b;
+			k.begin = begin + b;
 			k.tile = (int32_t)t;
-			k.count = (int32_t)std::min<int64_t>(CH, (int64_t)cnt[t] - b);
+			k.count = (int32_t)std::min<int64_t>(CH, n - b);
 			chunks.push_back(k);
 		}
-		total += (int64_t)cnt[t];
 	}
 	if ((int64_t)chunks.size() >= 0x7fffffffLL)
 		return cleanup(fail(YRTMI_EINVAL, "plan: too many chunks"));
+	// full chunks first, the tiles' partial tail chunks last (they end the
+	// launch while the full ones are still running)
+	std::stable_sort(chunks.begin(), chunks.end(),
+	                 [](const TileChunk& x, const TileChunk& y) { return x.count > y.count; });
 	q->records = total;
 	q->nChunks = (int64_t)chunks.size();
 	if (total > 0)
@@ -1071,7 +1249,7 @@ int yrtmi_plan_create(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
 		    hipMalloc(&q->chunks, sizeof(TileChunk) * chunks.size()) != hipSuccess)
 			return cleanup(fail(YRTMI_ENOMEM, "plan: records"));
 		q->bytes += total * 12 + (int64_t)(sizeof(TileChunk) * chunks.size());
-		e = hipMemcpyAsync(dOff, off.data(), sizeof(int64_t) * nT,
+		e = hipMemcpyAsync(dOff, off.data(), sizeof(int64_t) * nB,
 		                   hipMemcpyHostToDevice, c->stream);
 		if (e == hipSuccess)
 			e = hipMemcpyAsync(q->chunks, chunks.data(), sizeof(TileChunk) * chunks.size(),
@@ -1081,10 +1259,10 @@ int yrtmi_plan_create(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
 		pl.vox = q->vox;
 		if (e == hipSuccess)
 			e = launchBin(1, a, T, pl, grid, c->stream);
-		std::vector<unsigned int> cur(nT);
+		std::vector<unsigned int> cur(nB);
 		int herr = 0;
 		if (e == hipSuccess)
-			e = hipMemcpyAsync(cur.data(), dCursor, sizeof(unsigned int) * nT,
+			e = hipMemcpyAsync(cur.data(), dCursor, sizeof(unsigned int) * nB,
 			                   hipMemcpyDeviceToHost, c->stream);
 		if (e == hipSuccess)
 			e = hipMemcpyAsync(&herr, dErr, sizeof(int), hipMemcpyDeviceToHost, c->stream);
@@ -1095,8 +1273,8 @@ int yrtmi_plan_create(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
 		if (herr & 2)
 			return cleanup(fail(YRTMI_ESTATE, "plan: a ray's plane shift is out of range"));
 		bool same = herr == 0;
-		for (int64_t t = 0; t < nT && same; t++)
-			same = (unsigned long long)cur[t] == cnt[t];
+		for (int64_t b = 0; b < nB && same; b++)
+			same = (unsigned long long)cur[b] == cnt[b];
 		if (!same)
 			return cleanup(fail(YRTMI_ESTATE, "plan: fill pass disagrees with count pass"));
 	}
@@ -1109,6 +1287,14 @@ int yrtmi_plan_destroy(yrtmi_plan* q)
 		return YRTMI_OK;
 	hipSetDevice(q->device);
 	freePlan(q);
+	return YRTMI_OK;
+}
+
+int yrtmi_plan_set_persistent(yrtmi_plan* q, int workgroups)
+{
+	if (q == nullptr || workgroups < 0)
+		return fail(YRTMI_EINVAL, "plan_set_persistent: bad arguments");
+	q->persistent = workgroups;
 	return YRTMI_OK;
 }
 
@@ -1141,7 +1327,7 @@ int yrtmi_back_planned(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
 		return YRTMI_OK;
 	if (img == nullptr || vals == nullptr)
 		return fail(YRTMI_EINVAL, "back: null image or values");
-	HIP_TRY(launchTileBack(a, q->T, planRead(q), q->nChunks, vals, img, c->stream));
+	HIP_TRY(launchTileBack(a, q->T, planRead(q), vals, img, q->persistent, c->stream));
 	return YRTMI_OK;
 }
 
@@ -1200,7 +1386,7 @@ int yrtmi_em_subset_planned(yrtmi_ctx* c, const yrtmi_proj* p,
 	a.x = x;
 	setCorr(a, corr);
 	HIP_TRY(launchEmRatio(a, p->lpe, q->v, c->stream));
-	HIP_TRY(launchTileBack(a, q->T, planRead(q), q->nChunks, q->v, u, c->stream));
+	HIP_TRY(launchTileBack(a, q->T, planRead(q), q->v, u, q->persistent, c->stream));
 	return YRTMI_OK;
 }
 
@@ -1236,7 +1422,7 @@ int yrtmi_sensitivity_planned(yrtmi_ctx* c, const yrtmi_proj* p,
 	a.vals = sens;
 	a.L.tof = nullptr;
 	HIP_TRY(launchSensValue(a, q->v, c->stream));
-	HIP_TRY(launchTileBack(a, q->T, planRead(q), q->nChunks, q->v, img, c->stream));
+	HIP_TRY(launchTileBack(a, q->T, planRead(q), q->v, img, q->persistent, c->stream));
 	return YRTMI_OK;
 }
 
@@ -1247,6 +1433,9 @@ extern "C" {
 int yrtmi_device_alloc(yrtmi_ctx* c, int64_t bytes, void** out)
 {
 	int r = setDev(c);
+	if (r)
+		return r;
+	r = notCapturing(c, "device_alloc");
 	if (r)
 		return r;
 	if (out == nullptr || bytes < 0)
@@ -1267,6 +1456,9 @@ int yrtmi_device_free(yrtmi_ctx* c, void* p)
 	int r = setDev(c);
 	if (r)
 		return r;
+	r = notCapturing(c, "device_free");
+	if (r)
+		return r;
 	if (p == nullptr)
 		return YRTMI_OK;
 	HIP_TRY(hipStreamSynchronize(c->stream));
@@ -1277,6 +1469,9 @@ int yrtmi_device_free(yrtmi_ctx* c, void* p)
 int yrtmi_copy(yrtmi_ctx* c, void* dst, const void* src, int64_t bytes, int kind)
 {
 	int r = setDev(c);
+	if (r)
+		return r;
+	r = notCapturing(c, "copy");
 	if (r)
 		return r;
 	if (bytes < 0 || (bytes > 0 && (dst == nullptr || src == nullptr)))
@@ -1307,6 +1502,9 @@ int yrtmi_histo_subset_nonzero(yrtmi_ctx* c, const float* values, int S, int s,
                                uint32_t* bins, float* meas, int64_t* count)
 {
 	int r = setDev(c);
+	if (r)
+		return r;
+	r = notCapturing(c, "histo_subset_nonzero");
 	if (r)
 		return r;
 	if (!c->histoReady)
@@ -1436,6 +1634,163 @@ int yrtmi_axpby(yrtmi_ctx* c, int64_t n, float a, const float* x, float b, float
 		return YRTMI_OK;
 	HIP_TRY(launchAxpby(n, a, x, b, y, c->stream));
 	return YRTMI_OK;
+}
+
+int yrtmi_lor_order(yrtmi_ctx* c, const yrtmi_image* im, const yrtmi_lors* l, uint32_t* perm)
+{
+	int r = setDev(c);
+	if (r)
+		return r;
+	r = notCapturing(c, "lor_order");
+	if (r)
+		return r;
+	r = checkImage(im, "image");
+	if (r)
+		return r;
+	if (l == nullptr)
+		return fail(YRTMI_EINVAL, "lor_order: null LOR set");
+	if (l->n >= (int64_t(1) << 32))
+		return fail(YRTMI_EINVAL, "lor_order: n must be below 2^32");
+	if (im->nz > 1024)
+		return fail(YRTMI_EINVAL, "lor_order: nz above 1024");
+	if (l->n > 0 && perm == nullptr)
+		return fail(YRTMI_EINVAL, "lor_order: null output");
+	DLors L;
+	DScanner S;
+	r = buildLors(c, l, L, S);
+	if (r)
+		return r;
+	HIP_TRY(lorOrder(S, L, toDImage(im), perm, c->stream));
+	return YRTMI_OK;
+}
+
+int yrtmi_permute(yrtmi_ctx* c, const void* src, int elem_bytes, const uint32_t* perm,
+                  int64_t n, void* dst)
+{
+	int r = setDev(c);
+	if (r)
+		return r;
+	if (elem_bytes != 1 && elem_bytes != 2 && elem_bytes != 4 && elem_bytes != 8)
+		return fail(YRTMI_EINVAL, "permute: elem_bytes must be 1, 2, 4 or 8");
+	if (n < 0 || (n > 0 && (src == nullptr || perm == nullptr || dst == nullptr)))
+		return fail(YRTMI_EINVAL, "permute: bad arguments");
+	if (n > 0 && src == dst)
+		return fail(YRTMI_EINVAL, "permute: dst must not be src");
+	HIP_TRY(launchPermute(src, elem_bytes, perm, n, dst, c->stream));
+	return YRTMI_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int yrtmi_autotune(yrtmi_ctx* c, yrtmi_proj* p, const yrtmi_image* im,
+                   const yrtmi_lors* l, const float* x, yrtmi_tuning* best)
+{
+	KArgs a;
+	int r = prepare(c, p, im, l, a);
+	if (r)
+		return r;
+	r = notCapturing(c, "autotune");
+	if (r)
+		return r;
+	if (best == nullptr || x == nullptr)
+		return fail(YRTMI_EINVAL, "autotune: null argument");
+	if (p->d.type != YRTMI_PROJ_SIDDON)
+		return fail(YRTMI_EINVAL, "autotune: Siddon projector only");
+	*best = yrtmi_tuning{};
+	if (l->n == 0)
+		return YRTMI_OK;
+	int cus = 256;
+	(void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device);
+	float* v = nullptr;
+	float* y = nullptr;
+	hipEvent_t e0 = nullptr, e1 = nullptr;
+	const int64_t ny = a.im.nvox * a.im.nt;
+	auto done = [&](int rc) {
+		hipFree(v);
+		hipFree(y);
+		if (e0)
+			hipEventDestroy(e0);
+		if (e1)
+			hipEventDestroy(e1);
+		return rc;
+	};
+	if (hipMalloc(&v, sizeof(float) * l->n) != hipSuccess ||
+	    hipMalloc(&y, sizeof(float) * ny) != hipSuccess)
+		return done(fail(YRTMI_ENOMEM, "autotune: temporaries"));
+	if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+		return done(fail(YRTMI_EHIP, "autotune: events"));
+	a.x = x;
+	a.c.g = 1.0f;
+	a.c.denomThr = 1e-35f;
+	// time fn(): one untimed run, then the mean of two timed runs
+	auto timeIt = [&](auto&& fn, float& ms) -> hipError_t {
+		hipError_t e = fn();
+		if (e != hipSuccess)
+			return e;
+		e = hipEventRecord(e0, c->stream);
+		for (int k = 0; k < 2 && e == hipSuccess; k++)
+			e = fn();
+		if (e == hipSuccess)
+			e = hipEventRecord(e1, c->stream);
+		if (e == hipSuccess)
+			e = hipEventSynchronize(e1);
+		if (e == hipSuccess)
+			e = hipEventElapsedTime(&ms, e0, e1);
+		ms *= 0.5f;
+		return e;
+	};
+	const int lpes[4] = {1, 2, 4, 8};
+	best->ratio_ms = 1e30f;
+	for (int i = 0; i < 4; i++)
+	{
+		float ms = 0.f;
+		hipError_t e = timeIt([&]() { return launchEmRatio(a, lpes[i], v, c->stream); }, ms);
+		if (e != hipSuccess)
+			return done(hipFail(e, "autotune: EM ratio"));
+		best->ratio_ms_lpe[i] = ms;
+		if (ms < best->ratio_ms)
+		{
+			best->ratio_ms = ms;
+			best->lanes_per_event = lpes[i];
+		}
+	}
+	p->lpe = best->lanes_per_event;
+	// v now holds the EM ratios of x: backproject them through each shape
+	best->back_ms = 1e30f;
+	for (int shape = 0; shape < kNumTileShapes; shape++)
+	{
+		yrtmi_plan_options o{0, shape, 0};
+		yrtmi_plan* q = nullptr;
+		r = yrtmi_plan_create(c, p, im, l, &o, &q);
+		if (r)
+			return done(r);
+		// resident workgroups: LDS per CU (160 KB) over the tile's LDS
+		const int perCu = shape == 1 ? 8 : (shape == 2 ? 1 : 2);
+		for (int mode = 0; mode < 2; mode++)
+		{
+			const int pers = mode ? perCu * cus : 0;
+			float ms = 0.f;
+			hipError_t e = timeIt(
+			    [&]() { return launchTileBack(a, q->T, planRead(q), v, y, pers, c->stream); },
+			    ms);
+			if (e != hipSuccess)
+			{
+				yrtmi_plan_destroy(q);
+				return done(hipFail(e, "autotune: tile backprojection"));
+			}
+			best->back_ms_shape[shape][mode] = ms;
+			if (ms < best->back_ms)
+			{
+				best->back_ms = ms;
+				best->tile_shape = shape;
+				best->persistent = pers;
+			}
+		}
+		yrtmi_plan_destroy(q);
+	}
+	return done(YRTMI_OK);
 }
 
 }  // extern "C"

@@ -4,8 +4,11 @@
 //
 // Arithmetic: FP32 storage and accumulation; the ray-plane intersection
 // setup (FOV-cylinder clip, slab alphas, TOF window, first crossings, first
-// voxel index) runs in FP64 for near-grazing stability.
+// voxel index) in FP32 in the reference's operation order, or in FP64 on
+// request (near-grazing stability).
 #pragma once
+
+#include <cmath>
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -34,11 +37,33 @@ struct DImage
 };
 
 // TimeOfFlightHelper constants (TimeOfFlight.hpp:19-34): sigma, truncation
-// width (<= 0: none) and Gaussian norm, all mm.
+// width (<= 0: none) and Gaussian norm, all mm; rsigma = RN(1/sigma), with
+// which the walk divides by sigma exactly (divBySigma).
 struct DTof
 {
-	float sigma, trunc, norm;
+	float sigma, trunc, norm, rsigma;
 };
+
+// The float nearest 1/s (host side): of the double quotient rounded to float
+// and its two neighbours, the one with the smallest |c*s - 1|, a residual
+// that double evaluates exactly (a product of two floats has 48 bits).
+inline float rcpNearest(float s)
+{
+	const float c = (float)(1.0 / (double)s);
+	float best = c;
+	double be = std::fabs((double)c * (double)s - 1.0);
+	const float cand[2] = {std::nextafter(c, 0.f), std::nextafter(c, 2.f * c)};
+	for (float k : cand)
+	{
+		const double e = std::fabs((double)k * (double)s - 1.0);
+		if (e < be)
+		{
+			be = e;
+			best = k;
+		}
+	}
+	return best;
+}
 
 // The same constants in FP64 (the FP64 projector).
 struct DTof64

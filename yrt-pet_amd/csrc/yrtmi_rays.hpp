@@ -473,13 +473,30 @@ __device__ __forceinline__ WalkBox imageBox(const DImage& P)
 	return WalkBox{0, P.nx, 0, P.ny, 0, P.nz, P.nx, P.nxy, 0};
 }
 
+// x / sigma, correctly rounded, from the correctly rounded reciprocal
+// rs = RN(1/sigma): q = RN(x*rs) is within an ulp and one FMA residual step
+// rounds it to RN(x/sigma) (P. Markstein, IBM J. Res. Dev. 34(1), 1990;
+// finite x away from the overflow and underflow thresholds).
+__device__ __forceinline__ float divBySigma(float x, float sigma, float rs)
+{
+#pragma clang fp contract(off)
+	const float q = x * rs;
+	return fmaf(fmaf(-q, sigma, x), rs, q);
+}
+
 // FP32 walk (ProjectorSiddon.cpp:622-766) from r.a_cur to r.amax.  With
 // `first` the walk starts at the ray's entry and its first voxel is the one
 // holding the midpoint of the first nonempty segment; otherwise it starts in
 // (vx, vy, vz) (after sidRestart).  It ends when the voxel leaves `B`.
-// fn(vx, vy, vz, idx, a_start, w) is called for every nonempty segment.
+// fn(vx, vy, vz, idx, a_start, w) is called for every segment.
+//
+// Each step takes the nearest next plane m = min(axn, ayn, azn); every axis
+// whose plane is at m advances (the reference's three `<=` tests: ties
+// advance together), and the segment ends at min(m, amax).  The steady
+// loop is branch-free per axis (selects), so a wave runs one instruction
+// stream whatever the axes its lanes cross.
 template <bool TOF, bool S64, class Fn>
-__device__ __forceinline__ void sidWalk(const DImage& P, SidRay& r,
+__device__ __forceinline__ void sidWalk(const DImage& P, const SidRay& r,
                                         const DTof& tof, const WalkBox& B,
                                         bool first, int vx, int vy, int vz,
                                         Fn&& fn)
@@ -487,106 +504,149 @@ __device__ __forceinline__ void sidWalk(const DImage& P, SidRay& r,
 #pragma clang fp contract(off)
 	const float sdx = (float)r.dirx * P.vx, sdy = (float)r.diry * P.vy,
 	            sdz = (float)r.dirz * P.vz;
-	float axp = r.axn, ayp = r.ayn, azp = r.azn;
-	float a_cur = r.a_cur;
 	const float amaxf = r.amax;
-	int off = first ? 0 : B.base + vz * B.sz + vy * B.sy;
-	int dprev = 0;
-	while (a_cur < amaxf)
-	{
-		int dn = 0;
-		float a_next = -1.f;
-		if (axp <= ayp && axp <= azp)
-		{
-			a_next = r.axn;
-			r.xc += sdx;
-			r.axn = (r.xc - r.p1.x) * r.invx;
-			dn |= 1;
-		}
-		if (ayp <= axp && ayp <= azp)
-		{
-			a_next = r.ayn;
-			r.yc += sdy;
-			r.ayn = (r.yc - r.p1.y) * r.invy;
-			dn |= 2;
-		}
-		if (azp <= axp && azp <= ayp)
-		{
-			a_next = r.azn;
-			r.zc += sdz;
-			r.azn = (r.zc - r.p1.z) * r.invz;
-			dn |= 4;
-		}
-		if (a_next > amaxf)
-			a_next = amaxf;
-		if (a_cur >= a_next)
-		{
-			if (first)
-			{
-				// planes at or before the ray's start: consumed without a voxel
-				axp = r.axn;
-				ayp = r.ayn;
-				azp = r.azn;
-				continue;
-			}
-			// After the first voxel the walk's planes only come after a_cur
-			// (they advance monotonically and ties are taken together); a walk
-			// re-entered at a tile boundary can meet one recomputed an ulp
-			// early, which is still a crossing: an empty segment, then the move.
-			if (a_cur >= amaxf)
-				break;
-			a_next = a_cur;
-		}
-		if (first)
-		{
-			sidMidVoxel<S64>(P, r, a_cur, a_next, vx, vy, vz);
-			if (vx < B.x0 || vx >= B.x1 || vy < B.y0 || vy >= B.y1 || vz < B.z0 ||
-			    vz >= B.z1)
-				break;
-			off = B.base + vz * B.sz + vy * B.sy;
-			first = false;
-		}
-		else
-		{
-			if (dprev & 1)
-			{
-				vx += r.dirx;
-				if (vx < B.x0 || vx >= B.x1)
-					break;
-			}
-			if (dprev & 2)
-			{
-				vy += r.diry;
-				if (vy < B.y0 || vy >= B.y1)
-					break;
-				off += r.diry * B.sy;
-			}
-			if (dprev & 4)
-			{
-				vz += r.dirz;
-				if (vz < B.z0 || vz >= B.z1)
-					break;
-				off += r.dirz * B.sz;
-			}
-		}
-		dprev = dn;
-		float w = (a_next - a_cur) * r.dnorm;
+	const float dnorm = r.dnorm;
+	float axn = r.axn, ayn = r.ayn, azn = r.azn;
+	float xc = r.xc, yc = r.yc, zc = r.zc;
+	float a_cur = r.a_cur, a_next = 0.f;
+	int dn = 0;
+	auto step = [&]() {
+		const float m = fminf(axn, fminf(ayn, azn));
+		const bool mx = axn == m, my = ayn == m, mz = azn == m;
+		const float xn = xc + sdx, yn = yc + sdy, zn = zc + sdz;
+		xc = mx ? xn : xc;
+		yc = my ? yn : yc;
+		zc = mz ? zn : zc;
+		axn = mx ? (xn - r.p1.x) * r.invx : axn;
+		ayn = my ? (yn - r.p1.y) * r.invy : ayn;
+		azn = mz ? (zn - r.p1.z) * r.invz : azn;
+		dn = (mx ? 1 : 0) | (my ? 2 : 0) | (mz ? 4 : 0);
+		a_next = fminf(m, amaxf);
+	};
+	float acd = 0.f;  // a_cur * dnorm, carried from the previous segment's end
+	auto emit = [&](int off) {
+		float w = (a_next - a_cur) * dnorm;
 		if (TOF)
 		{
 			float xcn;
+			const float and_ = a_next * dnorm;
 			if constexpr (S64)
 				xcn = fmaf(0.5f * (a_cur + a_next), r.tA, -r.tB);
 			else  // TimeOfFlightHelper::getWeight (TimeOfFlight.hpp:175-185)
-				xcn = (0.5f * (a_cur * r.dnorm + a_next * r.dnorm) - r.tB) / tof.sigma;
+				xcn = divBySigma(0.5f * (acd + and_) - r.tB, tof.sigma, tof.rsigma);
+			acd = and_;
 			w *= gaussW(xcn, tof.norm);
 		}
 		fn(vx, vy, vz, off + vx, a_cur, w);
 		a_cur = a_next;
-		axp = r.axn;
-		ayp = r.ayn;
-		azp = r.azn;
+	};
+	int off;
+	int dprev = 0;
+	if (first)
+	{
+		// planes at or before the ray's start are consumed without a voxel
+		for (;;)
+		{
+			if (!(a_cur < amaxf))
+				return;
+			step();
+			if (a_cur < a_next)
+				break;
+		}
+		sidMidVoxel<S64>(P, r, a_cur, a_next, vx, vy, vz);
+		if (vx < B.x0 || vx >= B.x1 || vy < B.y0 || vy >= B.y1 || vz < B.z0 || vz >= B.z1)
+			return;
+		off = B.base + vz * B.sz + vy * B.sy;
+		if (TOF && !S64)
+			acd = a_cur * dnorm;
+		emit(off);
+		dprev = dn;
+	}
+	else
+	{
+		off = B.base + vz * B.sz + vy * B.sy;
+		if (TOF && !S64)
+			acd = a_cur * dnorm;
+	}
+	const int ys = r.diry * B.sy, zs = r.dirz * B.sz;
+	const unsigned bx = (unsigned)(B.x1 - B.x0), by = (unsigned)(B.y1 - B.y0),
+	               bz = (unsigned)(B.z1 - B.z0);
+	while (a_cur < amaxf)
+	{
+		vx += (dprev & 1) ? r.dirx : 0;
+		vy += (dprev & 2) ? r.diry : 0;
+		vz += (dprev & 4) ? r.dirz : 0;
+		off += ((dprev & 2) ? ys : 0) + ((dprev & 4) ? zs : 0);
+		if (((unsigned)(vx - B.x0) >= bx) | ((unsigned)(vy - B.y0) >= by) |
+		    ((unsigned)(vz - B.z0) >= bz))
+			break;
+		step();
+		// after the first voxel the planes only come after a_cur; a walk
+		// re-entered at a tile boundary can meet one recomputed an ulp early,
+		// which is still a crossing: an empty segment, then the move
+		a_next = fmaxf(a_next, a_cur);
+		emit(off);
+		dprev = dn;
 	}
 }
+
+// Forward-projection accumulator with the voxel loads issued four at a
+// time: segments are queued and their FMAs applied in the walk's order, so
+// the sum rounds exactly as one fmaf per segment does, while each lane has
+// four gathers in flight instead of one (the gathers miss L2 and dominate).
+struct FwdAcc4
+{
+	const float* img;
+	float acc;
+	int n;
+	int i0, i1, i2, i3;
+	float w0, w1, w2, w3;
+	__device__ __forceinline__ explicit FwdAcc4(const float* p)
+	    : img(p), acc(0.f), n(0), i0(0), i1(0), i2(0), i3(0), w0(0.f), w1(0.f), w2(0.f),
+	      w3(0.f)
+	{
+	}
+	__device__ __forceinline__ void push(int idx, float w)
+	{
+		if (n == 0)
+		{
+			i0 = idx;
+			w0 = w;
+		}
+		else if (n == 1)
+		{
+			i1 = idx;
+			w1 = w;
+		}
+		else if (n == 2)
+		{
+			i2 = idx;
+			w2 = w;
+		}
+		else
+		{
+			const float a = img[i0], b = img[i1], c = img[i2], d = img[idx];
+			acc = fmaf(w0, a, acc);
+			acc = fmaf(w1, b, acc);
+			acc = fmaf(w2, c, acc);
+			acc = fmaf(w, d, acc);
+			n = 0;
+			return;
+		}
+		n++;
+	}
+	__device__ __forceinline__ float finish()
+	{
+		if (n > 0)
+			acc = fmaf(w0, img[i0], acc);
+		if (n > 1)
+			acc = fmaf(w1, img[i1], acc);
+		if (n > 2)
+			acc = fmaf(w2, img[i2], acc);
+		n = 0;
+		return acc;
+	}
+};
 
 // Siddon ray over the whole image.  `img` points at the frame's voxels.
 // MODE_FWD returns the line integral; MODE_BACK adds val*w atomically;
@@ -601,17 +661,21 @@ __device__ __forceinline__ float siddonRay(const DImage& P,
 	SidRay r;
 	if (!sidSetup<TOF, S64>(P, p1, p2, tof, tofv, r))
 		return 0.f;
-	float acc = 0.f;
+	if constexpr (MODE == MODE_FWD)
+	{
+		FwdAcc4 q(img);
+		sidWalk<TOF, S64>(P, r, tof, imageBox(P), true, 0, 0, 0,
+		                  [&](int, int, int, int idx, float, float w) { q.push(idx, w); });
+		return q.finish();
+	}
 	sidWalk<TOF, S64>(P, r, tof, imageBox(P), true, 0, 0, 0,
 	                  [&](int, int, int, int idx, float, float w) {
-		                  if (MODE == MODE_FWD)
-			                  acc = fmaf(w, img[idx], acc);
-		                  else if (MODE == MODE_BACK)
+		                  if (MODE == MODE_BACK)
 			                  unsafeAtomicAdd(img + idx, val * w);
 		                  else
 			                  cnt++;
 	                  });
-	return acc;
+	return 0.f;
 }
 
 // Distance-driven ray (ProjectorDD::dd_project_ref, ProjectorDD.cpp:260-578).
@@ -843,7 +907,8 @@ __device__ float ddRay(const DImage& P, float* __restrict__ img, float3 d1,
 			// getWeight at the slab's [a_lo, a_hi] (ProjectorDD.cpp:507-515)
 			const float alo = (pix_xy - d1_i - 0.5f * dxy) / (d2_i - d1_i);
 			const float ahi = (pix_xy - d1_i + 0.5f * dxy) / (d2_i - d1_i);
-			const float xcn = (0.5f * (alo * d_norm + ahi * d_norm) - pc) / pr.tof.sigma;
+			const float xcn =
+			    divBySigma(0.5f * (alo * d_norm + ahi * d_norm) - pc, pr.tof.sigma, pr.tof.rsigma);
 			tw = gaussW(xcn, pr.tof.norm);
 		}
 		for (int yxi = yx0; yxi <= yx1; yxi++)

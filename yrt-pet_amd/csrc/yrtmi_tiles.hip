@@ -20,6 +20,8 @@
 //                 image with coalesced atomics (one per touched voxel).
 #include "yrtmi_tiles.hpp"
 
+#include <algorithm>
+
 namespace yrtmi
 {
 
@@ -31,15 +33,17 @@ __device__ __forceinline__ void ldsAdd(float* p, float v)
 __device__ __forceinline__ int tileOf(const DTiles& T, int f, int vx, int vy,
                                       int vz)
 {
-	return ((f * T.ntz + vz / kTileZ) * T.nty + vy / kTileY) * T.ntx + vx / kTileX;
+	return ((f * T.ntz + vz / T.tz) * T.nty + vy / T.ty) * T.ntx + vx / T.tx;
 }
 
 // Walks ray r of LOR e (global ray index t = e * R + r) and calls
-// onEnter(tile, alpha, packed entry) each time the walk enters a tile.  A
-// plane shift beyond the record's range sets *err (the ray is not binned).
+// onRecord(tile, alpha, packed entry, segments) for each tile the walk
+// passes through, once the walk has left it (so the piece's number of
+// segments is known).  A plane shift beyond the record's range sets *err
+// (the ray is not binned).
 template <bool TOF, bool S64, class Fn>
 __device__ __forceinline__ void binRay(const KArgs& a, const DTiles& T,
-                                       int64_t e, int r, int* err, Fn&& onEnter)
+                                       int64_t e, int r, int* err, Fn&& onRecord)
 {
 	const Lor l = fetchLor(a.sc, a.L, e, a.im.nt);
 	if (l.frame < 0)
@@ -59,52 +63,69 @@ __device__ __forceinline__ void binRay(const KArgs& a, const DTiles& T,
 		atomicOr(err, 2);
 		return;
 	}
-	int last = -1;
+	int last = -1, start = 0, nseg = 0;
+	float pa = 0.f;
+	uint32_t pv = 0;
 	sidWalk<TOF, S64>(a.im, s, a.pr.tof, imageBox(a.im), true, 0, 0, 0,
 	                  [&](int vx, int vy, int vz, int, float a0, float) {
 		                  const int t = tileOf(T, l.frame, vx, vy, vz);
 		                  if (t != last)
 		                  {
-			                  onEnter(t, a0,
-			                          packEntry(vx % kTileX, vy % kTileY, vz % kTileZ, dx,
-			                                    dy, dz, last < 0));
+			                  if (last >= 0)
+				                  onRecord(last, pa, pv, nseg - start);
+			                  pa = a0;
+			                  pv = packEntry(vx % T.tx, vy % T.ty, vz % T.tz, dx, dy, dz,
+			                                 last < 0);
+			                  start = nseg;
 			                  last = t;
 		                  }
+		                  nseg++;
 	                  });
+	if (last >= 0)
+		onRecord(last, pa, pv, nseg - start);
 }
 
-// PASS 0: count[t] += entries of tile t.  PASS 1: write the records; each
-// workgroup first counts its own entries, reserves a range per tile from
-// cursor[t], then walks again to fill it.  LDS: cnt[nTiles], base[nTiles].
+__device__ __forceinline__ int lengthClass(const DPlan& pl, int segs)
+{
+	int c = 0;
+	for (int k = 0; k < kMaxClasses - 1; k++)
+		c += (k < pl.C - 1 && segs >= pl.cut[k]) ? 1 : 0;
+	return c;
+}
+
+// PASS 0: count[b] += records of bin b.  PASS 1: write the records; each
+// workgroup first counts its own records, reserves a range per bin from
+// cursor[b], then walks again to fill it.  LDS: cnt[nB], base[nB].
 template <bool TOF, bool S64, int PASS>
 __global__ void __launch_bounds__(256) kBin(KArgs a, DTiles T, DPlan pl)
 {
 	extern __shared__ int sm[];
+	const int nB = T.nTiles * pl.C;
 	int* cnt = sm;
-	int* base = sm + T.nTiles;
+	int* base = sm + nB;
 	const int64_t total = a.L.n * (int64_t)pl.R;
 	const int64_t stride = (int64_t)gridDim.x * blockDim.x;
 	const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-	for (int i = threadIdx.x; i < T.nTiles; i += blockDim.x)
+	for (int i = threadIdx.x; i < nB; i += blockDim.x)
 		cnt[i] = 0;
 	__syncthreads();
 	for (int64_t t = t0; t < total; t += stride)
 	{
 		const int64_t e = t / pl.R;
 		const int r = (int)(t - e * pl.R);
-		binRay<TOF, S64>(a, T, e, r, pl.err, [&](int tile, float, uint32_t) {
-			atomicAdd(&cnt[tile], 1);
+		binRay<TOF, S64>(a, T, e, r, pl.err, [&](int tile, float, uint32_t, int segs) {
+			atomicAdd(&cnt[tile * pl.C + lengthClass(pl, segs)], 1);
 		});
 	}
 	__syncthreads();
 	if (PASS == 0)
 	{
-		for (int i = threadIdx.x; i < T.nTiles; i += blockDim.x)
+		for (int i = threadIdx.x; i < nB; i += blockDim.x)
 			if (cnt[i] != 0)
 				atomicAdd(&pl.count[i], (unsigned long long)cnt[i]);
 		return;
 	}
-	for (int i = threadIdx.x; i < T.nTiles; i += blockDim.x)
+	for (int i = threadIdx.x; i < nB; i += blockDim.x)
 	{
 		const int c = cnt[i];
 		base[i] = (c != 0) ? (int)atomicAdd(&pl.cursor[i], (unsigned int)c) : 0;
@@ -115,11 +136,12 @@ __global__ void __launch_bounds__(256) kBin(KArgs a, DTiles T, DPlan pl)
 	{
 		const int64_t e = t / pl.R;
 		const int r = (int)(t - e * pl.R);
-		binRay<TOF, S64>(a, T, e, r, pl.err, [&](int tile, float a0, uint32_t pv) {
-			const int slot = base[tile] + atomicAdd(&cnt[tile], 1);
-			if ((unsigned long long)slot < pl.count[tile])
+		binRay<TOF, S64>(a, T, e, r, pl.err, [&](int tile, float a0, uint32_t pv, int segs) {
+			const int b = tile * pl.C + lengthClass(pl, segs);
+			const int slot = base[b] + atomicAdd(&cnt[b], 1);
+			if ((unsigned long long)slot < pl.count[b])
 			{
-				const int64_t j = pl.tileOff[tile] + slot;
+				const int64_t j = pl.tileOff[b] + slot;
 				pl.key[j] = (uint32_t)t;
 				pl.alpha[j] = a0;
 				pl.vox[j] = pv;
@@ -130,19 +152,19 @@ __global__ void __launch_bounds__(256) kBin(KArgs a, DTiles T, DPlan pl)
 	}
 }
 
-// One workgroup per chunk: accumulate the chunk's ray pieces in the LDS
-// tile, then add the tile into y (frame f of the chunk's tile).  v[e] is
-// the value backprojected along LOR e (split over the rays as
-// ProjectorSiddon::backProjection does, ProjectorSiddon.cpp:321-328);
-// LORs with v == 0 are skipped.
-template <bool TOF, bool S64>
-__global__ void __launch_bounds__(kTileThreads) kTileBack(KArgs a, DTiles T,
-                                                          DPlanRead pl,
-                                                          const float* __restrict__ v,
-                                                          float* __restrict__ y)
+// One chunk: accumulate its ray pieces in the LDS tile `acc`, then add the
+// tile into y (frame f of the chunk's tile).  v[e] is the value
+// backprojected along LOR e (split over the rays as
+// ProjectorSiddon::backProjection does, ProjectorSiddon.cpp:321-328); LORs
+// with v == 0 are skipped.
+template <bool TOF, bool S64, int SHAPE>
+__device__ __forceinline__ void tileChunk(const KArgs& a, const DTiles& T,
+                                          const DPlanRead& pl, const TileChunk ch,
+                                          const float* __restrict__ v,
+                                          float* __restrict__ y, float* acc)
 {
-	__shared__ float acc[kTileVox];
-	const TileChunk ch = pl.chunks[blockIdx.x];
+	constexpr int TX = tileDim(SHAPE, 0), TY = tileDim(SHAPE, 1), TZ = tileDim(SHAPE, 2);
+	constexpr int NV = TX * TY * TZ, NT = tileThreads(SHAPE);
 	int t = ch.tile;
 	const int txi = t % T.ntx;
 	t /= T.ntx;
@@ -151,20 +173,20 @@ __global__ void __launch_bounds__(kTileThreads) kTileBack(KArgs a, DTiles T,
 	const int tzi = t % T.ntz;
 	const int f = t / T.ntz;
 	WalkBox B;
-	B.x0 = txi * kTileX;
-	B.y0 = tyi * kTileY;
-	B.z0 = tzi * kTileZ;
-	B.x1 = min(B.x0 + kTileX, a.im.nx);
-	B.y1 = min(B.y0 + kTileY, a.im.ny);
-	B.z1 = min(B.z0 + kTileZ, a.im.nz);
-	B.sy = kTileX;
-	B.sz = kTileX * kTileY;
+	B.x0 = txi * TX;
+	B.y0 = tyi * TY;
+	B.z0 = tzi * TZ;
+	B.x1 = min(B.x0 + TX, a.im.nx);
+	B.y1 = min(B.y0 + TY, a.im.ny);
+	B.z1 = min(B.z0 + TZ, a.im.nz);
+	B.sy = TX;
+	B.sz = TX * TY;
 	B.base = -(B.z0 * B.sz + B.y0 * B.sy + B.x0);
-	for (int i = threadIdx.x; i < kTileVox; i += kTileThreads)
+	for (int i = threadIdx.x; i < NV; i += NT)
 		acc[i] = 0.f;
 	__syncthreads();
 	const int R = pl.R;
-	for (int i = threadIdx.x; i < ch.count; i += kTileThreads)
+	for (int i = threadIdx.x; i < ch.count; i += NT)
 	{
 		const int64_t j = ch.begin + i;
 		const uint32_t key = pl.key[j];
@@ -179,9 +201,9 @@ __global__ void __launch_bounds__(kTileThreads) kTileBack(KArgs a, DTiles T,
 			val = val / (float)R;
 		const uint32_t pv = pl.vox[j];
 		const int vx = B.x0 + (int)(pv & 31u), vy = B.y0 + (int)((pv >> 5) & 31u),
-		          vz = B.z0 + (int)((pv >> 10) & 15u);
-		const int dx = (int)((pv >> 14) & 15u) - 8, dy = (int)((pv >> 18) & 15u) - 8,
-		          dz = (int)((pv >> 22) & 15u) - 8;
+		          vz = B.z0 + (int)((pv >> 10) & 31u);
+		const int dx = (int)((pv >> 15) & 15u) - 8, dy = (int)((pv >> 19) & 15u) - 8,
+		          dz = (int)((pv >> 23) & 15u) - 8;
 		const bool first = (pv >> 31) != 0;
 		if (!first && (vx >= B.x1 || vy >= B.y1 || vz >= B.z1))
 			continue;
@@ -204,13 +226,51 @@ __global__ void __launch_bounds__(kTileThreads) kTileBack(KArgs a, DTiles T,
 	}
 	__syncthreads();
 	float* yf = y + (int64_t)f * a.im.nvox;
-	for (int i = threadIdx.x; i < kTileVox; i += kTileThreads)
+	for (int i = threadIdx.x; i < NV; i += NT)
 	{
-		const int lx = i % kTileX, ly = (i / kTileX) % kTileY, lz = i / (kTileX * kTileY);
+		const int lx = i % TX, ly = (i / TX) % TY, lz = i / (TX * TY);
 		const int gx = B.x0 + lx, gy = B.y0 + ly, gz = B.z0 + lz;
 		const float s = acc[i];
 		if (s != 0.f && gx < B.x1 && gy < B.y1 && gz < B.z1)
 			unsafeAtomicAdd(yf + (int64_t)gz * a.im.nxy + (int64_t)gy * a.im.nx + gx, s);
+	}
+	__syncthreads();  // acc is reused by the next chunk
+}
+
+// One workgroup per chunk (pl.queue == null), or a persistent grid taking
+// chunks from the plan's queue until it is empty.  Every workgroup leaves
+// once the queue index passes the chunk count, so the grid always drains.
+template <bool TOF, bool S64, int SHAPE>
+__global__ void __launch_bounds__(tileThreads(SHAPE))
+    kTileBack(KArgs a, DTiles T, DPlanRead pl, const float* __restrict__ v,
+              float* __restrict__ y)
+{
+	__shared__ float acc[tileDim(SHAPE, 0) * tileDim(SHAPE, 1) * tileDim(SHAPE, 2)];
+	if (pl.queue == nullptr)
+	{
+		tileChunk<TOF, S64, SHAPE>(a, T, pl, pl.chunks[blockIdx.x], v, y, acc);
+		return;
+	}
+	__shared__ int64_t next;
+	for (;;)
+	{
+		if (threadIdx.x == 0)
+			next = (int64_t)atomicAdd(&pl.queue[0], 1u);
+		__syncthreads();
+		const int64_t c = next;
+		__syncthreads();
+		if (c >= pl.nChunks)
+			break;
+		tileChunk<TOF, S64, SHAPE>(a, T, pl, pl.chunks[c], v, y, acc);
+	}
+	if (threadIdx.x == 0)
+	{
+		__threadfence();
+		if (atomicAdd(&pl.queue[1], 1u) == gridDim.x - 1)
+		{
+			atomicExch(&pl.queue[0], 0u);
+			atomicExch(&pl.queue[1], 0u);
+		}
 	}
 }
 
@@ -218,12 +278,23 @@ __global__ void __launch_bounds__(kTileThreads) kTileBack(KArgs a, DTiles T,
 // forward projection of x, corrected, against the measurement; v[e] = 0
 // where the LOR is skipped (zero measurement / factor, invalid LOR, or a
 // denominator at or below the threshold).
+// The hardware deals workgroups to the XCDs round-robin (workgroup b runs on
+// XCD b % kXcds).  With the grid a multiple of kXcds, workgroup b takes the
+// range (b % kXcds) * (grid / kXcds) + b / kXcds instead, so each XCD works
+// through one contiguous stretch of the LORs: in locality order their voxels
+// then share that XCD's L2.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int64_t xcdRange(int64_t b, int64_t grid)
+{
+	return (b % kXcds) * (grid / kXcds) + b / kXcds;
+}
+
 template <bool TOF, int LPE, bool S64>
 __global__ void __launch_bounds__(256) kEmRatio(KArgs a, float* __restrict__ v)
 {
 	const int64_t total = a.L.n * LPE;
 	const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-	for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+	for (int64_t t = xcdRange(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; t < total;
 	     t += stride)
 	{
 		const int64_t e = t / LPE;
@@ -309,7 +380,7 @@ template <bool TOF, bool S64>
 static hipError_t binT(int pass, const KArgs& a, const DTiles& T,
                        const DPlan& pl, int grid, hipStream_t st)
 {
-	const size_t lds = (size_t)T.nTiles * 2 * sizeof(int);
+	const size_t lds = (size_t)T.nTiles * pl.C * 2 * sizeof(int);
 	if (pass == 0)
 		kBin<TOF, S64, 0><<<grid, 256, lds, st>>>(a, T, pl);
 	else
@@ -328,27 +399,59 @@ hipError_t launchBin(int pass, const KArgs& a, const DTiles& T, const DPlan& pl,
 	           : binT<false, false>(pass, a, T, pl, grid, st);
 }
 
-hipError_t launchTileBack(const KArgs& a, const DTiles& T, const DPlanRead& pl,
-                          int64_t nChunks, const float* v, float* y,
-                          hipStream_t st)
+template <bool TOF, bool S64, int SHAPE>
+static void tileBackT(const KArgs& a, const DTiles& T, const DPlanRead& pl,
+                      const float* v, float* y, unsigned grid, hipStream_t st)
 {
-	if (nChunks <= 0)
+	kTileBack<TOF, S64, SHAPE><<<grid, tileThreads(SHAPE), 0, st>>>(a, T, pl, v, y);
+}
+
+template <bool TOF, bool S64>
+static void tileBackShape(const KArgs& a, const DTiles& T, const DPlanRead& pl,
+                          const float* v, float* y, unsigned grid, hipStream_t st)
+{
+	switch (T.shape)
+	{
+	case 1:
+		tileBackT<TOF, S64, 1>(a, T, pl, v, y, grid, st);
+		break;
+	case 2:
+		tileBackT<TOF, S64, 2>(a, T, pl, v, y, grid, st);
+		break;
+	default:
+		tileBackT<TOF, S64, 0>(a, T, pl, v, y, grid, st);
+		break;
+	}
+}
+
+hipError_t launchTileBack(const KArgs& a, const DTiles& T, const DPlanRead& pl,
+                          const float* v, float* y, int persistent, hipStream_t st)
+{
+	if (pl.nChunks <= 0)
 		return hipSuccess;
-	const dim3 g((unsigned)nChunks);
+	DPlanRead q = pl;
+	unsigned grid;
+	if (persistent > 0 && pl.queue != nullptr)
+		grid = (unsigned)std::min<int64_t>(persistent, pl.nChunks);
+	else
+	{
+		q.queue = nullptr;
+		grid = (unsigned)pl.nChunks;
+	}
 	const bool tof = a.pr.useTof != 0;
 	if (a.pr.setup64)
 	{
 		if (tof)
-			kTileBack<true, true><<<g, kTileThreads, 0, st>>>(a, T, pl, v, y);
+			tileBackShape<true, true>(a, T, q, v, y, grid, st);
 		else
-			kTileBack<false, true><<<g, kTileThreads, 0, st>>>(a, T, pl, v, y);
+			tileBackShape<false, true>(a, T, q, v, y, grid, st);
 	}
 	else
 	{
 		if (tof)
-			kTileBack<true, false><<<g, kTileThreads, 0, st>>>(a, T, pl, v, y);
+			tileBackShape<true, false>(a, T, q, v, y, grid, st);
 		else
-			kTileBack<false, false><<<g, kTileThreads, 0, st>>>(a, T, pl, v, y);
+			tileBackShape<false, false>(a, T, q, v, y, grid, st);
 	}
 	return hipGetLastError();
 }
@@ -380,10 +483,11 @@ hipError_t launchEmRatio(const KArgs& a, int lpe, float* v, hipStream_t st)
 {
 	const int64_t threads = a.L.n * lpe;
 	int64_t nb = (threads + 255) / 256;
-	if (nb < 1)
-		nb = 1;
-	if (nb > 0x7fffffff)
-		nb = 0x7fffffff;
+	if (nb > 0x7ffffff0)
+		nb = 0x7ffffff0;
+	nb = (nb + kXcds - 1) / kXcds * kXcds;  // xcdRange is a bijection on [0, nb)
+	if (nb < kXcds)
+		nb = kXcds;
 	const dim3 g((unsigned)nb);
 	const bool tof = a.pr.useTof != 0;
 	if (a.pr.setup64)

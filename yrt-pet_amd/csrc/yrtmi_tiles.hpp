@@ -7,31 +7,42 @@
 namespace yrtmi
 {
 
-// Image tile held in LDS by one workgroup: 32 x 32 x 16 FP32 voxels, 64 KB.
-constexpr int kTileX = 32, kTileY = 32, kTileZ = 16;
-constexpr int kTileVox = kTileX * kTileY * kTileZ;
-constexpr int kTileThreads = 512;
-// Binning keeps two int counters per tile in LDS (64 KB at most).
+// Image tiles held in LDS by one workgroup (FP32 voxels), by shape:
+//   0: 32 x 32 x 16 (64 KB, 512 threads)   1: 16 x 16 x 16 (16 KB, 256)
+//   2: 32 x 32 x 32 (128 KB, 1024 threads)
+constexpr int kNumTileShapes = 3;
+__host__ __device__ constexpr int tileDim(int shape, int axis)
+{
+	return shape == 1 ? 16 : (shape == 2 ? 32 : (axis == 2 ? 16 : 32));
+}
+__host__ __device__ constexpr int tileThreads(int shape)
+{
+	return shape == 1 ? 256 : (shape == 2 ? 1024 : 512);
+}
+// Binning keeps two int counters per (tile, length class) in LDS (64 KB at
+// most): tiles x classes <= kMaxBins.
 constexpr int kMaxTiles = 8192;
+constexpr int kMaxBins = 8192;
+constexpr int kMaxClasses = 8;
 
-// Entry record of a ray in a tile: voxel within the tile (5 + 5 + 4 bits),
+// Entry record of a ray in a tile: voxel within the tile (5 bits per axis),
 // the ray's plane shift per axis (sidPlaneShift, 4 bits each, offset 8) and
 // whether the tile is where the ray's walk starts (bit 31).
-static_assert(kTileX <= 32 && kTileY <= 32 && kTileZ <= 16, "entry packing");
 constexpr int kMaxShift = 7;
 __host__ __device__ inline uint32_t packEntry(int lx, int ly, int lz, int dx, int dy,
                                               int dz, bool first)
 {
 	return (uint32_t)lx | ((uint32_t)ly << 5) | ((uint32_t)lz << 10) |
-	       ((uint32_t)(dx + 8) << 14) | ((uint32_t)(dy + 8) << 18) |
-	       ((uint32_t)(dz + 8) << 22) | (first ? 0x80000000u : 0u);
+	       ((uint32_t)(dx + 8) << 15) | ((uint32_t)(dy + 8) << 19) |
+	       ((uint32_t)(dz + 8) << 23) | (first ? 0x80000000u : 0u);
 }
 
 // Tiles of an image with nt frames: ntx * nty * ntz per frame, numbered
-// ((f * ntz + tz) * nty + ty) * ntx + tx.
+// ((f * ntz + tz) * nty + ty) * ntx + tx; tiles of tx * ty * tz voxels.
 struct DTiles
 {
 	int32_t ntx, nty, ntz, nTiles;
+	int32_t tx, ty, tz, shape;
 };
 
 // A chunk of one tile's records, processed by one workgroup.
@@ -42,36 +53,45 @@ struct TileChunk
 	int32_t count;
 };
 
-// Plan being built.
+// Plan being built.  Records are binned by (tile, length class): a record's
+// class counts the cuts at or below its number of segments in the tile, and
+// within a tile the records are stored class by class, so that the lanes of
+// a wave walk pieces of similar length.  Bin b = tile * C + class.
 struct DPlan
 {
 	int32_t R;                  // rays per LOR
-	unsigned long long* count;  // [nTiles] records per tile (count pass)
-	unsigned int* cursor;       // [nTiles] fill cursor within the tile
-	const int64_t* tileOff;     // [nTiles] first record of the tile
+	int32_t C;                  // length classes (1 .. kMaxClasses)
+	int32_t cut[kMaxClasses - 1];  // class k starts at cut[k-1] segments
+	unsigned long long* count;  // [nTiles * C] records per bin (count pass)
+	unsigned int* cursor;       // [nTiles * C] fill cursor within the bin
+	const int64_t* tileOff;     // [nTiles * C] first record of the bin
 	uint32_t* key;              // lor * R + ray
 	float* alpha;               // alpha where the walk enters the tile
 	uint32_t* vox;              // packed entry voxel (packEntry)
 	int* err;                   // set when a tile range would overflow
 };
 
-// Plan in use.
+// Plan in use.  queue != null: persistent launch, workgroups take chunks
+// from queue[0] (queue[1] counts finished workgroups; the last one resets
+// both, so the next launch starts from zero without a memset).
 struct DPlanRead
 {
 	int32_t R;
 	const TileChunk* chunks;
+	int64_t nChunks;
 	const uint32_t* key;
 	const float* alpha;
 	const uint32_t* vox;
+	unsigned int* queue;
 };
 
-// pass 0: count; pass 1: fill.  Dynamic LDS 8 * nTiles bytes.
+// pass 0: count; pass 1: fill.  Dynamic LDS 8 * nTiles * C bytes.
 hipError_t launchBin(int pass, const KArgs& a, const DTiles& T, const DPlan& pl,
                      int grid, hipStream_t st);
-// y += backprojection of v (per LOR) through the plan's records.
+// y += backprojection of v (per LOR) through the plan's records: one
+// workgroup per chunk, or `persistent` workgroups on the chunk queue.
 hipError_t launchTileBack(const KArgs& a, const DTiles& T, const DPlanRead& pl,
-                          int64_t nChunks, const float* v, float* y,
-                          hipStream_t st);
+                          const float* v, float* y, int persistent, hipStream_t st);
 // v[e] = EM ratio of LOR e (0 where skipped), Siddon.
 hipError_t launchEmRatio(const KArgs& a, int lpe, float* v, hipStream_t st);
 // v[e] = sensitivity value of LOR e (factor x attenuation).

@@ -4,7 +4,7 @@ generation and the OSEM subset loop) behind the reference's operator / OSEM
 names.  Hand-written HIP kernels for gfx950 behind a C ABI (include/yrtmi.h).
 """
 from . import capi
-from .core import Context, ImageParams, LorSet, Plan, Projector, ProjectorParams, Scanner
+from .core import Context, Graph, ImageParams, LorSet, Plan, Projector, ProjectorParams, Scanner
 
-__all__ = ["capi", "Context", "ImageParams", "LorSet", "Plan", "Projector",
+__all__ = ["capi", "Context", "Graph", "ImageParams", "LorSet", "Plan", "Projector",
            "ProjectorParams", "Scanner"]

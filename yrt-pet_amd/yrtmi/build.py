@@ -29,7 +29,8 @@ HIP_FLAGS = [
     "-Wno-unused-command-line-argument",
 ]
 
-LIB_SOURCES = ["yrtmi_kernels.hip", "yrtmi_tiles.hip", "yrtmi_f64.hip", "yrtmi_capi.hip"]
+LIB_SOURCES = ["yrtmi_kernels.hip", "yrtmi_tiles.hip", "yrtmi_f64.hip", "yrtmi_order.hip",
+               "yrtmi_capi.hip"]
 
 
 def hipcc():

@@ -74,6 +74,18 @@ class ScannerDesc(C.Structure):
 _lib = None
 
 _P = C.c_void_p
+class PlanOptions(C.Structure):
+    _fields_ = [("chunk_records", C.c_int), ("tile_shape", C.c_int), ("persistent", C.c_int)]
+
+
+TILE_32x32x16, TILE_16x16x16, TILE_32x32x32 = 0, 1, 2
+
+
+class Tuning(C.Structure):
+    _fields_ = [("lanes_per_event", C.c_int), ("tile_shape", C.c_int), ("persistent", C.c_int),
+                ("ratio_ms", C.c_float), ("back_ms", C.c_float),
+                ("ratio_ms_lpe", C.c_float * 4), ("back_ms_shape", (C.c_float * 2) * 3)]
+
 _SIGS = {
     "yrtmi_last_error": (C.c_char_p, []),
     "yrtmi_ctx_create": (C.c_int, [C.c_int, _P, C.POINTER(_P)]),
@@ -118,8 +130,18 @@ _SIGS = {
     "yrtmi_gather": (C.c_int, [_P, _P, C.c_int64, _P, C.c_int64, C.c_int, _P]),
     "yrtmi_histo_subset_gather": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, _P]),
     "yrtmi_axpby": (C.c_int, [_P, C.c_int64, C.c_float, _P, C.c_float, _P]),
-    "yrtmi_plan_create": (C.c_int, [_P, _P, C.POINTER(Image), C.POINTER(Lors), C.c_int,
-                                    C.POINTER(_P)]),
+    "yrtmi_lor_order": (C.c_int, [_P, C.POINTER(Image), C.POINTER(Lors), _P]),
+    "yrtmi_permute": (C.c_int, [_P, _P, C.c_int, _P, C.c_int64, _P]),
+    "yrtmi_plan_create": (C.c_int, [_P, _P, C.POINTER(Image), C.POINTER(Lors),
+                                    C.POINTER(PlanOptions), C.POINTER(_P)]),
+    "yrtmi_plan_set_persistent": (C.c_int, [_P, C.c_int]),
+    "yrtmi_ctx_own_stream": (C.c_int, [_P, C.POINTER(_P)]),
+    "yrtmi_autotune": (C.c_int, [_P, _P, C.POINTER(Image), C.POINTER(Lors), _P,
+                                 C.POINTER(Tuning)]),
+    "yrtmi_graph_begin": (C.c_int, [_P]),
+    "yrtmi_graph_end": (C.c_int, [_P, C.POINTER(_P)]),
+    "yrtmi_graph_launch": (C.c_int, [_P, _P]),
+    "yrtmi_graph_destroy": (C.c_int, [_P]),
     "yrtmi_plan_destroy": (C.c_int, [_P]),
     "yrtmi_plan_info": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                   C.POINTER(C.c_int64)]),

@@ -242,6 +242,18 @@ class LorSet:
                       self.num_subsets, self.subset, g(self.tof), g(self.frame),
                       g(self.motion_frame))
 
+    def permuted(self, ctx, perm):
+        """The same LORs in the order perm (pairs / bins sources): element i
+        of every per-LOR array is element perm[i] of this set's."""
+        if self.kind == capi.SRC_HISTO_SUBSET:
+            raise ValueError("cannot permute a subset iterator")
+        if perm.numel() != self.n:
+            raise ValueError("permutation size differs from the LOR count")
+        g = (lambda t: None if t is None else ctx.permute(t[:self.n], perm))
+        return LorSet(self.kind, self.n, g(self.d1), g(self.d2), g(self.bins),
+                      self.num_subsets, self.subset, g(self.tof), g(self.frame),
+                      g(self.motion_frame))
+
     def c(self):
         return capi.Lors(self.kind, self.n, _ptr(self.d1), _ptr(self.d2),
                          _ptr(self.bins), self.num_subsets, self.subset,
@@ -284,13 +296,19 @@ class Plan:
     memory until dropped.  Keeps the LOR set alive (the plan refers to its
     arrays)."""
 
-    def __init__(self, ctx, proj, im, lors, chunk_records=0):
+    def __init__(self, ctx, proj, im, lors, chunk_records=0, tile_shape=0, persistent=0):
         self.ctx, self.proj, self.im, self.lors = ctx, proj, im, lors
         h = _P()
         ci, cl = im.c(), lors.c()
+        opt = capi.PlanOptions(int(chunk_records), int(tile_shape), int(persistent))
         capi.check(capi.lib().yrtmi_plan_create(ctx.h, proj.h, C.byref(ci), C.byref(cl),
-                                                int(chunk_records), C.byref(h)))
+                                                C.byref(opt), C.byref(h)))
         self.h = h
+        self.tile_shape = tile_shape
+
+    def set_persistent(self, workgroups):
+        """0: one workgroup per chunk; n > 0: n workgroups on the chunk queue."""
+        capi.check(capi.lib().yrtmi_plan_set_persistent(self.h, int(workgroups)))
 
     def info(self):
         r, c, b = C.c_int64(), C.c_int64(), C.c_int64()
@@ -301,6 +319,38 @@ class Plan:
         try:
             if getattr(self, "h", None):
                 capi.lib().yrtmi_plan_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+class Graph:
+    """The calls made on a context inside ``with ctx.graph() as g:`` are
+    recorded (not run) and replayed by ``g.launch()`` (yrtmi_graph_*)."""
+
+    def __init__(self, ctx):
+        self.ctx, self.h = ctx, None
+
+    def __enter__(self):
+        capi.check(capi.lib().yrtmi_graph_begin(self.ctx.h))
+        return self
+
+    def __exit__(self, et, ev, tb):
+        h = _P()
+        rc = capi.lib().yrtmi_graph_end(self.ctx.h, C.byref(h))
+        if rc == 0:
+            self.h = h
+        if et is None:
+            capi.check(rc)
+        return False
+
+    def launch(self):
+        capi.check(capi.lib().yrtmi_graph_launch(self.ctx.h, self.h))
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                capi.lib().yrtmi_graph_destroy(self.h)
                 self.h = None
         except Exception:
             pass
@@ -357,8 +407,28 @@ class Context:
                                             C.byref(cl), _ptr(out)))
         return out
 
-    def plan(self, proj: Projector, im: ImageParams, lors: LorSet, chunk_records=0):
-        return Plan(self, proj, im, lors, chunk_records)
+    def graph(self):
+        """Record the calls of a ``with`` block as one replayable graph; the
+        context's stream must not be the null stream (torch's default
+        stream is: use ``set_stream(torch.cuda.Stream())``)."""
+        return Graph(self)
+
+    def autotune(self, proj: Projector, im: ImageParams, lors: LorSet, x):
+        """Time launch shapes on ``lors`` / image ``x`` (yrtmi_autotune): sets
+        the projector's lanes per event, returns the fastest settings."""
+        _check_dtype(x, torch.float32, "x")
+        t = capi.Tuning()
+        ci, cl = im.c(), lors.c()
+        capi.check(capi.lib().yrtmi_autotune(self.h, proj.h, C.byref(ci), C.byref(cl), _ptr(x),
+                                             C.byref(t)))
+        return dict(lanes_per_event=t.lanes_per_event, tile_shape=t.tile_shape,
+                    persistent=t.persistent, ratio_ms=t.ratio_ms, back_ms=t.back_ms,
+                    ratio_ms_lpe=list(t.ratio_ms_lpe),
+                    back_ms_shape=[list(r) for r in t.back_ms_shape])
+
+    def plan(self, proj: Projector, im: ImageParams, lors: LorSet, chunk_records=0,
+             tile_shape=0, persistent=0):
+        return Plan(self, proj, im, lors, chunk_records, tile_shape, persistent)
 
     def back(self, proj: Projector, im: ImageParams, img, lors: LorSet, vals, plan=None):
         _check_dtype(img, torch.float32, "img")
@@ -461,6 +531,25 @@ class Context:
     def fill(self, x, v):
         capi.check(capi.lib().yrtmi_fill(self.h, _ptr(x), x.numel(), float(v)))
         return x
+
+    def lor_order(self, im, lors):
+        """Locality order of `lors` (int32 tensor of the indices 0..n-1):
+        sorted by frame, axial midpoint, tilt, transaxial angle and offset,
+        so that LORs projected together read the same voxels."""
+        perm = torch.empty(lors.n, dtype=torch.int32, device=torch.device("cuda", self.device))
+        ci, cl = im.c(), lors.c()
+        capi.check(capi.lib().yrtmi_lor_order(self.h, C.byref(ci), C.byref(cl), _ptr(perm)))
+        return perm
+
+    def permute(self, src, perm):
+        """src[perm] (a new contiguous tensor) on the context's stream."""
+        if src.dim() != 1 or not src.is_contiguous() or src.numel() < perm.numel():
+            raise ValueError("permute: src must be a contiguous vector of at least n elements")
+        _check_dtype(perm, torch.int32, "perm")
+        out = torch.empty(perm.numel(), dtype=src.dtype, device=src.device)
+        capi.check(capi.lib().yrtmi_permute(self.h, _ptr(src), src.element_size(), _ptr(perm),
+                                            perm.numel(), _ptr(out)))
+        return out
 
     def count_segments(self, proj, im, lors):
         tot = C.c_int64()
