@@ -55,6 +55,25 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path[:0] = [REPO, os.path.join(REPO, "yrt-pet_amd")]
 
 HBM_PEAK_GBS = 8000.0
+# Per-launch HBM traffic of the default list-mode configuration, from two
+# rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) scaled by a copy of known
+# size; the file gives the commands.
+TRAFFIC_PROFILE = os.path.join(REPO, "profiles", "em_listmode_tof_100M_pmc_traffic_sq.md")
+
+
+def profiled_traffic(kernel_label):
+    """HBM bytes per launch of `kernel_label` (its part before the first
+    space) from the committed counter summary, or None."""
+    import re
+    try:
+        text = open(TRAFFIC_PROFILE).read()
+    except OSError:
+        return None
+    short = kernel_label.split(" ")[0]
+    for m in re.finditer(r"^- traffic `([^`]+)`: ([0-9.eE+-]+) B$", text, re.M):
+        if m.group(1) == short:
+            return float(m.group(2))
+    return None
 
 
 def parse():
@@ -303,6 +322,13 @@ def bench_listmode(args, torch, dev, stream):
         roof(f"kEvents<MODE_EM,Siddon,TOF,LPE={lpe}> (fused EM)", (0, 2),
              [12.0 * nev[s] + 12.0 * segs[s] for s in range(S)])
     dom = max(kernels, key=lambda k: k["avg_launch_ms"])
+    # measured traffic applies to the profiled configuration only
+    traffic, traffic_src = None, None
+    if (tiled and n == 100_000_000 and S == 10 and args.order == "lor"
+            and not args.no_tune):
+        traffic = profiled_traffic(dom["kernel"])
+        if traffic is not None:
+            traffic_src = os.path.relpath(TRAFFIC_PROFILE, REPO)
     log(args, f"{s_per_iter:.3f} s/iter ({'graph' if graph else 'calls'}); " + "; ".join(
         f"{k['kernel']}: {k['avg_launch_ms']:.2f} ms, {k['achieved']:.0f} GB/s alg" for k in kernels))
 
@@ -331,7 +357,8 @@ def bench_listmode(args, torch, dev, stream):
                    "event_order": args.order,
                    "graph": graph is not None, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": dom["achieved"], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": dom["frac"], "traffic": None,
+                     "unit": "GB/s", "frac": dom["frac"], "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "kernel": dom["kernel"], "avg_launch_ms": dom["avg_launch_ms"],
                      "alg_bytes_per_launch": dom["alg_bytes_per_launch"],
                      "segments_per_event": sum(segs) / n},

@@ -346,7 +346,11 @@ int yrtmi_permute(yrtmi_ctx* ctx, const void* src, int elem_bytes,
  * OSEM_GPU::generateSensitivityImageForCurrentSubset).
  *
  * A plan bins the Siddon walk of every (LOR, ray) of `lors` by the image
- * tiles it crosses, recording where it enters each.
+ * tiles it crosses, recording where it enters each (12 bytes per record,
+ * records of a tile ordered by the length of their piece), and keeps each
+ * ray's end points, TOF value and frame (32 bytes per LOR and ray), so that
+ * the backprojection reads one table entry per record instead of resolving
+ * the LOR.
  * It depends on geometry only (detector pairs or bins, TOF values, frames,
  * motion frames, the projector, the image geometry), not on image or LOR
  * values: a list-mode or histogram subset is planned once per

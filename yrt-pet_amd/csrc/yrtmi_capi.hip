@@ -1013,6 +1013,7 @@ struct yrtmi_plan
 	uint32_t* key = nullptr;
 	float* alpha = nullptr;
 	uint32_t* vox = nullptr;
+	float4* ray = nullptr;  // ray table, 2 x float4 per (LOR, ray)
 	TileChunk* chunks = nullptr;
 	float* v = nullptr;  // per-LOR values of the em / sensitivity calls
 	unsigned int* queue = nullptr;  // chunk queue of persistent launches
@@ -1027,6 +1028,7 @@ void freePlan(yrtmi_plan* q)
 	hipFree(q->key);
 	hipFree(q->alpha);
 	hipFree(q->vox);
+	hipFree(q->ray);
 	hipFree(q->chunks);
 	hipFree(q->v);
 	hipFree(q->queue);
@@ -1073,6 +1075,7 @@ DPlanRead planRead(const yrtmi_plan* q)
 	r.key = q->key;
 	r.alpha = q->alpha;
 	r.vox = q->vox;
+	r.ray = q->ray;
 	r.queue = q->queue;
 	return r;
 }
@@ -1246,9 +1249,11 @@ int yrtmi_plan_create(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
 		if (hipMalloc(&q->key, sizeof(uint32_t) * total) != hipSuccess ||
 		    hipMalloc(&q->alpha, sizeof(float) * total) != hipSuccess ||
 		    hipMalloc(&q->vox, sizeof(uint32_t) * total) != hipSuccess ||
+		    hipMalloc(&q->ray, 2 * sizeof(float4) * rays) != hipSuccess ||
 		    hipMalloc(&q->chunks, sizeof(TileChunk) * chunks.size()) != hipSuccess)
 			return cleanup(fail(YRTMI_ENOMEM, "plan: records"));
-		q->bytes += total * 12 + (int64_t)(sizeof(TileChunk) * chunks.size());
+		q->bytes += total * 12 + rays * 2 * (int64_t)sizeof(float4) +
+		            (int64_t)(sizeof(TileChunk) * chunks.size());
 		e = hipMemcpyAsync(dOff, off.data(), sizeof(int64_t) * nB,
 		                   hipMemcpyHostToDevice, c->stream);
 		if (e == hipSuccess)
@@ -1257,6 +1262,7 @@ int yrtmi_plan_create(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
 		pl.key = q->key;
 		pl.alpha = q->alpha;
 		pl.vox = q->vox;
+		pl.ray = q->ray;
 		if (e == hipSuccess)
 			e = launchBin(1, a, T, pl, grid, c->stream);
 		std::vector<unsigned int> cur(nB);
@@ -1385,6 +1391,7 @@ int yrtmi_em_subset_planned(yrtmi_ctx* c, const yrtmi_proj* p,
 		return fail(YRTMI_EINVAL, "em_subset: null image");
 	a.x = x;
 	setCorr(a, corr);
+	a.rays = q->ray;  // null when no ray meets the image (no records)
 	HIP_TRY(launchEmRatio(a, p->lpe, q->v, c->stream));
 	HIP_TRY(launchTileBack(a, q->T, planRead(q), q->v, u, q->persistent, c->stream));
 	return YRTMI_OK;

@@ -148,6 +148,10 @@ struct KArgs
 	float* part;
 	int64_t partStride;
 	int32_t nxcc;
+	// Ray table of a tile plan for these LORs (two float4 per LOR and ray:
+	// end points after offsets, TOF value, frame), or null: the forward of
+	// the EM ratio then reads it instead of resolving each LOR.
+	const float4* rays;
 };
 
 // Hardware id of the XCD (accelerator complex die) running this wave.

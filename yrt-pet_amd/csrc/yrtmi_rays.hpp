@@ -590,64 +590,6 @@ __device__ __forceinline__ void sidWalk(const DImage& P, const SidRay& r,
 	}
 }
 
-// Forward-projection accumulator with the voxel loads issued four at a
-// time: segments are queued and their FMAs applied in the walk's order, so
-// the sum rounds exactly as one fmaf per segment does, while each lane has
-// four gathers in flight instead of one (the gathers miss L2 and dominate).
-struct FwdAcc4
-{
-	const float* img;
-	float acc;
-	int n;
-	int i0, i1, i2, i3;
-	float w0, w1, w2, w3;
-	__device__ __forceinline__ explicit FwdAcc4(const float* p)
-	    : img(p), acc(0.f), n(0), i0(0), i1(0), i2(0), i3(0), w0(0.f), w1(0.f), w2(0.f),
-	      w3(0.f)
-	{
-	}
-	__device__ __forceinline__ void push(int idx, float w)
-	{
-		if (n == 0)
-		{
-			i0 = idx;
-			w0 = w;
-		}
-		else if (n == 1)
-		{
-			i1 = idx;
-			w1 = w;
-		}
-		else if (n == 2)
-		{
-			i2 = idx;
-			w2 = w;
-		}
-		else
-		{
-			const float a = img[i0], b = img[i1], c = img[i2], d = img[idx];
-			acc = fmaf(w0, a, acc);
-			acc = fmaf(w1, b, acc);
-			acc = fmaf(w2, c, acc);
-			acc = fmaf(w, d, acc);
-			n = 0;
-			return;
-		}
-		n++;
-	}
-	__device__ __forceinline__ float finish()
-	{
-		if (n > 0)
-			acc = fmaf(w0, img[i0], acc);
-		if (n > 1)
-			acc = fmaf(w1, img[i1], acc);
-		if (n > 2)
-			acc = fmaf(w2, img[i2], acc);
-		n = 0;
-		return acc;
-	}
-};
-
 // Siddon ray over the whole image.  `img` points at the frame's voxels.
 // MODE_FWD returns the line integral; MODE_BACK adds val*w atomically;
 // MODE_COUNT counts applied segments.
@@ -663,10 +605,14 @@ __device__ __forceinline__ float siddonRay(const DImage& P,
 		return 0.f;
 	if constexpr (MODE == MODE_FWD)
 	{
-		FwdAcc4 q(img);
+		// one FMA per segment in the walk's order (in locality order the
+		// gathers hit L2; keeping several in flight per lane measured slower)
+		float acc = 0.f;
 		sidWalk<TOF, S64>(P, r, tof, imageBox(P), true, 0, 0, 0,
-		                  [&](int, int, int, int idx, float, float w) { q.push(idx, w); });
-		return q.finish();
+		                  [&](int, int, int, int idx, float, float w) {
+			                  acc = fmaf(w, img[idx], acc);
+		                  });
+		return acc;
 	}
 	sidWalk<TOF, S64>(P, r, tof, imageBox(P), true, 0, 0, 0,
 	                  [&](int, int, int, int idx, float, float w) {

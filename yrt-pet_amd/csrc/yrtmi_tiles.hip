@@ -41,17 +41,28 @@ __device__ __forceinline__ int tileOf(const DTiles& T, int f, int vx, int vy,
 // passes through, once the walk has left it (so the piece's number of
 // segments is known).  A plane shift beyond the record's range sets *err
 // (the ray is not binned).
+// With tab, the ray's entry of the ray table is written (rayEntry).
 template <bool TOF, bool S64, class Fn>
 __device__ __forceinline__ void binRay(const KArgs& a, const DTiles& T,
-                                       int64_t e, int r, int* err, Fn&& onRecord)
+                                       int64_t e, int r, int* err, float4* tab,
+                                       Fn&& onRecord)
 {
 	const Lor l = fetchLor(a.sc, a.L, e, a.im.nt);
 	if (l.frame < 0)
+	{
+		if (tab != nullptr)
+		{
+			const float3 z = make_float3(0.f, 0.f, 0.f);
+			rayEntry(tab, (uint32_t)(e * a.pr.numRays + r), z, z, 0.f, -1);
+		}
 		return;
+	}
 	float3 q1, q2;
 	rayLine(a.pr, l, r, q1, q2);
 	q1 = make_float3(q1.x - a.im.ox, q1.y - a.im.oy, q1.z - a.im.oz);
 	q2 = make_float3(q2.x - a.im.ox, q2.y - a.im.oy, q2.z - a.im.oz);
+	if (tab != nullptr)
+		rayEntry(tab, (uint32_t)(e * a.pr.numRays + r), q1, q2, l.tof, l.frame);
 	SidRay s;
 	if (!sidSetup<TOF, S64>(a.im, q1, q2, a.pr.tof, l.tof, s))
 		return;
@@ -113,7 +124,7 @@ __global__ void __launch_bounds__(256) kBin(KArgs a, DTiles T, DPlan pl)
 	{
 		const int64_t e = t / pl.R;
 		const int r = (int)(t - e * pl.R);
-		binRay<TOF, S64>(a, T, e, r, pl.err, [&](int tile, float, uint32_t, int segs) {
+		binRay<TOF, S64>(a, T, e, r, pl.err, nullptr, [&](int tile, float, uint32_t, int segs) {
 			atomicAdd(&cnt[tile * pl.C + lengthClass(pl, segs)], 1);
 		});
 	}
@@ -136,7 +147,7 @@ __global__ void __launch_bounds__(256) kBin(KArgs a, DTiles T, DPlan pl)
 	{
 		const int64_t e = t / pl.R;
 		const int r = (int)(t - e * pl.R);
-		binRay<TOF, S64>(a, T, e, r, pl.err, [&](int tile, float a0, uint32_t pv, int segs) {
+		binRay<TOF, S64>(a, T, e, r, pl.err, pl.ray, [&](int tile, float a0, uint32_t pv, int segs) {
 			const int b = tile * pl.C + lengthClass(pl, segs);
 			const int slot = base[b] + atomicAdd(&cnt[b], 1);
 			if ((unsigned long long)slot < pl.count[b])
@@ -156,7 +167,10 @@ __global__ void __launch_bounds__(256) kBin(KArgs a, DTiles T, DPlan pl)
 // tile into y (frame f of the chunk's tile).  v[e] is the value
 // backprojected along LOR e (split over the rays as
 // ProjectorSiddon::backProjection does, ProjectorSiddon.cpp:321-328); LORs
-// with v == 0 are skipped.
+// with v == 0 are skipped.  Each lane runs a three-stage pipeline over its
+// records: the fields of the record two ahead are read, the ray-table entry
+// and value of the next one are in flight, and the current one is set up
+// from its table entry and walked (the walk itself issues no global loads).
 template <bool TOF, bool S64, int SHAPE>
 __device__ __forceinline__ void tileChunk(const KArgs& a, const DTiles& T,
                                           const DPlanRead& pl, const TileChunk ch,
@@ -185,44 +199,88 @@ __device__ __forceinline__ void tileChunk(const KArgs& a, const DTiles& T,
 	for (int i = threadIdx.x; i < NV; i += NT)
 		acc[i] = 0.f;
 	__syncthreads();
-	const int R = pl.R;
-	for (int i = threadIdx.x; i < ch.count; i += NT)
-	{
-		const int64_t j = ch.begin + i;
-		const uint32_t key = pl.key[j];
-		const int64_t e = key / (uint32_t)R;
-		const int r = (int)(key - (uint32_t)e * (uint32_t)R);
-		if (e >= a.L.n)
-			continue;
-		float val = v[e];
-		if (val == 0.f)
-			continue;
-		if (R > 1)
+	const uint32_t R = (uint32_t)pl.R;
+	const bool rPow2 = (R & (R - 1u)) == 0u;
+	const int rShift = __ffs((int)R) - 1;
+	const uint64_t nRays = (uint64_t)a.L.n * R;
+	// no record (keys are < n * R < 2^32); a key is checked against n * R
+	// where it is used, not where it is read, so that its load stays in
+	// flight during the walk
+	constexpr uint32_t kNone = 0xffffffffu;
+	auto lorOf = [&](uint32_t key) { return rPow2 ? key >> rShift : key / R; };
+	auto readRecord = [&](int k, uint32_t& key, float& al, uint32_t& pv) {
+		key = kNone;
+		al = 0.f;
+		pv = 0u;
+		if (k < ch.count)
+		{
+			const int64_t j = ch.begin + k;
+			key = pl.key[j];
+			al = pl.alpha[j];
+			pv = pl.vox[j];
+		}
+	};
+	auto readRay = [&](uint32_t key, float4& A, float4& Bq, float& val) {
+		A = make_float4(0.f, 0.f, 0.f, 0.f);
+		Bq = A;
+		val = 0.f;
+		if ((uint64_t)key < nRays)
+		{
+			A = pl.ray[2 * (uint64_t)key];
+			Bq = pl.ray[2 * (uint64_t)key + 1];
+			val = v[lorOf(key)];
+		}
+	};
+	auto walk = [&](uint32_t key, float al, uint32_t pv, float4 A, float4 Bq, float val) {
+		if ((uint64_t)key >= nRays || val == 0.f)
+			return;
+		if (R > 1u)
 			val = val / (float)R;
-		const uint32_t pv = pl.vox[j];
 		const int vx = B.x0 + (int)(pv & 31u), vy = B.y0 + (int)((pv >> 5) & 31u),
 		          vz = B.z0 + (int)((pv >> 10) & 31u);
 		const int dx = (int)((pv >> 15) & 15u) - 8, dy = (int)((pv >> 19) & 15u) - 8,
 		          dz = (int)((pv >> 23) & 15u) - 8;
 		const bool first = (pv >> 31) != 0;
 		if (!first && (vx >= B.x1 || vy >= B.y1 || vz >= B.z1))
-			continue;
-		const Lor l = fetchLor(a.sc, a.L, e, a.im.nt);
-		if (l.frame != f)
-			continue;
-		float3 q1, q2;
-		rayLine(a.pr, l, r, q1, q2);
-		q1 = make_float3(q1.x - a.im.ox, q1.y - a.im.oy, q1.z - a.im.oz);
-		q2 = make_float3(q2.x - a.im.ox, q2.y - a.im.oy, q2.z - a.im.oz);
+			return;
+		if (__float_as_int(Bq.w) != f)
+			return;
+		const float3 q1 = make_float3(A.x, A.y, A.z), q2 = make_float3(A.w, Bq.x, Bq.y);
 		SidRay s;
-		if (!sidSetup<TOF, S64>(a.im, q1, q2, a.pr.tof, l.tof, s))
-			continue;
+		if (!sidSetup<TOF, S64>(a.im, q1, q2, a.pr.tof, Bq.z, s))
+			return;
 		if (!first)
-			sidRestart<S64>(a.im, s, pl.alpha[j], vx, vy, vz, dx, dy, dz);
+			sidRestart<S64>(a.im, s, al, vx, vy, vz, dx, dy, dz);
 		sidWalk<TOF, S64>(a.im, s, a.pr.tof, B, first, vx, vy, vz,
 		                  [&](int, int, int, int idx, float, float w) {
 			                  ldsAdd(&acc[idx], val * w);
 		                  });
+	};
+	int i = threadIdx.x;
+	uint32_t k0, p0, k1, p1;
+	float a0, a1, v0;
+	float4 A0, B0;
+	readRecord(i, k0, a0, p0);
+	readRay(k0, A0, B0, v0);
+	readRecord(i + NT, k1, a1, p1);
+	for (; i < ch.count; i += NT)
+	{
+		float4 A1, B1;
+		float v1;
+		readRay(k1, A1, B1, v1);
+		uint32_t k2, p2;
+		float a2;
+		readRecord(i + 2 * NT, k2, a2, p2);
+		walk(k0, a0, p0, A0, B0, v0);
+		k0 = k1;
+		a0 = a1;
+		p0 = p1;
+		A0 = A1;
+		B0 = B1;
+		v0 = v1;
+		k1 = k2;
+		a1 = a2;
+		p1 = p2;
 	}
 	__syncthreads();
 	float* yf = y + (int64_t)f * a.im.nvox;
@@ -299,7 +357,21 @@ __global__ void __launch_bounds__(256) kEmRatio(KArgs a, float* __restrict__ v)
 	{
 		const int64_t e = t / LPE;
 		const int lane = (int)(t & (LPE - 1));
-		const Lor l = fetchLor(a.sc, a.L, e, a.im.nt);
+		const int R = a.pr.numRays;
+		// the LOR from the plan's ray table (consecutive LORs: coalesced), or
+		// resolved from the detector pair
+		Lor l;
+		float4 A, B;
+		if (a.rays != nullptr)
+		{
+			const int64_t t0 = e * R + min(lane, R - 1);
+			A = a.rays[2 * t0];
+			B = a.rays[2 * t0 + 1];
+			l.frame = __float_as_int(B.w);
+			l.tof = B.z;
+		}
+		else
+			l = fetchLor(a.sc, a.L, e, a.im.nt);
 		const float m = a.c.meas ? a.c.meas[e] : 1.0f;
 		const float sf = a.c.sens ? a.c.sens[e] : 1.0f;
 		const float af = a.c.att ? a.c.att[e] : 1.0f;
@@ -308,12 +380,25 @@ __global__ void __launch_bounds__(256) kEmRatio(KArgs a, float* __restrict__ v)
 		if (!skip)
 		{
 			const float* img = a.x + (int64_t)l.frame * a.im.nvox;
-			for (int r = lane; r < a.pr.numRays; r += LPE)
+			for (int r = lane; r < R; r += LPE)
 			{
 				float3 q1, q2;
-				rayLine(a.pr, l, r, q1, q2);
-				q1 = make_float3(q1.x - a.im.ox, q1.y - a.im.oy, q1.z - a.im.oz);
-				q2 = make_float3(q2.x - a.im.ox, q2.y - a.im.oy, q2.z - a.im.oz);
+				if (a.rays != nullptr)
+				{
+					if (r != lane)
+					{
+						A = a.rays[2 * (e * R + r)];
+						B = a.rays[2 * (e * R + r) + 1];
+					}
+					q1 = make_float3(A.x, A.y, A.z);
+					q2 = make_float3(A.w, B.x, B.y);
+				}
+				else
+				{
+					rayLine(a.pr, l, r, q1, q2);
+					q1 = make_float3(q1.x - a.im.ox, q1.y - a.im.oy, q1.z - a.im.oz);
+					q2 = make_float3(q2.x - a.im.ox, q2.y - a.im.oy, q2.z - a.im.oz);
+				}
 				uint32_t c = 0;
 				s += siddonRay<MODE_FWD, TOF, S64>(a.im, const_cast<float*>(img), q1,
 				                                   q2, 0.f, a.pr.tof, l.tof, c);

@@ -68,8 +68,20 @@ struct DPlan
 	uint32_t* key;              // lor * R + ray
 	float* alpha;               // alpha where the walk enters the tile
 	uint32_t* vox;              // packed entry voxel (packEntry)
+	float4* ray;                // [2 * n * R] ray table (rayEntry), fill pass
 	int* err;                   // set when a tile range would overflow
 };
+
+// Ray table of a plan, two float4 per (LOR, ray) t = lor * R + ray: the
+// ray's end points after multi-ray offsets, motion and the image offset
+// (q1, q2, exactly as the walk sets it up), its TOF value and its frame
+// (-1: skipped).  The backprojection reads it instead of resolving the LOR.
+__device__ __forceinline__ void rayEntry(float4* tab, uint32_t t, float3 q1, float3 q2,
+                                         float tof, int32_t frame)
+{
+	tab[2 * (uint64_t)t] = make_float4(q1.x, q1.y, q1.z, q2.x);
+	tab[2 * (uint64_t)t + 1] = make_float4(q2.y, q2.z, tof, __int_as_float(frame));
+}
 
 // Plan in use.  queue != null: persistent launch, workgroups take chunks
 // from queue[0] (queue[1] counts finished workgroups; the last one resets
@@ -82,6 +94,7 @@ struct DPlanRead
 	const uint32_t* key;
 	const float* alpha;
 	const uint32_t* vox;
+	const float4* ray;  // rayEntry table
 	unsigned int* queue;
 };
 
