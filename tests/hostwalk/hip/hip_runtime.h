@@ -18,6 +18,10 @@ struct float3
 	float x, y, z;
 };
 inline float3 make_float3(float a, float b, float c) { return {a, b, c}; }
+struct float4
+{
+	float x, y, z, w;
+};
 inline float __fadd_rn(float a, float b) { return a + b; }
 inline float __fsub_rn(float a, float b) { return a - b; }
 inline float __fmul_rn(float a, float b) { return a * b; }

@@ -140,6 +140,62 @@ def test_back_planned_frames_motion_offset(gpu):
     check("frames_planned_vs_oracle", b.cpu().numpy(), ref, TOL_ORACLE)
 
 
+def test_back_planned_fixed_point_range(gpu):
+    """Tile sums are accumulated in 64-bit fixed point scaled from the
+    launch's values (terms out of range go to global atomics): values spread
+    over 12 decades, or of both signs, match the direct FP32 backprojection
+    (bars: 2e-6; 1e-5 for signed values, whose cancellation magnifies the
+    direct path's own FP32 rounding) and the FP32 oracle (1e-5)."""
+    sc, im, pp, ctx, p = setup(gpu, "dyadic", 4, 250.0, 0, 4)
+    n = 6000
+    ev = events(sc, n, 103, 250.0)
+    rng = np.random.default_rng(5)
+    L = dev_lors(ev, gpu)
+    plan = ctx.plan(p, im, L)
+    cases = (("decades", (10.0 ** rng.uniform(-6, 6, n)).astype(np.float32), TOL_DIRECT),
+             ("signed", rng.uniform(-1.0, 1.0, n).astype(np.float32), 1e-5))
+    for name, vals, tol in cases:
+        vg = torch.from_numpy(vals).to(gpu)
+        a = torch.zeros(im.shape, device=gpu)
+        b = torch.zeros(im.shape, device=gpu)
+        ctx.back(p, im, a, L, vg)
+        ctx.back(p, im, b, L, vg, plan=plan)
+        ctx.synchronize()
+        check(f"fixed_{name}_planned_vs_direct", b.cpu().numpy(), a.cpu().numpy(), tol)
+    ref = O.back(oracle_image(im), oracle_proj(pp), oracle_lors(sc, ev), vals)
+    check("fixed_signed_planned_vs_oracle", b.cpu().numpy(), ref, 1e-5)
+
+
+def test_back_planned_nonfinite_values(gpu):
+    """A launch whose values are not all finite takes the FP32 tile path:
+    the same voxels become non-finite as in the direct backprojection and
+    the finite ones agree."""
+    sc, im, pp, ctx, p = setup(gpu, "dyadic", 1, None, 0, 1)
+    n = 4000
+    ev = events(sc, n, 104, None)
+    vals = np.random.default_rng(6).uniform(0.1, 1.0, n).astype(np.float32)
+    L = dev_lors(ev, gpu)
+    plan = ctx.plan(p, im, L)
+    # the LOR with the most records, so that the infinity reaches the image
+    x1 = torch.ones(im.shape, device=gpu)
+    fw = torch.zeros(n, device=gpu)
+    ctx.forward(p, im, x1, L, fw)
+    ctx.synchronize()
+    vals[int(torch.argmax(fw).item())] = np.inf
+    vg = torch.from_numpy(vals).to(gpu)
+    a = torch.zeros(im.shape, device=gpu)
+    b = torch.zeros(im.shape, device=gpu)
+    ctx.back(p, im, a, L, vg)
+    ctx.back(p, im, b, L, vg, plan=plan)
+    ctx.synchronize()
+    a, b = a.cpu().numpy(), b.cpu().numpy()
+    fa, fb = np.isfinite(a), np.isfinite(b)
+    print(f"non-finite voxels: direct {int((~fa).sum())}, planned {int((~fb).sum())}")
+    assert (~fa).sum() > 0
+    assert np.array_equal(fa, fb)
+    check("nonfinite_finite_part", b[fb], a[fa], TOL_DIRECT)
+
+
 @pytest.mark.parametrize("rays,tof", [(1, None), (4, 250.0)])
 def test_em_subset_planned(gpu, rays, tof):
     sc, im, pp, ctx, p = setup(gpu, "dyadic", rays, tof, 0, 4 if rays == 4 else 1)

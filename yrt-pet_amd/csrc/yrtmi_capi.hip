@@ -1077,6 +1077,7 @@ DPlanRead planRead(const yrtmi_plan* q)
 	r.vox = q->vox;
 	r.ray = q->ray;
 	r.queue = q->queue;
+	r.vexp = reinterpret_cast<const int*>(q->queue + 2);
 	return r;
 }
 
@@ -1134,8 +1135,11 @@ int yrtmi_plan_create(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
 	q->useTof = p->d.useTof;
 	q->T = T;
 	q->persistent = o.persistent;
-	if (hipMalloc(&q->queue, 2 * sizeof(unsigned int)) != hipSuccess ||
-	    hipMemset(q->queue, 0, 2 * sizeof(unsigned int)) != hipSuccess)
+	// [0] next chunk, [1] finished workgroups (persistent launches),
+	// [2] fixed-point scale exponent of the current launch, [3] unused,
+	// [4 .. 259] exponent histogram of its values
+	if (hipMalloc(&q->queue, 260 * sizeof(unsigned int)) != hipSuccess ||
+	    hipMemset(q->queue, 0, 260 * sizeof(unsigned int)) != hipSuccess)
 	{
 		freePlan(q);
 		return fail(YRTMI_ENOMEM, "plan: chunk queue");

@@ -30,6 +30,12 @@ __device__ __forceinline__ void ldsAdd(float* p, float v)
 	__hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+__device__ __forceinline__ void ldsAdd64(unsigned long long* p, long long v)
+{
+	__hip_atomic_fetch_add(p, (unsigned long long)v, __ATOMIC_RELAXED,
+	                       __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 __device__ __forceinline__ int tileOf(const DTiles& T, int f, int vx, int vy,
                                       int vz)
 {
@@ -171,12 +177,33 @@ __global__ void __launch_bounds__(256) kBin(KArgs a, DTiles T, DPlan pl)
 // records: the fields of the record two ahead are read, the ray-table entry
 // and value of the next one are in flight, and the current one is set up
 // from its table entry and walked (the walk itself issues no global loads).
+//
+// Tile accumulation.  FP32 atomic adds into LDS run at about half a lane per
+// clock per CU and bounded the kernel; 64-bit integer LDS atomics do not.
+// So the tile is accumulated in fixed point, each contribution c = v/R * w
+// rounded once to a multiple of 2^-k and added as an integer (exact and
+// order-free, so the tile sum is deterministic); the flush converts back
+// (int64 -> float, times 2^-k: exact scaling).  A voxel takes at most one
+// contribution per record, so with every term below 2^(62 - e2) in
+// magnitude (records < 2^e2) no partial sum can overflow; a term that does
+// not fit — and any non-finite one — is added to the image directly with a
+// global FP32 atomic, as the direct backprojection does.  k is chosen from
+// the launch's values: E, the binade of the median nonzero |v| (kVExpHist,
+// kPickExp), 2^10 of headroom, and wmax, the largest segment weight (voxel
+// diagonal, times the TOF norm): k = 62 - e2 - (E + 10 + ew), wmax < 2^ew.
+// Typical contributions then keep ~35 bits below their magnitude; values
+// more than ~1000 times the median (the heavy tail of EM ratios m / Ax on
+// LORs with tiny projections) cost a global atomic instead of precision.
+// The scale only trades precision against that fallback: no choice of it
+// can overflow the tile.  The 32^3 shape (whose 64-bit tile would need
+// 256 KB of LDS) accumulates in FP32.
 template <bool TOF, bool S64, int SHAPE>
 __device__ __forceinline__ void tileChunk(const KArgs& a, const DTiles& T,
                                           const DPlanRead& pl, const TileChunk ch,
                                           const float* __restrict__ v,
-                                          float* __restrict__ y, float* acc)
+                                          float* __restrict__ y, unsigned long long* acc64)
 {
+	float* acc = reinterpret_cast<float*>(acc64);
 	constexpr int TX = tileDim(SHAPE, 0), TY = tileDim(SHAPE, 1), TZ = tileDim(SHAPE, 2);
 	constexpr int NV = TX * TY * TZ, NT = tileThreads(SHAPE);
 	int t = ch.tile;
@@ -196,8 +223,31 @@ __device__ __forceinline__ void tileChunk(const KArgs& a, const DTiles& T,
 	B.sy = TX;
 	B.sz = TX * TY;
 	B.base = -(B.z0 * B.sz + B.y0 * B.sy + B.x0);
-	for (int i = threadIdx.x; i < NV; i += NT)
-		acc[i] = 0.f;
+	constexpr int NW = (SHAPE == 2) ? NV / 2 : NV;  // 64-bit words of the LDS tile
+	for (int i = threadIdx.x; i < NW; i += NT)
+		acc64[i] = 0ull;
+	// fixed-point scale of this chunk (uniform)
+	int fixed = 0, k = 0;
+	float limit = 0.f;
+	if constexpr (SHAPE != 2)
+	{
+		if (pl.vexp != nullptr)
+		{
+			float wmax = 1.01f * sqrtf(a.im.vx * a.im.vx + a.im.vy * a.im.vy +
+			                           a.im.vz * a.im.vz);
+			if (TOF)
+				wmax *= 1.01f * a.pr.tof.norm;
+			int ew = 0;
+			(void)frexpf(wmax, &ew);
+			const int e2 = 32 - __clz(ch.count);
+			k = 62 - e2 - (*pl.vexp + 10 + ew);
+			limit = ldexpf(1.f, 62 - e2);
+			fixed = 1;
+		}
+	}
+	fixed = __builtin_amdgcn_readfirstlane(fixed);
+	k = __builtin_amdgcn_readfirstlane(k);
+	float* yf = y + (int64_t)f * a.im.nvox;
 	__syncthreads();
 	const uint32_t R = (uint32_t)pl.R;
 	const bool rPow2 = (R & (R - 1u)) == 0u;
@@ -231,7 +281,8 @@ __device__ __forceinline__ void tileChunk(const KArgs& a, const DTiles& T,
 			val = v[lorOf(key)];
 		}
 	};
-	auto walk = [&](uint32_t key, float al, uint32_t pv, float4 A, float4 Bq, float val) {
+	auto walk = [&](uint32_t key, float al, uint32_t pv, float4 A, float4 Bq, float val,
+	                auto&& add) {
 		if ((uint64_t)key >= nRays || val == 0.f)
 			return;
 		if (R > 1u)
@@ -252,43 +303,54 @@ __device__ __forceinline__ void tileChunk(const KArgs& a, const DTiles& T,
 		if (!first)
 			sidRestart<S64>(a.im, s, al, vx, vy, vz, dx, dy, dz);
 		sidWalk<TOF, S64>(a.im, s, a.pr.tof, B, first, vx, vy, vz,
-		                  [&](int, int, int, int idx, float, float w) {
-			                  ldsAdd(&acc[idx], val * w);
+		                  [&](int gx, int gy, int gz, int idx, float, float w) {
+			                  add(idx, gx, gy, gz, val * w);
 		                  });
 	};
-	int i = threadIdx.x;
-	uint32_t k0, p0, k1, p1;
-	float a0, a1, v0;
-	float4 A0, B0;
-	readRecord(i, k0, a0, p0);
-	readRay(k0, A0, B0, v0);
-	readRecord(i + NT, k1, a1, p1);
-	for (; i < ch.count; i += NT)
-	{
-		float4 A1, B1;
-		float v1;
-		readRay(k1, A1, B1, v1);
-		uint32_t k2, p2;
-		float a2;
-		readRecord(i + 2 * NT, k2, a2, p2);
-		walk(k0, a0, p0, A0, B0, v0);
-		k0 = k1;
-		a0 = a1;
-		p0 = p1;
-		A0 = A1;
-		B0 = B1;
-		v0 = v1;
-		k1 = k2;
-		a1 = a2;
-		p1 = p2;
-	}
+	auto pipeline = [&](auto&& add) {
+		int i = threadIdx.x;
+		uint32_t k0, p0, k1, p1;
+		float a0, a1, v0;
+		float4 A0, B0;
+		readRecord(i, k0, a0, p0);
+		readRay(k0, A0, B0, v0);
+		readRecord(i + NT, k1, a1, p1);
+		for (; i < ch.count; i += NT)
+		{
+			float4 A1, B1;
+			float v1;
+			readRay(k1, A1, B1, v1);
+			uint32_t k2, p2;
+			float a2;
+			readRecord(i + 2 * NT, k2, a2, p2);
+			walk(k0, a0, p0, A0, B0, v0, add);
+			k0 = k1;
+			a0 = a1;
+			p0 = p1;
+			A0 = A1;
+			B0 = B1;
+			v0 = v1;
+			k1 = k2;
+			a1 = a2;
+			p1 = p2;
+		}
+	};
+	if (fixed)
+		pipeline([&](int idx, int gx, int gy, int gz, float c) {
+			const float xs = ldexpf(c, k);
+			if (fabsf(xs) < limit)
+				ldsAdd64(&acc64[idx], (long long)__builtin_rintf(xs));
+			else  // out of the fixed-point range, or not finite
+				unsafeAtomicAdd(yf + (int64_t)gz * a.im.nxy + (int64_t)gy * a.im.nx + gx, c);
+		});
+	else
+		pipeline([&](int idx, int, int, int, float c) { ldsAdd(&acc[idx], c); });
 	__syncthreads();
-	float* yf = y + (int64_t)f * a.im.nvox;
 	for (int i = threadIdx.x; i < NV; i += NT)
 	{
 		const int lx = i % TX, ly = (i / TX) % TY, lz = i / (TX * TY);
 		const int gx = B.x0 + lx, gy = B.y0 + ly, gz = B.z0 + lz;
-		const float s = acc[i];
+		const float s = fixed ? ldexpf((float)(long long)acc64[i], -k) : acc[i];
 		if (s != 0.f && gx < B.x1 && gy < B.y1 && gz < B.z1)
 			unsafeAtomicAdd(yf + (int64_t)gz * a.im.nxy + (int64_t)gy * a.im.nx + gx, s);
 	}
@@ -303,7 +365,9 @@ __global__ void __launch_bounds__(tileThreads(SHAPE))
     kTileBack(KArgs a, DTiles T, DPlanRead pl, const float* __restrict__ v,
               float* __restrict__ y)
 {
-	__shared__ float acc[tileDim(SHAPE, 0) * tileDim(SHAPE, 1) * tileDim(SHAPE, 2)];
+	// NV 64-bit words (fixed point), or NV floats for the 32^3 shape
+	constexpr int NV = tileDim(SHAPE, 0) * tileDim(SHAPE, 1) * tileDim(SHAPE, 2);
+	__shared__ unsigned long long acc[SHAPE == 2 ? NV / 2 : NV];
 	if (pl.queue == nullptr)
 	{
 		tileChunk<TOF, S64, SHAPE>(a, T, pl, pl.chunks[blockIdx.x], v, y, acc);
@@ -509,11 +573,71 @@ static void tileBackShape(const KArgs& a, const DTiles& T, const DPlanRead& pl,
 	}
 }
 
+__global__ void __launch_bounds__(256) kZeroWords(unsigned int* p, int n)
+{
+	for (int i = threadIdx.x; i < n; i += blockDim.x)
+		p[i] = 0u;
+}
+
+// hist[b] += number of i with biased exponent field b of |v[i]| (255: not
+// finite).
+__global__ void __launch_bounds__(256) kVExpHist(const float* __restrict__ v, int64_t n,
+                                                 unsigned int* hist)
+{
+	__shared__ unsigned int h[256];
+	h[threadIdx.x] = 0u;
+	__syncthreads();
+	const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+	for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+		atomicAdd(&h[(__float_as_uint(v[i]) >> 23) & 0xffu], 1u);
+	__syncthreads();
+	if (h[threadIdx.x] != 0u)
+		atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
+}
+
+// *e = E, the binade of the median of the nonzero finite |v|: the smallest
+// exponent with |v| < 2^E for at least half of them (|v| < 2^(b - 126) for
+// biased exponent b); -126 when every value is zero or subnormal.
+__global__ void kPickExp(const unsigned int* __restrict__ hist, int64_t n, int* e)
+{
+	if (threadIdx.x != 0)
+		return;
+	uint64_t total = 0;
+	for (int b = 1; b <= 254; b++)
+		total += hist[b];
+	uint64_t above = 0;
+	int E = -126;
+	for (int b = 254; b >= 1 && total > 0; b--)
+	{
+		above += hist[b];
+		if (2 * above >= total)
+		{
+			E = b - 126;
+			break;
+		}
+	}
+	*e = E;
+}
+
 hipError_t launchTileBack(const KArgs& a, const DTiles& T, const DPlanRead& pl,
                           const float* v, float* y, int persistent, hipStream_t st)
 {
 	if (pl.nChunks <= 0)
 		return hipSuccess;
+	if (pl.vexp != nullptr)
+	{
+		// the scale word is followed by the 256-bin exponent histogram
+		int* ve = const_cast<int*>(pl.vexp);
+		unsigned int* hist = reinterpret_cast<unsigned int*>(ve) + 2;
+		const int64_t n = a.L.n;
+		kZeroWords<<<1, 256, 0, st>>>(hist, 256);
+		const unsigned g = (unsigned)std::min<int64_t>(std::max<int64_t>((n + 2047) / 2048, 1), 2048);
+		kVExpHist<<<g, 256, 0, st>>>(v, n, hist);
+		kPickExp<<<1, 64, 0, st>>>(hist, n, ve);
+		const hipError_t e = hipGetLastError();
+		if (e != hipSuccess)
+			return e;
+	}
 	DPlanRead q = pl;
 	unsigned grid;
 	if (persistent > 0 && pl.queue != nullptr)

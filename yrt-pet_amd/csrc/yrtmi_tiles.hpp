@@ -96,6 +96,10 @@ struct DPlanRead
 	const uint32_t* vox;
 	const float4* ray;  // rayEntry table
 	unsigned int* queue;
+	// binade E of the median nonzero |v| of the launch (kVExpHist + kPickExp
+	// before each launch): sets the fixed-point scale of the tile
+	// accumulation
+	const int* vexp;
 };
 
 // pass 0: count; pass 1: fill.  Dynamic LDS 8 * nTiles * C bytes.
