@@ -364,13 +364,16 @@ int yrtmi_permute(yrtmi_ctx* ctx, const void* src, int elem_bytes,
  * allocates device memory (yrtmi_plan_info's bytes); the plan owns it.
  *
  * Options (NULL: defaults): chunk_records, records per workgroup (0: 4096);
- * tile_shape, YRTMI_TILE_32x32x16 (64 KB of LDS, default),
- * YRTMI_TILE_16x16x16 (16 KB) or YRTMI_TILE_32x32x32 (128 KB); persistent,
+ * tile_shape, YRTMI_TILE_16x16x16 (default), YRTMI_TILE_32x32x16,
+ * YRTMI_TILE_32x32x32 or YRTMI_TILE_32x32x8 (voxels of the image held per
+ * workgroup; the tile is accumulated in LDS in 64-bit fixed point, FP32 for
+ * the 32x32x32 shape); persistent,
  * 0 for one workgroup per chunk, else that many workgroups taking chunks from
  * a device queue (changeable with yrtmi_plan_set_persistent). */
-#define YRTMI_TILE_32x32x16 0
-#define YRTMI_TILE_16x16x16 1
+#define YRTMI_TILE_16x16x16 0
+#define YRTMI_TILE_32x32x16 1
 #define YRTMI_TILE_32x32x32 2
+#define YRTMI_TILE_32x32x8 3
 typedef struct
 {
 	int chunk_records;
@@ -402,7 +405,7 @@ typedef struct
 	float ratio_ms;          /* EM ratio kernel, best lanes per event */
 	float back_ms;           /* planned backprojection, best settings */
 	float ratio_ms_lpe[4];   /* lanes per event 1, 2, 4, 8 */
-	float back_ms_shape[3][2]; /* [tile shape][one per chunk, persistent] */
+	float back_ms_shape[4][2]; /* [tile shape][one per chunk, persistent] */
 } yrtmi_tuning;
 int yrtmi_autotune(yrtmi_ctx* ctx, yrtmi_proj* p, const yrtmi_image* im,
                    const yrtmi_lors* lors, const float* x, yrtmi_tuning* best);

@@ -45,7 +45,7 @@ def setup(gpu, n=20000, nt=1, seed=2, stream=False):
     return sc, im, pp, ctx, proj, ev, L, rng
 
 
-@pytest.mark.parametrize("shape", [0, 1, 2])
+@pytest.mark.parametrize("shape", [0, 1, 2, 3])
 @pytest.mark.parametrize("persistent", [0, 300])
 def test_tile_shapes_and_persistent_queue(gpu, shape, persistent):
     sc, im, pp, ctx, proj, ev, L, rng = setup(gpu)
@@ -66,7 +66,7 @@ def test_autotune(gpu):
     t = ctx.autotune(proj, im, L, x)
     print("autotune:", t)
     assert t["lanes_per_event"] in (1, 2, 4, 8)
-    assert t["tile_shape"] in (0, 1, 2)
+    assert t["tile_shape"] in (0, 1, 2, 3)
     assert 0 < t["ratio_ms"] == min(t["ratio_ms_lpe"])
     assert 0 < t["back_ms"] == min(min(r) for r in t["back_ms_shape"])
     # the tuned settings give the same EM update image

@@ -1119,6 +1119,9 @@ int yrtmi_plan_create(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
 	T.tx = tileDim(T.shape, 0);
 	T.ty = tileDim(T.shape, 1);
 	T.tz = tileDim(T.shape, 2);
+	T.lx = __builtin_ctz((unsigned)T.tx);
+	T.ly = __builtin_ctz((unsigned)T.ty);
+	T.lz = __builtin_ctz((unsigned)T.tz);
 	T.ntx = (int32_t)((im->nx + T.tx - 1) / T.tx);
 	T.nty = (int32_t)((im->ny + T.ty - 1) / T.ty);
 	T.ntz = (int32_t)((im->nz + T.tz - 1) / T.tz);

@@ -39,7 +39,7 @@ __device__ __forceinline__ void ldsAdd64(unsigned long long* p, long long v)
 __device__ __forceinline__ int tileOf(const DTiles& T, int f, int vx, int vy,
                                       int vz)
 {
-	return ((f * T.ntz + vz / T.tz) * T.nty + vy / T.ty) * T.ntx + vx / T.tx;
+	return ((f * T.ntz + (vz >> T.lz)) * T.nty + (vy >> T.ly)) * T.ntx + (vx >> T.lx);
 }
 
 // Walks ray r of LOR e (global ray index t = e * R + r) and calls
@@ -91,7 +91,7 @@ __device__ __forceinline__ void binRay(const KArgs& a, const DTiles& T,
 			                  if (last >= 0)
 				                  onRecord(last, pa, pv, nseg - start);
 			                  pa = a0;
-			                  pv = packEntry(vx % T.tx, vy % T.ty, vz % T.tz, dx, dy, dz,
+			                  pv = packEntry(vx & (T.tx - 1), vy & (T.ty - 1), vz & (T.tz - 1), dx, dy, dz,
 			                                 last < 0);
 			                  start = nseg;
 			                  last = t;
@@ -566,6 +566,9 @@ static void tileBackShape(const KArgs& a, const DTiles& T, const DPlanRead& pl,
 		break;
 	case 2:
 		tileBackT<TOF, S64, 2>(a, T, pl, v, y, grid, st);
+		break;
+	case 3:
+		tileBackT<TOF, S64, 3>(a, T, pl, v, y, grid, st);
 		break;
 	default:
 		tileBackT<TOF, S64, 0>(a, T, pl, v, y, grid, st);

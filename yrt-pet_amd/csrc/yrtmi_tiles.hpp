@@ -7,17 +7,18 @@
 namespace yrtmi
 {
 
-// Image tiles held in LDS by one workgroup (FP32 voxels), by shape:
-//   0: 32 x 32 x 16 (64 KB, 512 threads)   1: 16 x 16 x 16 (16 KB, 256)
-//   2: 32 x 32 x 32 (128 KB, 1024 threads)
-constexpr int kNumTileShapes = 3;
+// Image tiles held in LDS by one workgroup, by shape (64-bit fixed-point
+// voxels, FP32 for shape 2):
+//   0: 16 x 16 x 16 (32 KB, 256 threads)   1: 32 x 32 x 16 (128 KB, 512)
+//   2: 32 x 32 x 32 (128 KB FP32, 1024)    3: 32 x 32 x 8 (64 KB, 512)
+constexpr int kNumTileShapes = 4;
 __host__ __device__ constexpr int tileDim(int shape, int axis)
 {
-	return shape == 1 ? 16 : (shape == 2 ? 32 : (axis == 2 ? 16 : 32));
+	return shape == 0 ? 16 : (shape == 2 ? 32 : (axis < 2 ? 32 : (shape == 1 ? 16 : 8)));
 }
 __host__ __device__ constexpr int tileThreads(int shape)
 {
-	return shape == 1 ? 256 : (shape == 2 ? 1024 : 512);
+	return shape == 0 ? 256 : (shape == 2 ? 1024 : 512);
 }
 // Binning keeps two int counters per (tile, length class) in LDS (64 KB at
 // most): tiles x classes <= kMaxBins.
@@ -43,6 +44,7 @@ struct DTiles
 {
 	int32_t ntx, nty, ntz, nTiles;
 	int32_t tx, ty, tz, shape;
+	int32_t lx, ly, lz;  // log2 of tx, ty, tz (tile sides are powers of two)
 };
 
 // A chunk of one tile's records, processed by one workgroup.

@@ -78,13 +78,13 @@ class PlanOptions(C.Structure):
     _fields_ = [("chunk_records", C.c_int), ("tile_shape", C.c_int), ("persistent", C.c_int)]
 
 
-TILE_32x32x16, TILE_16x16x16, TILE_32x32x32 = 0, 1, 2
+TILE_16x16x16, TILE_32x32x16, TILE_32x32x32, TILE_32x32x8 = 0, 1, 2, 3
 
 
 class Tuning(C.Structure):
     _fields_ = [("lanes_per_event", C.c_int), ("tile_shape", C.c_int), ("persistent", C.c_int),
                 ("ratio_ms", C.c_float), ("back_ms", C.c_float),
-                ("ratio_ms_lpe", C.c_float * 4), ("back_ms_shape", (C.c_float * 2) * 3)]
+                ("ratio_ms_lpe", C.c_float * 4), ("back_ms_shape", (C.c_float * 2) * 4)]
 
 _SIGS = {
     "yrtmi_last_error": (C.c_char_p, []),
