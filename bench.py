@@ -96,6 +96,8 @@ def parse():
                          "atomics into per-XCD images / the target directly")
     ap.add_argument("--chunk", type=int, default=0, help="tile plan records per workgroup")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
+    ap.add_argument("--tile-shape", type=int, default=0,
+                    help="tile shape of untuned runs (yrtmi.capi.TILE_*)")
     ap.add_argument("--order", choices=("lor", "none"), default="lor",
                     help="project each subset's events in locality order (yrtmi_lor_order)")
     ap.add_argument("--no-graph", action="store_true", help="launch the iteration call by call")
@@ -235,7 +237,7 @@ def bench_listmode(args, torch, dev, stream):
     plans, plan_s, plan_bytes, plan_records, tune, tune_s = [], 0.0, 0, 0, None, 0.0
     vbuf = None
     if tiled:
-        shape, pers = 0, 0
+        shape, pers = args.tile_shape, 0
         if not args.no_tune:
             t0 = time.time()
             tune = ctx.autotune(proj, im, sub[0], x)
@@ -315,7 +317,7 @@ def bench_listmode(args, torch, dev, stream):
     if tiled:
         roof(f"kEmRatio<TOF,LPE={lpe}> (forward + ratio)", (0, 1),
              [12.0 * nev[s] + 4.0 * segs[s] + 4.0 * nev[s] for s in range(S)])
-        shape = tune["tile_shape"] if tune else 0
+        shape = tune["tile_shape"] if tune else args.tile_shape
         roof(f"kTileBack<TOF,shape={shape}> (tile-binned backprojection)", (1, 2),
              [12.0 * nev[s] + 4.0 * nev[s] + 8.0 * segs[s] for s in range(S)])
     else:

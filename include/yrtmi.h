@@ -359,8 +359,9 @@ int yrtmi_permute(yrtmi_ctx* ctx, const void* src, int elem_bytes,
  * calls' up to FP32 summation order (and, on voxel sizes that are not short
  * binary fractions, ulp-level shifts of segment ends at tile boundaries).
  *
- * Limits: Siddon only; nx, ny, nz <= 1024; n * num_rays < 2^32; at most 8192
- * tiles (frames included).  The call synchronises the context's stream and
+ * Limits: Siddon only; nx, ny, nz <= 1024; n * num_rays < 2^32; tiles
+ * (frames included) at most the device's LDS per workgroup / 12 bytes
+ * (5459 with 64 KB).  The call synchronises the context's stream and
  * allocates device memory (yrtmi_plan_info's bytes); the plan owns it.
  *
  * Options (NULL: defaults): chunk_records, records per workgroup (0: 4096);

@@ -1126,8 +1126,21 @@ int yrtmi_plan_create(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
 	T.nty = (int32_t)((im->ny + T.ty - 1) / T.ty);
 	T.ntz = (int32_t)((im->nz + T.tz - 1) / T.tz);
 	const int64_t nT = (int64_t)T.ntx * T.nty * T.ntz * im->nt;
-	if (nT > kMaxTiles)
-		return fail(YRTMI_EINVAL, "plan: more than 8192 image tiles (frames included)");
+	// the fill pass keeps a counter and a 64-bit base per bin in LDS
+	int maxLds = 65536;
+	if (hipDeviceGetAttribute(&maxLds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) !=
+	    hipSuccess)
+		maxLds = 65536;
+	const int64_t binCap = (int64_t)maxLds / 12 - 2;
+	if (nT > binCap)
+	{
+		char msg[160];
+		std::snprintf(msg, sizeof msg,
+		              "plan: %lld image tiles (frames included), at most %lld: use a larger "
+		              "tile shape",
+		              (long long)nT, (long long)binCap);
+		return fail(YRTMI_EINVAL, msg);
+	}
 	T.nTiles = (int32_t)nT;
 
 	yrtmi_plan* q = new yrtmi_plan();
@@ -1158,11 +1171,13 @@ int yrtmi_plan_create(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
 	// scratch of the build
 	unsigned long long* dCount = nullptr;
 	unsigned int* dCursor = nullptr;
+	unsigned int* dBlockCnt = nullptr;
 	int64_t* dOff = nullptr;
 	int* dErr = nullptr;
 	auto cleanup = [&](int rc) {
 		hipFree(dCount);
 		hipFree(dCursor);
+		hipFree(dBlockCnt);
 		hipFree(dOff);
 		hipFree(dErr);
 		if (rc != YRTMI_OK)
@@ -1173,13 +1188,13 @@ int yrtmi_plan_create(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
 	};
 	if (l->n == 0)
 		return cleanup(YRTMI_OK);
-	// length classes: as many as the LDS counters allow (tiles x classes <=
-	// kMaxBins), cut at fractions of the longest piece a tile can hold
-	// (tx + ty + tz segments)
+	// length classes: up to 8, as long as tiles x classes stays within
+	// 4096 bins (48 KB of fill-pass LDS) and the LDS limit; cut at fractions
+	// of the longest piece a tile can hold (tx + ty + tz segments)
 	DPlan pl{};
 	pl.R = R;
 	pl.C = 1;
-	while (pl.C < kMaxClasses && nT * pl.C * 2 <= kMaxBins)
+	while (pl.C < kMaxClasses && nT * pl.C * 2 <= std::min<int64_t>(binCap, kMaxBins))
 		pl.C *= 2;
 	{
 		const int L = T.tx + T.ty + T.tz;
@@ -1189,11 +1204,17 @@ int yrtmi_plan_create(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
 			pl.cut[k] = std::max(1, L * f[k] / 40);
 	}
 	const int64_t nB = nT * pl.C;
+	const int64_t rays = l->n * (int64_t)R;
+	const int grid = (int)std::min<int64_t>((rays + 255) / 256, 2048);
 	if (hipMalloc(&dCount, sizeof(unsigned long long) * nB) != hipSuccess ||
 	    hipMalloc(&dCursor, sizeof(unsigned int) * nB) != hipSuccess ||
+	    hipMalloc(&dBlockCnt, sizeof(unsigned int) * nB * grid) != hipSuccess ||
 	    hipMalloc(&dOff, sizeof(int64_t) * nB) != hipSuccess ||
 	    hipMalloc(&dErr, sizeof(int)) != hipSuccess)
 		return cleanup(fail(YRTMI_ENOMEM, "plan: build scratch"));
+	if (hipMalloc(&q->ray, 2 * sizeof(float4) * rays) != hipSuccess)
+		return cleanup(fail(YRTMI_ENOMEM, "plan: ray table"));
+	q->bytes += rays * 2 * (int64_t)sizeof(float4);
 	hipError_t e = hipMemsetAsync(dCount, 0, sizeof(unsigned long long) * nB, c->stream);
 	if (e == hipSuccess)
 		e = hipMemsetAsync(dCursor, 0, sizeof(unsigned int) * nB, c->stream);
@@ -1204,11 +1225,13 @@ int yrtmi_plan_create(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
 
 	pl.count = dCount;
 	pl.cursor = dCursor;
+	pl.blockCnt = dBlockCnt;
 	pl.tileOff = dOff;
 	pl.err = dErr;
-	const int64_t rays = l->n * (int64_t)R;
-	const int grid = (int)std::min<int64_t>((rays + 255) / 256, 2048);
-	e = launchBin(0, a, T, pl, grid, c->stream);
+	pl.ray = q->ray;
+	e = launchRayPrep(a, q->ray, c->stream);
+	if (e == hipSuccess)
+		e = launchBin(0, a, T, pl, grid, c->stream);
 	std::vector<unsigned long long> cnt(nB);
 	if (e == hipSuccess)
 		e = hipMemcpyAsync(cnt.data(), dCount, sizeof(unsigned long long) * nB,
@@ -1256,11 +1279,9 @@ int yrtmi_plan_create(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
 		if (hipMalloc(&q->key, sizeof(uint32_t) * total) != hipSuccess ||
 		    hipMalloc(&q->alpha, sizeof(float) * total) != hipSuccess ||
 		    hipMalloc(&q->vox, sizeof(uint32_t) * total) != hipSuccess ||
-		    hipMalloc(&q->ray, 2 * sizeof(float4) * rays) != hipSuccess ||
 		    hipMalloc(&q->chunks, sizeof(TileChunk) * chunks.size()) != hipSuccess)
 			return cleanup(fail(YRTMI_ENOMEM, "plan: records"));
-		q->bytes += total * 12 + rays * 2 * (int64_t)sizeof(float4) +
-		            (int64_t)(sizeof(TileChunk) * chunks.size());
+		q->bytes += total * 12 + (int64_t)(sizeof(TileChunk) * chunks.size());
 		e = hipMemcpyAsync(dOff, off.data(), sizeof(int64_t) * nB,
 		                   hipMemcpyHostToDevice, c->stream);
 		if (e == hipSuccess)
@@ -1269,7 +1290,6 @@ int yrtmi_plan_create(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
 		pl.key = q->key;
 		pl.alpha = q->alpha;
 		pl.vox = q->vox;
-		pl.ray = q->ray;
 		if (e == hipSuccess)
 			e = launchBin(1, a, T, pl, grid, c->stream);
 		std::vector<unsigned int> cur(nB);
@@ -1398,7 +1418,7 @@ int yrtmi_em_subset_planned(yrtmi_ctx* c, const yrtmi_proj* p,
 		return fail(YRTMI_EINVAL, "em_subset: null image");
 	a.x = x;
 	setCorr(a, corr);
-	a.rays = q->ray;  // null when no ray meets the image (no records)
+	a.rays = q->ray;
 	HIP_TRY(launchEmRatio(a, p->lpe, q->v, c->stream));
 	HIP_TRY(launchTileBack(a, q->T, planRead(q), q->v, u, q->persistent, c->stream));
 	return YRTMI_OK;

@@ -42,38 +42,52 @@ __device__ __forceinline__ int tileOf(const DTiles& T, int f, int vx, int vy,
 	return ((f * T.ntz + (vz >> T.lz)) * T.nty + (vy >> T.ly)) * T.ntx + (vx >> T.lx);
 }
 
-// Walks ray r of LOR e (global ray index t = e * R + r) and calls
-// onRecord(tile, alpha, packed entry, segments) for each tile the walk
-// passes through, once the walk has left it (so the piece's number of
-// segments is known).  A plane shift beyond the record's range sets *err
-// (the ray is not binned).
-// With tab, the ray's entry of the ray table is written (rayEntry).
-template <bool TOF, bool S64, class Fn>
-__device__ __forceinline__ void binRay(const KArgs& a, const DTiles& T,
-                                       int64_t e, int r, int* err, float4* tab,
-                                       Fn&& onRecord)
+// Ray table of a LOR set (rayEntry for every (LOR, ray) t = e * R + r):
+// the LOR resolved once (detector pair or histogram bin, LUT, motion,
+// frame), the multi-ray line and the image offset.  Skipped LORs get frame -1.
+__global__ void __launch_bounds__(256) kRayPrep(KArgs a, float4* __restrict__ tab)
 {
-	const Lor l = fetchLor(a.sc, a.L, e, a.im.nt);
-	if (l.frame < 0)
+	const int R = a.pr.numRays;
+	const int64_t total = a.L.n * (int64_t)R;
+	const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+	for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += stride)
 	{
-		if (tab != nullptr)
+		const int64_t e = t / R;
+		const int r = (int)(t - e * R);
+		const Lor l = fetchLor(a.sc, a.L, e, a.im.nt);
+		if (l.frame < 0)
 		{
 			const float3 z = make_float3(0.f, 0.f, 0.f);
-			rayEntry(tab, (uint32_t)(e * a.pr.numRays + r), z, z, 0.f, -1);
+			rayEntry(tab, (uint32_t)t, z, z, 0.f, -1);
+			continue;
 		}
-		return;
+		float3 q1, q2;
+		rayLine(a.pr, l, r, q1, q2);
+		q1 = make_float3(q1.x - a.im.ox, q1.y - a.im.oy, q1.z - a.im.oz);
+		q2 = make_float3(q2.x - a.im.ox, q2.y - a.im.oy, q2.z - a.im.oz);
+		rayEntry(tab, (uint32_t)t, q1, q2, l.tof, l.frame);
 	}
-	float3 q1, q2;
-	rayLine(a.pr, l, r, q1, q2);
-	q1 = make_float3(q1.x - a.im.ox, q1.y - a.im.oy, q1.z - a.im.oz);
-	q2 = make_float3(q2.x - a.im.ox, q2.y - a.im.oy, q2.z - a.im.oz);
-	if (tab != nullptr)
-		rayEntry(tab, (uint32_t)(e * a.pr.numRays + r), q1, q2, l.tof, l.frame);
+}
+
+// Walks ray t of the ray table and calls onRecord(tile, alpha, packed
+// entry, segments) for each tile the walk passes through, once the walk has
+// left it (so the piece's number of segments is known).  A plane shift
+// beyond the record's range sets *err (the ray is not binned).
+template <bool TOF, bool S64, class Fn>
+__device__ __forceinline__ void binRay(const DImage& im, const DTof& tof, const DTiles& T,
+                                       const float4* __restrict__ tab, uint32_t t, int* err,
+                                       Fn&& onRecord)
+{
+	const float4 A = tab[2 * (uint64_t)t], Bq = tab[2 * (uint64_t)t + 1];
+	const int frame = __float_as_int(Bq.w);
+	if (frame < 0)
+		return;
+	const float3 q1 = make_float3(A.x, A.y, A.z), q2 = make_float3(A.w, Bq.x, Bq.y);
 	SidRay s;
-	if (!sidSetup<TOF, S64>(a.im, q1, q2, a.pr.tof, l.tof, s))
+	if (!sidSetup<TOF, S64>(im, q1, q2, tof, Bq.z, s))
 		return;
 	int dx, dy, dz;
-	if (!sidPlaneShift<S64>(a.im, s, dx, dy, dz))
+	if (!sidPlaneShift<S64>(im, s, dx, dy, dz))
 		return;
 	if (abs(dx) > kMaxShift || abs(dy) > kMaxShift || abs(dz) > kMaxShift)
 	{
@@ -83,18 +97,18 @@ __device__ __forceinline__ void binRay(const KArgs& a, const DTiles& T,
 	int last = -1, start = 0, nseg = 0;
 	float pa = 0.f;
 	uint32_t pv = 0;
-	sidWalk<TOF, S64>(a.im, s, a.pr.tof, imageBox(a.im), true, 0, 0, 0,
+	sidWalk<TOF, S64>(im, s, tof, imageBox(im), true, 0, 0, 0,
 	                  [&](int vx, int vy, int vz, int, float a0, float) {
-		                  const int t = tileOf(T, l.frame, vx, vy, vz);
-		                  if (t != last)
+		                  const int tl = tileOf(T, frame, vx, vy, vz);
+		                  if (tl != last)
 		                  {
 			                  if (last >= 0)
 				                  onRecord(last, pa, pv, nseg - start);
 			                  pa = a0;
-			                  pv = packEntry(vx & (T.tx - 1), vy & (T.ty - 1), vz & (T.tz - 1), dx, dy, dz,
-			                                 last < 0);
+			                  pv = packEntry(vx & (T.tx - 1), vy & (T.ty - 1), vz & (T.tz - 1),
+			                                 dx, dy, dz, last < 0);
 			                  start = nseg;
-			                  last = t;
+			                  last = tl;
 		                  }
 		                  nseg++;
 	                  });
@@ -110,63 +124,66 @@ __device__ __forceinline__ int lengthClass(const DPlan& pl, int segs)
 	return c;
 }
 
-// PASS 0: count[b] += records of bin b.  PASS 1: write the records; each
-// workgroup first counts its own records, reserves a range per bin from
-// cursor[b], then walks again to fill it.  LDS: cnt[nB], base[nB].
+// PASS 0: each workgroup counts the records of its rays per bin (LDS),
+// keeps its counts in blockCnt[block][bin] and adds them to count[bin].
+// PASS 1 (same grid, same rays per workgroup): each workgroup reserves its
+// counts in every bin from cursor[bin] and walks its rays again, writing
+// each record into its range (an LDS countdown per bin, so a record beyond
+// the reservation is caught without a global read).
+// LDS: cnt[nB] ints, then (PASS 1) base[nB] 64-bit slots.
 template <bool TOF, bool S64, int PASS>
-__global__ void __launch_bounds__(256) kBin(KArgs a, DTiles T, DPlan pl)
+__global__ void __launch_bounds__(256) kBin(DImage im, DTof tof, int64_t nRays, DTiles T,
+                                            DPlan pl)
 {
 	extern __shared__ int sm[];
 	const int nB = T.nTiles * pl.C;
 	int* cnt = sm;
-	int* base = sm + nB;
-	const int64_t total = a.L.n * (int64_t)pl.R;
+	int64_t* base = reinterpret_cast<int64_t*>(sm + ((nB + 1) & ~1));
 	const int64_t stride = (int64_t)gridDim.x * blockDim.x;
 	const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-	for (int i = threadIdx.x; i < nB; i += blockDim.x)
-		cnt[i] = 0;
-	__syncthreads();
-	for (int64_t t = t0; t < total; t += stride)
-	{
-		const int64_t e = t / pl.R;
-		const int r = (int)(t - e * pl.R);
-		binRay<TOF, S64>(a, T, e, r, pl.err, nullptr, [&](int tile, float, uint32_t, int segs) {
-			atomicAdd(&cnt[tile * pl.C + lengthClass(pl, segs)], 1);
-		});
-	}
-	__syncthreads();
+	unsigned int* myCnt = pl.blockCnt + (int64_t)blockIdx.x * nB;
 	if (PASS == 0)
 	{
 		for (int i = threadIdx.x; i < nB; i += blockDim.x)
+			cnt[i] = 0;
+		__syncthreads();
+		for (int64_t t = t0; t < nRays; t += stride)
+			binRay<TOF, S64>(im, tof, T, pl.ray, (uint32_t)t, pl.err,
+			                 [&](int tile, float, uint32_t, int segs) {
+				                 atomicAdd(&cnt[tile * pl.C + lengthClass(pl, segs)], 1);
+			                 });
+		__syncthreads();
+		for (int i = threadIdx.x; i < nB; i += blockDim.x)
+		{
+			myCnt[i] = (unsigned int)cnt[i];
 			if (cnt[i] != 0)
 				atomicAdd(&pl.count[i], (unsigned long long)cnt[i]);
+		}
 		return;
 	}
 	for (int i = threadIdx.x; i < nB; i += blockDim.x)
 	{
-		const int c = cnt[i];
-		base[i] = (c != 0) ? (int)atomicAdd(&pl.cursor[i], (unsigned int)c) : 0;
-		cnt[i] = 0;
+		const int c = (int)myCnt[i];
+		base[i] = (c != 0) ? pl.tileOff[i] + (int64_t)atomicAdd(&pl.cursor[i], (unsigned int)c)
+		                   : 0;
+		cnt[i] = c;
 	}
 	__syncthreads();
-	for (int64_t t = t0; t < total; t += stride)
-	{
-		const int64_t e = t / pl.R;
-		const int r = (int)(t - e * pl.R);
-		binRay<TOF, S64>(a, T, e, r, pl.err, pl.ray, [&](int tile, float a0, uint32_t pv, int segs) {
-			const int b = tile * pl.C + lengthClass(pl, segs);
-			const int slot = base[b] + atomicAdd(&cnt[b], 1);
-			if ((unsigned long long)slot < pl.count[b])
-			{
-				const int64_t j = pl.tileOff[b] + slot;
-				pl.key[j] = (uint32_t)t;
-				pl.alpha[j] = a0;
-				pl.vox[j] = pv;
-			}
-			else
-				atomicOr(pl.err, 1);
-		});
-	}
+	for (int64_t t = t0; t < nRays; t += stride)
+		binRay<TOF, S64>(im, tof, T, pl.ray, (uint32_t)t, pl.err,
+		                 [&](int tile, float a0, uint32_t pv, int segs) {
+			                 const int b = tile * pl.C + lengthClass(pl, segs);
+			                 const int left = atomicSub(&cnt[b], 1);
+			                 if (left > 0)
+			                 {
+				                 const int64_t j = base[b] + left - 1;
+				                 pl.key[j] = (uint32_t)t;
+				                 pl.alpha[j] = a0;
+				                 pl.vox[j] = pv;
+			                 }
+			                 else
+				                 atomicOr(pl.err, 1);
+		                 });
 }
 
 // One chunk: accumulate its ray pieces in the LDS tile `acc`, then add the
@@ -529,11 +546,23 @@ template <bool TOF, bool S64>
 static hipError_t binT(int pass, const KArgs& a, const DTiles& T,
                        const DPlan& pl, int grid, hipStream_t st)
 {
-	const size_t lds = (size_t)T.nTiles * pl.C * 2 * sizeof(int);
+	const int nB = T.nTiles * pl.C;
+	const int64_t nRays = a.L.n * (int64_t)pl.R;
 	if (pass == 0)
-		kBin<TOF, S64, 0><<<grid, 256, lds, st>>>(a, T, pl);
+		kBin<TOF, S64, 0><<<grid, 256, (size_t)nB * sizeof(int), st>>>(a.im, a.pr.tof, nRays,
+		                                                                T, pl);
 	else
-		kBin<TOF, S64, 1><<<grid, 256, lds, st>>>(a, T, pl);
+		kBin<TOF, S64, 1><<<grid, 256, binLdsBytes(nB), st>>>(a.im, a.pr.tof, nRays, T, pl);
+	return hipGetLastError();
+}
+
+hipError_t launchRayPrep(const KArgs& a, float4* tab, hipStream_t st)
+{
+	const int64_t total = a.L.n * (int64_t)a.pr.numRays;
+	if (total == 0)
+		return hipSuccess;
+	const unsigned g = (unsigned)std::min<int64_t>(std::max<int64_t>((total + 255) / 256, 1), 65536);
+	kRayPrep<<<g, 256, 0, st>>>(a, tab);
 	return hipGetLastError();
 }
 

@@ -20,9 +20,9 @@ __host__ __device__ constexpr int tileThreads(int shape)
 {
 	return shape == 0 ? 256 : (shape == 2 ? 1024 : 512);
 }
-// Binning keeps two int counters per (tile, length class) in LDS (64 KB at
-// most): tiles x classes <= kMaxBins.
-constexpr int kMaxTiles = 8192;
+// Binning keeps a counter (count pass) and a counter plus a 64-bit base
+// (fill pass) per (tile, length class) in LDS; length classes are added
+// while tiles x classes <= kMaxBins / 2.
 constexpr int kMaxBins = 8192;
 constexpr int kMaxClasses = 8;
 
@@ -65,12 +65,13 @@ struct DPlan
 	int32_t C;                  // length classes (1 .. kMaxClasses)
 	int32_t cut[kMaxClasses - 1];  // class k starts at cut[k-1] segments
 	unsigned long long* count;  // [nTiles * C] records per bin (count pass)
+	unsigned int* blockCnt;     // [grid][nTiles * C] records per workgroup and bin
 	unsigned int* cursor;       // [nTiles * C] fill cursor within the bin
 	const int64_t* tileOff;     // [nTiles * C] first record of the bin
 	uint32_t* key;              // lor * R + ray
 	float* alpha;               // alpha where the walk enters the tile
 	uint32_t* vox;              // packed entry voxel (packEntry)
-	float4* ray;                // [2 * n * R] ray table (rayEntry), fill pass
+	const float4* ray;          // [2 * n * R] ray table (rayEntry, kRayPrep)
 	int* err;                   // set when a tile range would overflow
 };
 
@@ -104,7 +105,14 @@ struct DPlanRead
 	const int* vexp;
 };
 
-// pass 0: count; pass 1: fill.  Dynamic LDS 8 * nTiles * C bytes.
+// LDS of the fill pass: an int counter and a 64-bit base per bin.
+inline size_t binLdsBytes(int nB)
+{
+	return (size_t)((nB + 1) & ~1) * sizeof(int) + (size_t)nB * sizeof(int64_t);
+}
+// The plan's ray table for a.L (before the binning passes).
+hipError_t launchRayPrep(const KArgs& a, float4* tab, hipStream_t st);
+// pass 0: count; pass 1: fill (same grid in both).
 hipError_t launchBin(int pass, const KArgs& a, const DTiles& T, const DPlan& pl,
                      int grid, hipStream_t st);
 // y += backprojection of v (per LOR) through the plan's records: one
