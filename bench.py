@@ -204,7 +204,7 @@ def bench_listmode(args, torch, dev, stream):
     for s in range(SS):
         Ls = yrtmi.LorSet.histo_subset(ctx, SS, s)
         if tiled:
-            pl = ctx.plan(psens, im, Ls, args.chunk)
+            pl = ctx.plan(psens, im, Ls, args.chunk, tile_shape=yrtmi.capi.TILE_32x32x8)
             ctx.sensitivity(psens, im, sens, Ls, plan=pl)
             del pl
         else:

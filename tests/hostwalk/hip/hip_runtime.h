@@ -28,6 +28,7 @@ inline float __fmul_rn(float a, float b) { return a * b; }
 inline float __fdiv_rn(float a, float b) { return a / b; }
 inline float __fsqrt_rn(float a) { return std::sqrt(a); }
 inline float __expf(float a) { return std::exp(a); }
+#define __builtin_amdgcn_exp2f(x) std::exp2(x)
 inline void unsafeAtomicAdd(float* p, float v) { *p += v; }
 using std::max;
 using std::min;

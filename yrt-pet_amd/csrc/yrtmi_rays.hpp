@@ -16,6 +16,17 @@
 namespace yrtmi
 {
 
+// x / sigma, correctly rounded, from the correctly rounded reciprocal
+// rs = RN(1/sigma): q = RN(x*rs) is within an ulp and one FMA residual step
+// rounds it to RN(x/sigma) (P. Markstein, IBM J. Res. Dev. 34(1), 1990;
+// finite x away from the overflow and underflow thresholds).
+__device__ __forceinline__ float divBySigma(float x, float sigma, float rs)
+{
+#pragma clang fp contract(off)
+	const float q = x * rs;
+	return fmaf(fmaf(-q, sigma, x), rs, q);
+}
+
 // util::get_alpha (ProjectorUtils.hpp:15-41): slab [r0, r1] entry / exit
 // alphas; a ray flat in this axis and outside the slab gives the empty range
 // amin = 1, amax = 0 (callers decide what an empty range means).
@@ -60,7 +71,12 @@ struct SidRay
 	float axn, ayn, azn;  // alpha of the next plane per axis (FMAX: flat)
 	float xc, yc, zc;     // coordinate of that plane
 	float dnorm;
-	float tA, tB;  // TOF: S64 x_c = a_mid*tA - tB; S32 pc = tB
+	// TOF.  FP32 setup: tB = pc = d/2 + tof_mm, and the weight of a segment
+	// [a0, a1] follows the reference's operation order.  FP64 setup: the
+	// Gaussian's argument is x_c = (a0 + a1) * tA - tB (tA = d / (2 sigma),
+	// tB = pc / sigma, both from FP64) and the weight (a1 - a0) * wn *
+	// exp(-x_c^2 / 2), wn = d * norm.
+	float tA, tB, wn;
 	int dirx, diry, dirz;
 	// index of the first plane per axis, counted from the image face the
 	// ray enters through (x_start + k * dir * v)
@@ -87,6 +103,7 @@ __device__ __forceinline__ bool sidSetup(const DImage& P, float3 p1, float3 p2,
 	r.azn = FMAX;
 	r.tA = 0.f;
 	r.tB = 0.f;
+	r.wn = 0.f;
 	r.kx = r.ky = r.kz = 0;
 	r.x1d = r.y1d = r.z1d = r.ddx = r.ddy = r.ddz = 0.0;
 	if constexpr (S64)
@@ -158,7 +175,8 @@ __device__ __forceinline__ bool sidSetup(const DImage& P, float3 p1, float3 p2,
 				amax = fmin(amax, 0.5 + (tmm + (double)tof.trunc) / dnormd);
 			}
 			r.tB = (float)((0.5 * dnormd + tmm) / (double)tof.sigma);
-			r.tA = (float)(dnormd / (double)tof.sigma);
+			r.tA = (float)(0.5 * dnormd / (double)tof.sigma);
+			r.wn = (float)(dnormd * (double)tof.norm);
 		}
 		if ((invx >= 0.0 && x1d > hx) || (invx < 0.0 && x1d < -hx) ||
 		    (invy >= 0.0 && y1d > hy) || (invy < 0.0 && y1d < -hy) ||
@@ -473,17 +491,6 @@ __device__ __forceinline__ WalkBox imageBox(const DImage& P)
 	return WalkBox{0, P.nx, 0, P.ny, 0, P.nz, P.nx, P.nxy, 0};
 }
 
-// x / sigma, correctly rounded, from the correctly rounded reciprocal
-// rs = RN(1/sigma): q = RN(x*rs) is within an ulp and one FMA residual step
-// rounds it to RN(x/sigma) (P. Markstein, IBM J. Res. Dev. 34(1), 1990;
-// finite x away from the overflow and underflow thresholds).
-__device__ __forceinline__ float divBySigma(float x, float sigma, float rs)
-{
-#pragma clang fp contract(off)
-	const float q = x * rs;
-	return fmaf(fmaf(-q, sigma, x), rs, q);
-}
-
 // FP32 walk (ProjectorSiddon.cpp:622-766) from r.a_cur to r.amax.  With
 // `first` the walk starts at the ray's entry and its first voxel is the one
 // holding the midpoint of the first nonempty segment; otherwise it starts in
@@ -525,18 +532,26 @@ __device__ __forceinline__ void sidWalk(const DImage& P, const SidRay& r,
 	};
 	float acd = 0.f;  // a_cur * dnorm, carried from the previous segment's end
 	auto emit = [&](int off) {
-		float w = (a_next - a_cur) * dnorm;
+		float w;
 		if (TOF)
 		{
-			float xcn;
-			const float and_ = a_next * dnorm;
 			if constexpr (S64)
-				xcn = fmaf(0.5f * (a_cur + a_next), r.tA, -r.tB);
-			else  // TimeOfFlightHelper::getWeight (TimeOfFlight.hpp:175-185)
-				xcn = divBySigma(0.5f * (acd + and_) - r.tB, tof.sigma, tof.rsigma);
-			acd = and_;
-			w *= gaussW(xcn, tof.norm);
+			{
+				// exp(-x^2/2) = 2^(-x^2 log2(e) / 2)
+				const float xc = fmaf(a_cur + a_next, r.tA, -r.tB);
+				w = (a_next - a_cur) * r.wn * __builtin_amdgcn_exp2f(-0.72134752f * xc * xc);
+			}
+			else
+			{
+				// TimeOfFlightHelper::getWeight (TimeOfFlight.hpp:175-185)
+				const float and_ = a_next * dnorm;
+				const float xc = divBySigma(0.5f * (acd + and_) - r.tB, tof.sigma, tof.rsigma);
+				acd = and_;
+				w = (a_next - a_cur) * dnorm * gaussW(xc, tof.norm);
+			}
 		}
+		else
+			w = (a_next - a_cur) * dnorm;
 		fn(vx, vy, vz, off + vx, a_cur, w);
 		a_cur = a_next;
 	};
