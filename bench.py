@@ -443,7 +443,10 @@ def bench_frames(args, torch, dev, stream):
     for s in range(S):
         fr = torch.cat([torch.full((per[f][s].n,), f, dtype=torch.int32, device=dev)
                         for f in range(F)])
-        bsub.append(yrtmi.LorSet.pairs(cat("d1", s), cat("d2", s), tof=cat("tof", s), frame=fr))
+        Lb = yrtmi.LorSet.pairs(cat("d1", s), cat("d2", s), tof=cat("tof", s), frame=fr)
+        if args.order == "lor":
+            Lb = Lb.permuted(ctx, ctx.lor_order(imF, Lb))  # frame-major locality order
+        bsub.append(Lb)
     tune = ctx.autotune(proj, imF, bsub[0], xF)
     log(args, f"autotune: {tune}")
     bplans = [ctx.plan(proj, imF, L, tile_shape=tune["tile_shape"],

@@ -406,7 +406,8 @@ typedef struct
 	float ratio_ms;          /* EM ratio kernel, best lanes per event */
 	float back_ms;           /* planned backprojection, best settings */
 	float ratio_ms_lpe[4];   /* lanes per event 1, 2, 4, 8 */
-	float back_ms_shape[4][2]; /* [tile shape][one per chunk, persistent] */
+	float back_ms_shape[4][2]; /* [tile shape][one per chunk, persistent];
+	                            * -1: the shape does not fit the image */
 } yrtmi_tuning;
 int yrtmi_autotune(yrtmi_ctx* ctx, yrtmi_proj* p, const yrtmi_image* im,
                    const yrtmi_lors* lors, const float* x, yrtmi_tuning* best);

@@ -1797,11 +1797,15 @@ int yrtmi_autotune(yrtmi_ctx* c, yrtmi_proj* p, const yrtmi_image* im,
 	{
 		yrtmi_plan_options o{0, shape, 0};
 		yrtmi_plan* q = nullptr;
+		best->back_ms_shape[shape][0] = best->back_ms_shape[shape][1] = -1.f;
 		r = yrtmi_plan_create(c, p, im, l, &o, &q);
+		if (r == YRTMI_EINVAL)  // this shape does not fit the image (tile count)
+			continue;
 		if (r)
 			return done(r);
-		// resident workgroups: LDS per CU (160 KB) over the tile's LDS
-		const int perCu = shape == 1 ? 8 : (shape == 2 ? 1 : 2);
+		// resident workgroups: 160 KB of LDS per CU over the tile's LDS
+		// (16^3: 32 KB; 32x32x16: 128 KB; 32^3: 128 KB; 32x32x8: 64 KB)
+		const int perCu = shape == 0 ? 5 : (shape == 3 ? 2 : 1);
 		for (int mode = 0; mode < 2; mode++)
 		{
 			const int pers = mode ? perCu * cus : 0;
@@ -1824,6 +1828,8 @@ int yrtmi_autotune(yrtmi_ctx* c, yrtmi_proj* p, const yrtmi_image* im,
 		}
 		yrtmi_plan_destroy(q);
 	}
+	if (best->back_ms >= 1e30f)
+		return done(fail(YRTMI_EINVAL, "autotune: no tile shape fits the image"));
 	return done(YRTMI_OK);
 }
 
