@@ -81,8 +81,11 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=("listmode", "frames", "histogram"),
+    ap.add_argument("--workload", choices=("listmode", "frames", "histogram", "dd"),
                     default="listmode")
+    ap.add_argument("--dd-min", type=float, default=1e6, help="dd workload: min counts/frame")
+    ap.add_argument("--dd-max", type=float, default=40e6, help="dd workload: max counts/frame")
+    ap.add_argument("--dd-subsets", type=int, default=12)
     ap.add_argument("--events", type=int, default=100_000_000)
     ap.add_argument("--subsets", type=int, default=10)
     ap.add_argument("--seed", type=int, default=2024)
@@ -134,7 +137,7 @@ def subset_major(torch, ev, n, S, dev):
     """Event i belongs to subset i mod S (BinIteratorChronologicalInterleaved);
     permute once so each subset is contiguous."""
     perm = torch.cat([torch.arange(s, n, S, device=dev) for s in range(S)])
-    out = {k: ev[k][perm].contiguous() for k in ("d1", "d2", "tof")}
+    out = {k: ev[k][perm].contiguous() for k in ("d1", "d2", "tof") if k in ev}
     bounds = [0]
     for s in range(S):
         bounds.append(bounds[-1] + len(range(s, n, S)))
@@ -163,6 +166,8 @@ def main():
             out = bench_frames(args, torch, dev, stream)
         elif args.workload == "histogram":
             out = bench_histogram(args, torch, dev, stream)
+        elif args.workload == "dd":
+            out = bench_dd(args, torch, dev, stream)
         else:
             out = bench_listmode(args, torch, dev, stream)
     print(json.dumps(out))
@@ -486,6 +491,117 @@ def bench_frames(args, torch, dev, stream):
     }
 
 
+def bench_dd(args, torch, dev, stream):
+    """Batched distance-driven OSEM with projection-space PSF over dynamic
+    frames (the 48-ring configuration), plain per-frame vs one launch."""
+    import yrtmi
+    from yrtmi import synth
+
+    # 48 rings x 384 crystals, R 350 mm, 4 x 4 x 20 mm crystals
+    sc = yrtmi.Scanner("cfg3", 192.0, 4.0, 4.0, 20.0, 350.0, 384, 48)
+    im1 = yrtmi.ImageParams(128, 128, 96, 2.5, 2.5, 2.5)
+    F, S = 16, args.dd_subsets
+    rng = np.random.default_rng(args.seed + 3)
+    counts = [int(c) for c in np.exp(rng.uniform(math.log(args.dd_min), math.log(args.dd_max), F))]
+    # activity: water cylinder (1.0) and 5 spheres whose uptake follows a
+    # one-tissue compartment curve 4 (1 - exp(-k2 t)) / (1 - exp(-k2 T)),
+    # k2 ~ U[0.01, 0.3] / s, frames at t = 60 s (f + 1)
+    spheres = synth.random_spheres_in_cylinder(rng, 120.0, 160.0, [6.0, 9.0, 12.0, 15.0, 18.0], 4.0)
+    k2 = rng.uniform(0.01, 0.3, len(spheres))
+    T = 60.0 * F
+    t0 = time.time()
+    evs = []
+    for f in range(F):
+        t = 60.0 * (f + 1)
+        sph = [(cx, cy, cz, r, 1.0 + 3.0 * (1 - math.exp(-k * t)) / (1 - math.exp(-k * T)))
+               for (cx, cy, cz, r, _), k in zip(spheres, k2)]
+        act = synth.cylinder_spheres(im1, 120.0, 160.0, 1.0, sph, supersample=1)
+        evs.append(synth.sample_events(sc, im1, act, counts[f], seed=args.seed + 11 * f,
+                                       device=dev))
+    n = sum(counts)
+    log(args, f"DD: {F} frames, {n} events ({min(counts)}..{max(counts)}) in {time.time() - t0:.1f}s")
+    ctx = yrtmi.Context(sc, stream=stream)
+    pp = yrtmi.ProjectorParams(sc, projectorType="DD")
+    k, s_step, ks = synth.gaussian_psf_row(2.0, k_spacing=0.25, num_s=1, s_step=1000.0)
+    pp.psf_kernels, pp.psf_s_step, pp.psf_k_spacing = k, s_step, ks
+    proj = ctx.projector(pp)
+    proj.set_accumulation(True)
+    sens = torch.full(im1.shape, 1.0, device=dev)
+    per = []
+    for f in range(F):
+        e, b = subset_major(torch, evs[f], counts[f], S, dev)
+        per.append([yrtmi.LorSet.pairs(e["d1"][b[s]:b[s + 1]], e["d2"][b[s]:b[s + 1]])
+                    for s in range(S)])
+    del evs
+    imF = yrtmi.ImageParams(im1.nx, im1.ny, im1.nz, im1.vx, im1.vy, im1.vz, nt=F)
+    xF = torch.full(imF.shape, 0.125, device=dev)
+    uF = torch.zeros(imF.shape, device=dev)
+    x1 = [xF[f:f + 1] for f in range(F)]
+    u1 = [uF[f:f + 1] for f in range(F)]
+    # footprint voxels (the DD work unit), exactly
+    fp = sum(ctx.count_segments(proj, im1, per[f][s]) for f in range(F) for s in range(S))
+
+    def plain():
+        for f in range(F):
+            for s in range(S):
+                ctx.fill(u1[f], 0.0)
+                ctx.em_subset(proj, im1, x1[f], u1[f], per[f][s])
+                ctx.em_update(x1[f], u1[f], sens, im1.nvox)
+
+    cat = lambda key, s: torch.cat([getattr(per[f][s], key) for f in range(F)]).contiguous()
+    bsub = []
+    for s in range(S):
+        fr = torch.cat([torch.full((per[f][s].n,), f, dtype=torch.int32, device=dev)
+                        for f in range(F)])
+        Lb = yrtmi.LorSet.pairs(cat("d1", s), cat("d2", s), frame=fr)
+        if args.order == "lor":
+            Lb = Lb.permuted(ctx, ctx.lor_order(imF, Lb))
+        bsub.append(Lb)
+
+    def batched():
+        for s in range(S):
+            ctx.fill(uF, 0.0)
+            ctx.em_subset(proj, imF, xF, uF, bsub[s])
+            ctx.em_update(xF, uF, sens, im1.nvox)
+
+    def timed(fn, steps):
+        fn()
+        ctx.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        ctx.synchronize()
+        return (time.perf_counter() - t) / steps
+
+    res = {"plain_s": timed(plain, args.steps)}
+    log(args, f"DD plain (per frame, fused EM): {res['plain_s']:.3f} s/iter")
+    batched()
+    ctx.synchronize()
+    with ctx.graph() as g:
+        batched()
+    res["batched_s"] = timed(g.launch, args.steps)
+    log(args, f"DD batched (frames in one launch, locality order, graph): "
+              f"{res['batched_s']:.3f} s/iter")
+    return {
+        "metric": "fwd+back event-projections/s, batched DD+PSF dynamic frames vs plain per-frame",
+        "value": 2.0 * n / res["batched_s"],
+        "unit": "event-projections/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": 1,
+        "ms_per_step": res["batched_s"] * 1e3,
+        "s_per_osem_iter": res["batched_s"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded dynamic phantom; each count one LOR)",
+        "config": {"workload": f"DD + projection PSF (2 mm FWHM), 48x384 scanner R350, "
+                               f"128x128x96 @2.5mm, {F} frames log-uniform "
+                               f"[{args.dd_min:g}, {args.dd_max:g}] counts, {S} subsets",
+                   "frame_counts": counts, "events": n, "footprint_voxels": fp},
+        "plain_value": 2.0 * n / res["plain_s"],
+        "speedup_vs_plain": res["plain_s"] / res["batched_s"],
+        "footprint_voxels_per_s": 2.0 * fp / res["batched_s"],
+        "seconds": res,
+    }
+
+
 def bench_histogram(args, torch, dev, stream):
     """Histogram OSEM, full size, GPU (C++ host layer) and CPU oracle."""
     import oracle as O
@@ -573,7 +689,8 @@ def bench_histogram(args, torch, dev, stream):
     cpu = {"value": L.n * iters / c_rec, "unit": "bins/s", "cores": nthreads, "kind": "port",
            "sample": "the full reconstruction (5 iterations x 8 subsets) by the oracle's FP32 "
                      "restatement of the reference CPU OSEM",
-           "seconds": c_rec, "s_per_osem_iter": c_rec / iters, "sensitivity_s": c_sens}
+           "seconds": c_rec, "s_per_osem_iter": c_rec / iters, "sensitivity_s": c_sens,
+           **host_cpu()}
     out = _histo_result(L.n, counts, iters, S, t_rec, t_sens, cpu)
     out["parity_rel_l2_vs_cpu"] = r
     return out
@@ -597,6 +714,19 @@ def _histo_result(nbins, counts, iters, S, t_rec, t_sens, cpu):
         "sensitivity_s": t_sens,
         "cpu_baseline": cpu,
     }
+
+
+def host_cpu():
+    """nproc and the CPU model of the machine the CPU baseline runs on."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model}
 
 
 def cpu_baseline(args, sc, im, pp, d1, d2, tof, bounds, x):
@@ -631,7 +761,7 @@ def cpu_baseline(args, sc, im, pp, d1, d2, tof, bounds, x):
     return {"value": rate, "unit": "event-projections/s", "cores": nthreads, "kind": "port",
             "sample": f"EM subset (forward+backprojection) of the first {k} events of subset 0, "
                       f"oracle FP32 restatement, {nthreads} std::threads",
-            "seconds": dt}
+            "seconds": dt, **host_cpu()}
 
 
 if __name__ == "__main__":
