@@ -276,8 +276,8 @@ def test_plan_misuse_rejected(gpu):
     with pytest.raises(yrtmi.capi.YrtmiError, match="TOF off"):
         ctx.sensitivity(pt, im, img, Lt, plan=plan_t)
     pdd = ctx.projector(yrtmi.ProjectorParams(sc, projectorType="DD"))
-    with pytest.raises(yrtmi.capi.YrtmiError, match="Siddon"):
-        ctx.plan(pdd, im, L)
+    with pytest.raises(yrtmi.capi.YrtmiError, match="projector"):  # Siddon plan, DD projector
+        ctx.back(pdd, im, img, L, vals, plan=plan)
     # more tiles (frames included) than any device's LDS holds bin counters for
     big = yrtmi.ImageParams(8, 8, 8, 1.0, 1.0, 1.0, nt=200000)
     with pytest.raises(yrtmi.capi.YrtmiError, match="image tiles"):

@@ -1099,8 +1099,7 @@ int yrtmi_plan_create(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
 	r = notCapturing(c, "plan_create");
 	if (r)
 		return r;
-	if (p->d.type != YRTMI_PROJ_SIDDON)
-		return fail(YRTMI_EINVAL, "plan: tile plans need the Siddon projector");
+	const bool dd = p->d.type == YRTMI_PROJ_DD;
 	if (im->nx > 1024 || im->ny > 1024 || im->nz > 1024)
 		return fail(YRTMI_EINVAL, "plan: image dimensions above 1024");
 	const int R = p->d.numRays;
@@ -1189,13 +1188,24 @@ int yrtmi_plan_create(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
 	if (l->n == 0)
 		return cleanup(YRTMI_OK);
 	// length classes: up to 8, as long as tiles x classes stays within
-	// 4096 bins (48 KB of fill-pass LDS) and the LDS limit; cut at fractions
-	// of the longest piece a tile can hold (tx + ty + tz segments)
+	// 4096 bins (48 KB of fill-pass LDS) and the LDS limit.  Siddon: cut at
+	// fractions of the longest piece a tile can hold (tx + ty + tz segments);
+	// DD: at multiples of the slices of a tile slab (a piece of a footprint
+	// a few voxels wide and high holds a few voxels per slice)
 	DPlan pl{};
 	pl.R = R;
 	pl.C = 1;
 	while (pl.C < kMaxClasses && nT * pl.C * 2 <= std::min<int64_t>(binCap, kMaxBins))
 		pl.C *= 2;
+	if (dd)
+	{
+		const int L = std::max(T.tx, T.ty);
+		static const int f8[7] = {1, 2, 4, 8, 12, 16, 24}, f4[3] = {2, 6, 16}, f2[1] = {8};
+		const int* f = pl.C == 8 ? f8 : (pl.C == 4 ? f4 : f2);
+		for (int k = 0; k < pl.C - 1; k++)
+			pl.cut[k] = std::max(1, L * f[k] / 4);
+	}
+	else
 	{
 		const int L = T.tx + T.ty + T.tz;
 		static const int f8[7] = {3, 6, 9, 12, 16, 21, 28}, f4[3] = {6, 12, 21}, f2[1] = {12};
@@ -1212,9 +1222,11 @@ int yrtmi_plan_create(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
 	    hipMalloc(&dOff, sizeof(int64_t) * nB) != hipSuccess ||
 	    hipMalloc(&dErr, sizeof(int)) != hipSuccess)
 		return cleanup(fail(YRTMI_ENOMEM, "plan: build scratch"));
-	if (hipMalloc(&q->ray, 2 * sizeof(float4) * rays) != hipSuccess)
+	// ray table: two float4 per Siddon ray, four per DD LOR
+	const int64_t tabVec = dd ? 4 : 2;
+	if (hipMalloc(&q->ray, tabVec * sizeof(float4) * rays) != hipSuccess)
 		return cleanup(fail(YRTMI_ENOMEM, "plan: ray table"));
-	q->bytes += rays * 2 * (int64_t)sizeof(float4);
+	q->bytes += rays * tabVec * (int64_t)sizeof(float4);
 	hipError_t e = hipMemsetAsync(dCount, 0, sizeof(unsigned long long) * nB, c->stream);
 	if (e == hipSuccess)
 		e = hipMemsetAsync(dCursor, 0, sizeof(unsigned int) * nB, c->stream);
@@ -1276,12 +1288,13 @@ int yrtmi_plan_create(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
 	q->nChunks = (int64_t)chunks.size();
 	if (total > 0)
 	{
+		// DD records are the key alone
 		if (hipMalloc(&q->key, sizeof(uint32_t) * total) != hipSuccess ||
-		    hipMalloc(&q->alpha, sizeof(float) * total) != hipSuccess ||
-		    hipMalloc(&q->vox, sizeof(uint32_t) * total) != hipSuccess ||
+		    (!dd && hipMalloc(&q->alpha, sizeof(float) * total) != hipSuccess) ||
+		    (!dd && hipMalloc(&q->vox, sizeof(uint32_t) * total) != hipSuccess) ||
 		    hipMalloc(&q->chunks, sizeof(TileChunk) * chunks.size()) != hipSuccess)
 			return cleanup(fail(YRTMI_ENOMEM, "plan: records"));
-		q->bytes += total * 12 + (int64_t)(sizeof(TileChunk) * chunks.size());
+		q->bytes += total * (dd ? 4 : 12) + (int64_t)(sizeof(TileChunk) * chunks.size());
 		e = hipMemcpyAsync(dOff, off.data(), sizeof(int64_t) * nB,
 		                   hipMemcpyHostToDevice, c->stream);
 		if (e == hipSuccess)
@@ -1388,8 +1401,6 @@ int yrtmi_em_ratio(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
 	int r = prepare(c, p, im, l, a);
 	if (r)
 		return r;
-	if (p->d.type != YRTMI_PROJ_SIDDON)
-		return fail(YRTMI_EINVAL, "em_ratio: Siddon projector only");
 	if (l->n == 0)
 		return YRTMI_OK;
 	if (x == nullptr || v == nullptr)

@@ -639,15 +639,31 @@ __device__ __forceinline__ float siddonRay(const DImage& P,
 	return 0.f;
 }
 
-// Distance-driven ray (ProjectorDD::dd_project_ref, ProjectorDD.cpp:260-578).
-// d1/d2: endpoints with the image offset subtracted.  Footprint arithmetic
-// keeps the reference's FP32 order without contraction: the voxel range of
-// the PSF-extended footprint is a rounding decision (rintf) whose outcome
-// must not change with FMA fusion.
-template <int MODE, bool TOF, bool S64>
-__device__ float ddRay(const DImage& P, float* __restrict__ img, float3 d1,
-                       float3 d2, float3 n1, float3 n2, float val,
-                       const DProj& pr, float tofv, uint32_t& cnt)
+// Distance-driven ray (ProjectorDD::dd_project_ref, ProjectorDD.cpp:260-578),
+// split into its per-LOR set-up (ddSetup) and the sweep over the slices of
+// the major axis (ddVisit), so that the sweep can be restricted to a box of
+// voxels (a tile) with the same arithmetic.  d1/d2: endpoints with the image
+// offset subtracted.  Footprint arithmetic keeps the reference's FP32 order
+// without contraction: the voxel range of the PSF-extended footprint is a
+// rounding decision (rintf) whose outcome must not change with FMA fusion.
+struct DDRay
+{
+	bool usePsf;
+	const float* pk;  // PSF taps (flipped for backprojection)
+	float ext, d_norm, d1_i, d2_i;
+	int xy_i_0, xy_i_1, nyx, strideXY, strideYX;
+	bool flag_y;  // major axis y (else x)
+	float lxy, lyx, dxy, dyx;
+	float d1_xy_lo, d1_xy_hi, d2_xy_lo, d2_xy_hi;
+	float d1_yx_lo, d1_yx_hi, d2_yx_lo, d2_yx_hi;
+	float d1zloI, d1zloZ, d1zhiI, d1zhiZ, d2zloI, d2zloZ, d2zhiI, d2zhiZ;
+	float dxy_cos_theta, yx_off, inv_dyx, z_off, inv_dz, half_dyx, hvz, pc;
+};
+
+template <bool TOF, bool S64>
+__device__ __forceinline__ void ddSetup(const DImage& P, float3 d1, float3 d2, float3 n1,
+                                        float3 n2, const DProj& pr, float tofv,
+                                        bool flipPsf, DDRay& R)
 {
 #pragma clang fp contract(off)
 	const float d12x = d1.x - d2.x, d12y = d1.y - d2.y, d12z = d1.z - d2.z;
@@ -672,8 +688,7 @@ __device__ float ddRay(const DImage& P, float* __restrict__ img, float3 d1,
 			if (s > pr.psf.numS - 1)
 				s = pr.psf.numS - 1;
 		}
-		pk = (MODE == MODE_FWD || MODE == MODE_COUNT ? pr.psf.k : pr.psf.kflip) +
-		     s * pr.psf.K;
+		pk = (flipPsf ? pr.psf.kflip : pr.psf.k) + s * pr.psf.K;
 		ext = pr.psf.halfWidth;
 	}
 	const float dx = P.vx, dy = P.vy, dz = P.vz;
@@ -829,82 +844,174 @@ __device__ float ddRay(const DImage& P, float* __restrict__ img, float3 d1,
 	float pc = 0.f;
 	if (TOF)
 		pc = (float)(0.5 * (double)d_norm + (double)tofMm(tofv));
-	float acc = 0.f;
-	for (int xyi = xy_i_0; xyi <= xy_i_1; xyi++)
+	R.usePsf = usePsf;
+	R.pk = pk;
+	R.ext = ext;
+	R.d_norm = d_norm;
+	R.d1_i = d1_i;
+	R.d2_i = d2_i;
+	R.xy_i_0 = xy_i_0;
+	R.xy_i_1 = xy_i_1;
+	R.nyx = nyx;
+	R.strideXY = strideXY;
+	R.strideYX = strideYX;
+	R.flag_y = flag_y;
+	R.lxy = lxy;
+	R.lyx = lyx;
+	R.dxy = dxy;
+	R.dyx = dyx;
+	R.d1_xy_lo = d1_xy_lo;
+	R.d1_xy_hi = d1_xy_hi;
+	R.d2_xy_lo = d2_xy_lo;
+	R.d2_xy_hi = d2_xy_hi;
+	R.d1_yx_lo = d1_yx_lo;
+	R.d1_yx_hi = d1_yx_hi;
+	R.d2_yx_lo = d2_yx_lo;
+	R.d2_yx_hi = d2_yx_hi;
+	R.d1zloI = d1zloI;
+	R.d1zloZ = d1zloZ;
+	R.d1zhiI = d1zhiI;
+	R.d1zhiZ = d1zhiZ;
+	R.d2zloI = d2zloI;
+	R.d2zloZ = d2zloZ;
+	R.d2zhiI = d2zhiI;
+	R.d2zhiZ = d2zhiZ;
+	R.dxy_cos_theta = dxy_cos_theta;
+	R.yx_off = yx_off;
+	R.inv_dyx = inv_dyx;
+	R.z_off = z_off;
+	R.inv_dz = inv_dz;
+	R.half_dyx = half_dyx;
+	R.hvz = hvz;
+	R.pc = pc;
+}
+
+// Footprint range of slice xyi: minor-axis voxels [yx0, yx1], z voxels
+// [zi0, zi1] (clamped to the image), and the quantities the voxel weights
+// need.
+struct DDSlice
+{
+	float r0, r1, r0ov, r1ov, widthFrac_yx, z0r, z1r, widthFrac_z, tw;
+	int yx0, yx1, zi0, zi1;
+};
+
+template <bool TOF>
+__device__ __forceinline__ void ddSlice(const DImage& P, const DDRay& R, const DProj& pr,
+                                        int xyi, DDSlice& q)
+{
+#pragma clang fp contract(off)
+	const float pix_xy = -0.5f * R.lxy + ((float)xyi + 0.5f) * R.dxy;
+	const float a_xy_lo = (pix_xy - R.d1_xy_lo) / (R.d2_xy_hi - R.d1_xy_lo);
+	const float a_xy_hi = (pix_xy - R.d1_xy_hi) / (R.d2_xy_lo - R.d1_xy_hi);
+	const float a_z_lo = (pix_xy - R.d1zloI) / (R.d2zloI - R.d1zloI);
+	const float a_z_hi = (pix_xy - R.d1zhiI) / (R.d2zhiI - R.d1zhiI);
+	float r0 = R.d1_yx_lo + a_xy_lo * (R.d2_yx_hi - R.d1_yx_lo);
+	float r1 = R.d1_yx_hi + a_xy_hi * (R.d2_yx_lo - R.d1_yx_hi);
+	if (r0 > r1)
 	{
-		const float pix_xy = -0.5f * lxy + ((float)xyi + 0.5f) * dxy;
-		const float a_xy_lo = (pix_xy - d1_xy_lo) / (d2_xy_hi - d1_xy_lo);
-		const float a_xy_hi = (pix_xy - d1_xy_hi) / (d2_xy_lo - d1_xy_hi);
-		const float a_z_lo = (pix_xy - d1zloI) / (d2zloI - d1zloI);
-		const float a_z_hi = (pix_xy - d1zhiI) / (d2zhiI - d1zhiI);
-		float r0 = d1_yx_lo + a_xy_lo * (d2_yx_hi - d1_yx_lo);
-		float r1 = d1_yx_hi + a_xy_hi * (d2_yx_lo - d1_yx_hi);
-		if (r0 > r1)
-		{
-			const float t = r0;
-			r0 = r1;
-			r1 = t;
-		}
-		const float widthFrac_yx = r1 - r0;
-		const float r0ov = r0, r1ov = r1;
-		r0 -= ext;
-		r1 += ext;
-		const int yx0 = max(0, (int)rintf(r0 * inv_dyx + yx_off));
-		const int yx1 = min(nyx - 1, (int)rintf(r1 * inv_dyx + yx_off));
-		float z0r = d1zloZ + a_z_lo * (d2zloZ - d1zloZ);
-		float z1r = d1zhiZ + a_z_hi * (d2zhiZ - d1zhiZ);
-		if (z0r > z1r)
-		{
-			const float t = z0r;
-			z0r = z1r;
-			z1r = t;
-		}
-		const float widthFrac_z = z1r - z0r;
-		const int zi0 = max(0, (int)rintf(z0r * inv_dz + z_off));
-		const int zi1 = min(P.nz - 1, (int)rintf(z1r * inv_dz + z_off));
-		float tw = 1.f;
-		if (TOF)
-		{
-			// getWeight at the slab's [a_lo, a_hi] (ProjectorDD.cpp:507-515)
-			const float alo = (pix_xy - d1_i - 0.5f * dxy) / (d2_i - d1_i);
-			const float ahi = (pix_xy - d1_i + 0.5f * dxy) / (d2_i - d1_i);
-			const float xcn =
-			    divBySigma(0.5f * (alo * d_norm + ahi * d_norm) - pc, pr.tof.sigma, pr.tof.rsigma);
-			tw = gaussW(xcn, pr.tof.norm);
-		}
+		const float t = r0;
+		r0 = r1;
+		r1 = t;
+	}
+	q.widthFrac_yx = r1 - r0;
+	q.r0ov = r0;
+	q.r1ov = r1;
+	r0 -= R.ext;
+	r1 += R.ext;
+	q.r0 = r0;
+	q.r1 = r1;
+	q.yx0 = max(0, (int)rintf(r0 * R.inv_dyx + R.yx_off));
+	q.yx1 = min(R.nyx - 1, (int)rintf(r1 * R.inv_dyx + R.yx_off));
+	float z0r = R.d1zloZ + a_z_lo * (R.d2zloZ - R.d1zloZ);
+	float z1r = R.d1zhiZ + a_z_hi * (R.d2zhiZ - R.d1zhiZ);
+	if (z0r > z1r)
+	{
+		const float t = z0r;
+		z0r = z1r;
+		z1r = t;
+	}
+	q.z0r = z0r;
+	q.z1r = z1r;
+	q.widthFrac_z = z1r - z0r;
+	q.zi0 = max(0, (int)rintf(z0r * R.inv_dz + R.z_off));
+	q.zi1 = min(P.nz - 1, (int)rintf(z1r * R.inv_dz + R.z_off));
+	q.tw = 1.f;
+	if (TOF)
+	{
+		// getWeight at the slab's [a_lo, a_hi] (ProjectorDD.cpp:507-515)
+		const float alo = (pix_xy - R.d1_i - 0.5f * R.dxy) / (R.d2_i - R.d1_i);
+		const float ahi = (pix_xy - R.d1_i + 0.5f * R.dxy) / (R.d2_i - R.d1_i);
+		const float xcn =
+		    divBySigma(0.5f * (alo * R.d_norm + ahi * R.d_norm) - R.pc, pr.tof.sigma,
+		               pr.tof.rsigma);
+		q.tw = gaussW(xcn, pr.tof.norm);
+	}
+}
+
+// The footprint voxels of slices [xlo, xhi] of the major axis, minor-axis
+// index in [ylo, yhi] and z in [zlo, zhi] (each also clamped to the ray's own
+// range): fn(io, xyi, yxi, zi, w), io the voxel's index in the frame.  The
+// whole image (ranges covering it) is ProjectorDD's sweep.
+template <bool TOF, class Fn>
+__device__ __forceinline__ void ddVisit(const DImage& P, const DDRay& R, const DProj& pr,
+                                        int xlo, int xhi, int ylo, int yhi, int zlo, int zhi,
+                                        Fn&& fn)
+{
+#pragma clang fp contract(off)
+	const int x0 = max(R.xy_i_0, xlo), x1 = min(R.xy_i_1, xhi);
+	for (int xyi = x0; xyi <= x1; xyi++)
+	{
+		DDSlice q;
+		ddSlice<TOF>(P, R, pr, xyi, q);
+		const int yx0 = max(q.yx0, ylo), yx1 = min(q.yx1, yhi);
+		const int zi0 = max(q.zi0, zlo), zi1 = min(q.zi1, zhi);
 		for (int yxi = yx0; yxi <= yx1; yxi++)
 		{
-			const float pix_yx = -0.5f * lyx + ((float)yxi + 0.5f) * dyx;
-			const float p0 = pix_yx - half_dyx;
-			const float p1 = pix_yx + half_dyx;
-			if (!(r1 >= p0 && r0 < p1))
+			const float pix_yx = -0.5f * R.lyx + ((float)yxi + 0.5f) * R.dyx;
+			const float p0 = pix_yx - R.half_dyx;
+			const float p1 = pix_yx + R.half_dyx;
+			if (!(q.r1 >= p0 && q.r0 < p1))
 				continue;
 			const float wxy =
-			    usePsf ? psfWeight(pr.psf, pk, p0 - r1ov, p1 - r0ov)
-			           : (fminf(p1, r1ov) - fmaxf(p0, r0ov));
-			const float wxy_s = wxy / widthFrac_yx;
+			    R.usePsf ? psfWeight(pr.psf, R.pk, p0 - q.r1ov, p1 - q.r0ov)
+			             : (fminf(p1, q.r1ov) - fmaxf(p0, q.r0ov));
+			const float wxy_s = wxy / q.widthFrac_yx;
 			for (int zi = zi0; zi <= zi1; zi++)
 			{
 				const float pix_z = -0.5f * P.lz + ((float)zi + 0.5f) * P.vz;
-				const float zp0 = pix_z - hvz;
-				const float zp1 = pix_z + hvz;
-				if (!(z1r >= zp0 && z0r < zp1))
+				const float zp0 = pix_z - R.hvz;
+				const float zp1 = pix_z + R.hvz;
+				if (!(q.z1r >= zp0 && q.z0r < zp1))
 					continue;
-				const float wz = fminf(zp1, z1r) - fmaxf(zp0, z0r);
-				const float wz_s = wz / widthFrac_z;
-				const int io = zi * P.nxy + xyi * strideXY + yxi * strideYX;
-				float w = wxy_s * wz_s * dxy_cos_theta;
+				const float wz = fminf(zp1, q.z1r) - fmaxf(zp0, q.z0r);
+				const float wz_s = wz / q.widthFrac_z;
+				const int io = zi * P.nxy + xyi * R.strideXY + yxi * R.strideYX;
+				float w = wxy_s * wz_s * R.dxy_cos_theta;
 				if (TOF)
-					w *= tw;
-				if (MODE == MODE_FWD)
-					acc = fmaf(w, img[io], acc);
-				else if (MODE == MODE_BACK)
-					unsafeAtomicAdd(img + io, val * w);
-				else
-					cnt++;
+					w *= q.tw;
+				fn(io, xyi, yxi, zi, w);
 			}
 		}
 	}
+}
+
+template <int MODE, bool TOF, bool S64>
+__device__ float ddRay(const DImage& P, float* __restrict__ img, float3 d1,
+                       float3 d2, float3 n1, float3 n2, float val,
+                       const DProj& pr, float tofv, uint32_t& cnt)
+{
+	DDRay R;
+	ddSetup<TOF, S64>(P, d1, d2, n1, n2, pr, tofv, !(MODE == MODE_FWD || MODE == MODE_COUNT), R);
+	float acc = 0.f;
+	ddVisit<TOF>(P, R, pr, 0, 1 << 30, 0, 1 << 30, 0, 1 << 30,
+	             [&](int io, int, int, int, float w) {
+		             if (MODE == MODE_FWD)
+			             acc = fmaf(w, img[io], acc);
+		             else if (MODE == MODE_BACK)
+			             unsafeAtomicAdd(img + io, val * w);
+		             else
+			             cnt++;
+	             });
 	return acc;
 }
 

@@ -1,4 +1,4 @@
-// Tile-binned Siddon backprojection for gfx950.
+// Tile-binned backprojection (Siddon and distance-driven) for gfx950.
 //
 // Random FP32 atomics into a 20 MB image run at a few tens of G/s on the
 // chip, an order of magnitude below the rate at which rays are walked, so
@@ -6,18 +6,20 @@
 // (LOR, ray) is cut at the boundaries of TX x TY x TZ image tiles:
 //
 //  plan (once per LOR set; geometry only):
+//    kRayPrep     resolve every (LOR, ray) once into the ray table
 //    kBin<COUNT>  walk every ray, count tile entries per tile (LDS counters,
 //                 one global atomic per tile and workgroup)
 //    host         prefix sum -> tile ranges, chunk table
 //    kBin<FILL>   walk again, write one record per tile entry into its
 //                 tile's range: key = lor * R + ray, the alpha where the
 //                 walk enters the tile, the voxel it enters and the ray's
-//                 plane shift (sidPlaneShift)
+//                 plane shift (sidPlaneShift); a DD record is the LOR alone
 //  per backprojection:
 //    kTileBack    one workgroup per chunk of a tile's records: re-enter each
-//                 ray's walk at its record (sidRestart), accumulate in the
-//                 tile held in LDS (ds_add_f32), then add the tile to the
-//                 image with coalesced atomics (one per touched voxel).
+//                 ray's walk at its record (sidRestart) or sweep the DD
+//                 footprint over the tile, accumulate in the tile held in
+//                 LDS, then add the tile to the image with coalesced atomics
+//                 (one per touched voxel).
 #include "yrtmi_tiles.hpp"
 
 #include <algorithm>
@@ -45,6 +47,9 @@ __device__ __forceinline__ int tileOf(const DTiles& T, int f, int vx, int vy,
 // Ray table of a LOR set (rayEntry for every (LOR, ray) t = e * R + r):
 // the LOR resolved once (detector pair or histogram bin, LUT, motion,
 // frame), the multi-ray line and the image offset.  Skipped LORs get frame -1.
+// Distance-driven (DD): ddEntry per LOR, which adds the detector
+// orientations.
+template <bool DD>
 __global__ void __launch_bounds__(256) kRayPrep(KArgs a, float4* __restrict__ tab)
 {
 	const int R = a.pr.numRays;
@@ -55,9 +60,21 @@ __global__ void __launch_bounds__(256) kRayPrep(KArgs a, float4* __restrict__ ta
 		const int64_t e = t / R;
 		const int r = (int)(t - e * R);
 		const Lor l = fetchLor(a.sc, a.L, e, a.im.nt);
+		const float3 z = make_float3(0.f, 0.f, 0.f);
+		if (DD)
+		{
+			if (l.frame < 0)
+			{
+				ddEntry(tab, (uint32_t)t, z, z, z, z, 0.f, -1);
+				continue;
+			}
+			const float3 d1 = make_float3(l.p1.x - a.im.ox, l.p1.y - a.im.oy, l.p1.z - a.im.oz);
+			const float3 d2 = make_float3(l.p2.x - a.im.ox, l.p2.y - a.im.oy, l.p2.z - a.im.oz);
+			ddEntry(tab, (uint32_t)t, d1, d2, l.n1, l.n2, l.tof, l.frame);
+			continue;
+		}
 		if (l.frame < 0)
 		{
-			const float3 z = make_float3(0.f, 0.f, 0.f);
 			rayEntry(tab, (uint32_t)t, z, z, 0.f, -1);
 			continue;
 		}
@@ -116,6 +133,61 @@ __device__ __forceinline__ void binRay(const DImage& im, const DTof& tof, const 
 		onRecord(last, pa, pv, nseg - start);
 }
 
+// DD: the tiles of LOR t's footprint.  The slices of the major axis are
+// taken one tile slab at a time; a slab's records are the tiles of the
+// bounding box of its slices' footprint ranges (ddSlice, the ranges the
+// sweep itself visits), onRecord(tile, 0, 0, estimated voxels of the piece).
+// A tile of the box that the footprint misses costs its record a set-up and
+// nothing else.
+template <bool TOF, bool S64, class Fn>
+__device__ __forceinline__ void binRayDD(const DImage& im, const DProj& pr, const DTiles& T,
+                                         const float4* __restrict__ tab, uint32_t t,
+                                         Fn&& onRecord)
+{
+	const float4 A = tab[4 * (uint64_t)t], Bq = tab[4 * (uint64_t)t + 1];
+	const int frame = __float_as_int(Bq.w);
+	if (frame < 0)
+		return;
+	const float4 Cq = tab[4 * (uint64_t)t + 2], Dq = tab[4 * (uint64_t)t + 3];
+	DDRay R;
+	ddSetup<TOF, S64>(im, make_float3(A.x, A.y, A.z), make_float3(A.w, Bq.x, Bq.y),
+	                  make_float3(Cq.x, Cq.y, Cq.z), make_float3(Cq.w, Dq.x, Dq.y), pr, Bq.z,
+	                  false, R);
+	const int lm = R.flag_y ? T.ly : T.lx;  // major axis
+	const int ln = R.flag_y ? T.lx : T.ly;  // minor axis
+	for (int s0 = R.xy_i_0; s0 <= R.xy_i_1;)
+	{
+		const int m = s0 >> lm;
+		const int s1 = min(R.xy_i_1, ((m + 1) << lm) - 1);
+		int nlo = 1 << 30, nhi = -1, zlo = 1 << 30, zhi = -1, vox = 0;
+		for (int xyi = s0; xyi <= s1; xyi++)
+		{
+			DDSlice q;
+			ddSlice<false>(im, R, pr, xyi, q);
+			if (q.yx0 <= q.yx1 && q.zi0 <= q.zi1)
+			{
+				nlo = min(nlo, q.yx0);
+				nhi = max(nhi, q.yx1);
+				zlo = min(zlo, q.zi0);
+				zhi = max(zhi, q.zi1);
+				vox += (q.yx1 - q.yx0 + 1) * (q.zi1 - q.zi0 + 1);
+			}
+		}
+		if (nhi >= 0)
+		{
+			const int nt0 = nlo >> ln, nt1 = nhi >> ln, zt0 = zlo >> T.lz, zt1 = zhi >> T.lz;
+			const int est = vox / ((nt1 - nt0 + 1) * (zt1 - zt0 + 1));
+			for (int zt = zt0; zt <= zt1; zt++)
+				for (int nt = nt0; nt <= nt1; nt++)
+				{
+					const int tx = R.flag_y ? nt : m, ty = R.flag_y ? m : nt;
+					onRecord(((frame * T.ntz + zt) * T.nty + ty) * T.ntx + tx, 0.f, 0u, est);
+				}
+		}
+		s0 = s1 + 1;
+	}
+}
+
 __device__ __forceinline__ int lengthClass(const DPlan& pl, int segs)
 {
 	int c = 0;
@@ -129,10 +201,11 @@ __device__ __forceinline__ int lengthClass(const DPlan& pl, int segs)
 // PASS 1 (same grid, same rays per workgroup): each workgroup reserves its
 // counts in every bin from cursor[bin] and walks its rays again, writing
 // each record into its range (an LDS countdown per bin, so a record beyond
-// the reservation is caught without a global read).
+// the reservation is caught without a global read).  DD records are the key
+// alone.
 // LDS: cnt[nB] ints, then (PASS 1) base[nB] 64-bit slots.
-template <bool TOF, bool S64, int PASS>
-__global__ void __launch_bounds__(256) kBin(DImage im, DTof tof, int64_t nRays, DTiles T,
+template <bool DD, bool TOF, bool S64, int PASS>
+__global__ void __launch_bounds__(256) kBin(DImage im, DProj pr, int64_t nRays, DTiles T,
                                             DPlan pl)
 {
 	extern __shared__ int sm[];
@@ -142,16 +215,21 @@ __global__ void __launch_bounds__(256) kBin(DImage im, DTof tof, int64_t nRays, 
 	const int64_t stride = (int64_t)gridDim.x * blockDim.x;
 	const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
 	unsigned int* myCnt = pl.blockCnt + (int64_t)blockIdx.x * nB;
+	auto binOne = [&](uint32_t t, auto&& onRecord) {
+		if constexpr (DD)
+			binRayDD<TOF, S64>(im, pr, T, pl.ray, t, onRecord);
+		else
+			binRay<TOF, S64>(im, pr.tof, T, pl.ray, t, pl.err, onRecord);
+	};
 	if (PASS == 0)
 	{
 		for (int i = threadIdx.x; i < nB; i += blockDim.x)
 			cnt[i] = 0;
 		__syncthreads();
 		for (int64_t t = t0; t < nRays; t += stride)
-			binRay<TOF, S64>(im, tof, T, pl.ray, (uint32_t)t, pl.err,
-			                 [&](int tile, float, uint32_t, int segs) {
-				                 atomicAdd(&cnt[tile * pl.C + lengthClass(pl, segs)], 1);
-			                 });
+			binOne((uint32_t)t, [&](int tile, float, uint32_t, int segs) {
+				atomicAdd(&cnt[tile * pl.C + lengthClass(pl, segs)], 1);
+			});
 		__syncthreads();
 		for (int i = threadIdx.x; i < nB; i += blockDim.x)
 		{
@@ -170,20 +248,22 @@ __global__ void __launch_bounds__(256) kBin(DImage im, DTof tof, int64_t nRays, 
 	}
 	__syncthreads();
 	for (int64_t t = t0; t < nRays; t += stride)
-		binRay<TOF, S64>(im, tof, T, pl.ray, (uint32_t)t, pl.err,
-		                 [&](int tile, float a0, uint32_t pv, int segs) {
-			                 const int b = tile * pl.C + lengthClass(pl, segs);
-			                 const int left = atomicSub(&cnt[b], 1);
-			                 if (left > 0)
-			                 {
-				                 const int64_t j = base[b] + left - 1;
-				                 pl.key[j] = (uint32_t)t;
-				                 pl.alpha[j] = a0;
-				                 pl.vox[j] = pv;
-			                 }
-			                 else
-				                 atomicOr(pl.err, 1);
-		                 });
+		binOne((uint32_t)t, [&](int tile, float a0, uint32_t pv, int segs) {
+			const int b = tile * pl.C + lengthClass(pl, segs);
+			const int left = atomicSub(&cnt[b], 1);
+			if (left > 0)
+			{
+				const int64_t j = base[b] + left - 1;
+				pl.key[j] = (uint32_t)t;
+				if (!DD)
+				{
+					pl.alpha[j] = a0;
+					pl.vox[j] = pv;
+				}
+			}
+			else
+				atomicOr(pl.err, 1);
+		});
 }
 
 // One chunk: accumulate its ray pieces in the LDS tile `acc`, then add the
@@ -214,7 +294,12 @@ __global__ void __launch_bounds__(256) kBin(DImage im, DTof tof, int64_t nRays, 
 // The scale only trades precision against that fallback: no choice of it
 // can overflow the tile.  The 32^3 shape (whose 64-bit tile would need
 // 256 KB of LDS) accumulates in FP32.
-template <bool TOF, bool S64, int SHAPE>
+//
+// DD records (DD = true) hold the LOR alone: its table entry (ddEntry) is set
+// up (ddSetup) and its footprint swept over the slices, minor-axis voxels and
+// z planes of the tile (ddVisit), the voxel weights exactly those of the
+// direct sweep.  A voxel again takes one contribution per record.
+template <bool DD, bool TOF, bool S64, int SHAPE>
 __device__ __forceinline__ void tileChunk(const KArgs& a, const DTiles& T,
                                           const DPlanRead& pl, const TileChunk ch,
                                           const float* __restrict__ v,
@@ -250,8 +335,13 @@ __device__ __forceinline__ void tileChunk(const KArgs& a, const DTiles& T,
 	{
 		if (pl.vexp != nullptr)
 		{
-			float wmax = 1.01f * sqrtf(a.im.vx * a.im.vx + a.im.vy * a.im.vy +
-			                           a.im.vz * a.im.vz);
+			// largest voxel weight: a Siddon segment is at most the voxel
+			// diagonal; a DD weight is at most the slice width over the cosine
+			// of the LOR to the major axis (taken as up to 1/4 here: a larger
+			// weight only takes the global-atomic path)
+			float wmax = DD ? 4.f * fmaxf(a.im.vx, a.im.vy)
+			                : 1.01f * sqrtf(a.im.vx * a.im.vx + a.im.vy * a.im.vy +
+			                                a.im.vz * a.im.vz);
 			if (TOF)
 				wmax *= 1.01f * a.pr.tof.norm;
 			int ew = 0;
@@ -275,81 +365,116 @@ __device__ __forceinline__ void tileChunk(const KArgs& a, const DTiles& T,
 	// flight during the walk
 	constexpr uint32_t kNone = 0xffffffffu;
 	auto lorOf = [&](uint32_t key) { return rPow2 ? key >> rShift : key / R; };
-	auto readRecord = [&](int k, uint32_t& key, float& al, uint32_t& pv) {
-		key = kNone;
-		al = 0.f;
-		pv = 0u;
+	struct Rec
+	{
+		uint32_t key, pv;
+		float al;
+	};
+	struct Ray
+	{
+		float4 A, B, C, D;
+		float val;
+	};
+	auto readRecord = [&](int k, Rec& r) {
+		r.key = kNone;
+		r.al = 0.f;
+		r.pv = 0u;
 		if (k < ch.count)
 		{
 			const int64_t j = ch.begin + k;
-			key = pl.key[j];
-			al = pl.alpha[j];
-			pv = pl.vox[j];
+			r.key = pl.key[j];
+			if (!DD)
+			{
+				r.al = pl.alpha[j];
+				r.pv = pl.vox[j];
+			}
 		}
 	};
-	auto readRay = [&](uint32_t key, float4& A, float4& Bq, float& val) {
-		A = make_float4(0.f, 0.f, 0.f, 0.f);
-		Bq = A;
-		val = 0.f;
+	auto readRay = [&](uint32_t key, Ray& q) {
+		const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+		q.A = z;
+		q.B = z;
+		q.C = z;
+		q.D = z;
+		q.val = 0.f;
 		if ((uint64_t)key < nRays)
 		{
-			A = pl.ray[2 * (uint64_t)key];
-			Bq = pl.ray[2 * (uint64_t)key + 1];
-			val = v[lorOf(key)];
+			if (DD)
+			{
+				q.A = pl.ray[4 * (uint64_t)key];
+				q.B = pl.ray[4 * (uint64_t)key + 1];
+				q.C = pl.ray[4 * (uint64_t)key + 2];
+				q.D = pl.ray[4 * (uint64_t)key + 3];
+				q.val = v[key];
+			}
+			else
+			{
+				q.A = pl.ray[2 * (uint64_t)key];
+				q.B = pl.ray[2 * (uint64_t)key + 1];
+				q.val = v[lorOf(key)];
+			}
 		}
 	};
-	auto walk = [&](uint32_t key, float al, uint32_t pv, float4 A, float4 Bq, float val,
-	                auto&& add) {
-		if ((uint64_t)key >= nRays || val == 0.f)
+	auto walk = [&](const Rec& rc, const Ray& q, auto&& add) {
+		float val = q.val;
+		if ((uint64_t)rc.key >= nRays || val == 0.f)
 			return;
-		if (R > 1u)
-			val = val / (float)R;
-		const int vx = B.x0 + (int)(pv & 31u), vy = B.y0 + (int)((pv >> 5) & 31u),
-		          vz = B.z0 + (int)((pv >> 10) & 31u);
-		const int dx = (int)((pv >> 15) & 15u) - 8, dy = (int)((pv >> 19) & 15u) - 8,
-		          dz = (int)((pv >> 23) & 15u) - 8;
-		const bool first = (pv >> 31) != 0;
-		if (!first && (vx >= B.x1 || vy >= B.y1 || vz >= B.z1))
+		if (__float_as_int(q.B.w) != f)
 			return;
-		if (__float_as_int(Bq.w) != f)
-			return;
-		const float3 q1 = make_float3(A.x, A.y, A.z), q2 = make_float3(A.w, Bq.x, Bq.y);
-		SidRay s;
-		if (!sidSetup<TOF, S64>(a.im, q1, q2, a.pr.tof, Bq.z, s))
-			return;
-		if (!first)
-			sidRestart<S64>(a.im, s, al, vx, vy, vz, dx, dy, dz);
-		sidWalk<TOF, S64>(a.im, s, a.pr.tof, B, first, vx, vy, vz,
-		                  [&](int gx, int gy, int gz, int idx, float, float w) {
-			                  add(idx, gx, gy, gz, val * w);
-		                  });
+		const float3 q1 = make_float3(q.A.x, q.A.y, q.A.z), q2 = make_float3(q.A.w, q.B.x, q.B.y);
+		if constexpr (DD)
+		{
+			DDRay D;
+			ddSetup<TOF, S64>(a.im, q1, q2, make_float3(q.C.x, q.C.y, q.C.z),
+			                  make_float3(q.C.w, q.D.x, q.D.y), a.pr, q.B.z, true, D);
+			const int xlo = D.flag_y ? B.y0 : B.x0, xhi = (D.flag_y ? B.y1 : B.x1) - 1;
+			const int ylo = D.flag_y ? B.x0 : B.y0, yhi = (D.flag_y ? B.x1 : B.y1) - 1;
+			ddVisit<TOF>(a.im, D, a.pr, xlo, xhi, ylo, yhi, B.z0, B.z1 - 1,
+			             [&](int, int xyi, int yxi, int zi, float w) {
+				             const int gx = D.flag_y ? yxi : xyi, gy = D.flag_y ? xyi : yxi;
+				             add(zi * B.sz + gy * B.sy + gx + B.base, gx, gy, zi, val * w);
+			             });
+		}
+		else
+		{
+			if (R > 1u)
+				val = val / (float)R;
+			const uint32_t pv = rc.pv;
+			const int vx = B.x0 + (int)(pv & 31u), vy = B.y0 + (int)((pv >> 5) & 31u),
+			          vz = B.z0 + (int)((pv >> 10) & 31u);
+			const int dx = (int)((pv >> 15) & 15u) - 8, dy = (int)((pv >> 19) & 15u) - 8,
+			          dz = (int)((pv >> 23) & 15u) - 8;
+			const bool first = (pv >> 31) != 0;
+			if (!first && (vx >= B.x1 || vy >= B.y1 || vz >= B.z1))
+				return;
+			SidRay s;
+			if (!sidSetup<TOF, S64>(a.im, q1, q2, a.pr.tof, q.B.z, s))
+				return;
+			if (!first)
+				sidRestart<S64>(a.im, s, rc.al, vx, vy, vz, dx, dy, dz);
+			sidWalk<TOF, S64>(a.im, s, a.pr.tof, B, first, vx, vy, vz,
+			                  [&](int gx, int gy, int gz, int idx, float, float w) {
+				                  add(idx, gx, gy, gz, val * w);
+			                  });
+		}
 	};
 	auto pipeline = [&](auto&& add) {
 		int i = threadIdx.x;
-		uint32_t k0, p0, k1, p1;
-		float a0, a1, v0;
-		float4 A0, B0;
-		readRecord(i, k0, a0, p0);
-		readRay(k0, A0, B0, v0);
-		readRecord(i + NT, k1, a1, p1);
+		Rec r0, r1;
+		Ray q0;
+		readRecord(i, r0);
+		readRay(r0.key, q0);
+		readRecord(i + NT, r1);
 		for (; i < ch.count; i += NT)
 		{
-			float4 A1, B1;
-			float v1;
-			readRay(k1, A1, B1, v1);
-			uint32_t k2, p2;
-			float a2;
-			readRecord(i + 2 * NT, k2, a2, p2);
-			walk(k0, a0, p0, A0, B0, v0, add);
-			k0 = k1;
-			a0 = a1;
-			p0 = p1;
-			A0 = A1;
-			B0 = B1;
-			v0 = v1;
-			k1 = k2;
-			a1 = a2;
-			p1 = p2;
+			Ray q1;
+			readRay(r1.key, q1);
+			Rec r2;
+			readRecord(i + 2 * NT, r2);
+			walk(r0, q0, add);
+			r0 = r1;
+			q0 = q1;
+			r1 = r2;
 		}
 	};
 	if (fixed)
@@ -377,7 +502,7 @@ __device__ __forceinline__ void tileChunk(const KArgs& a, const DTiles& T,
 // One workgroup per chunk (pl.queue == null), or a persistent grid taking
 // chunks from the plan's queue until it is empty.  Every workgroup leaves
 // once the queue index passes the chunk count, so the grid always drains.
-template <bool TOF, bool S64, int SHAPE>
+template <bool DD, bool TOF, bool S64, int SHAPE>
 __global__ void __launch_bounds__(tileThreads(SHAPE))
     kTileBack(KArgs a, DTiles T, DPlanRead pl, const float* __restrict__ v,
               float* __restrict__ y)
@@ -387,7 +512,7 @@ __global__ void __launch_bounds__(tileThreads(SHAPE))
 	__shared__ unsigned long long acc[SHAPE == 2 ? NV / 2 : NV];
 	if (pl.queue == nullptr)
 	{
-		tileChunk<TOF, S64, SHAPE>(a, T, pl, pl.chunks[blockIdx.x], v, y, acc);
+		tileChunk<DD, TOF, S64, SHAPE>(a, T, pl, pl.chunks[blockIdx.x], v, y, acc);
 		return;
 	}
 	__shared__ int64_t next;
@@ -400,7 +525,7 @@ __global__ void __launch_bounds__(tileThreads(SHAPE))
 		__syncthreads();
 		if (c >= pl.nChunks)
 			break;
-		tileChunk<TOF, S64, SHAPE>(a, T, pl, pl.chunks[c], v, y, acc);
+		tileChunk<DD, TOF, S64, SHAPE>(a, T, pl, pl.chunks[c], v, y, acc);
 	}
 	if (threadIdx.x == 0)
 	{
@@ -428,9 +553,10 @@ __device__ __forceinline__ int64_t xcdRange(int64_t b, int64_t grid)
 	return (b % kXcds) * (grid / kXcds) + b / kXcds;
 }
 
-template <bool TOF, int LPE, bool S64>
+template <bool DD, bool TOF, int LPE, bool S64>
 __global__ void __launch_bounds__(256) kEmRatio(KArgs a, float* __restrict__ v)
 {
+	static_assert(!DD || LPE == 1, "DD: one lane per LOR");
 	const int64_t total = a.L.n * LPE;
 	const int64_t stride = (int64_t)gridDim.x * blockDim.x;
 	for (int64_t t = xcdRange(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; t < total;
@@ -442,17 +568,38 @@ __global__ void __launch_bounds__(256) kEmRatio(KArgs a, float* __restrict__ v)
 		// the LOR from the plan's ray table (consecutive LORs: coalesced), or
 		// resolved from the detector pair
 		Lor l;
-		float4 A, B;
+		float4 A, B, C, D;
 		if (a.rays != nullptr)
 		{
-			const int64_t t0 = e * R + min(lane, R - 1);
-			A = a.rays[2 * t0];
-			B = a.rays[2 * t0 + 1];
+			if (DD)
+			{
+				A = a.rays[4 * e];
+				B = a.rays[4 * e + 1];
+				C = a.rays[4 * e + 2];
+				D = a.rays[4 * e + 3];
+				l.p1 = make_float3(A.x, A.y, A.z);
+				l.p2 = make_float3(A.w, B.x, B.y);
+				l.n1 = make_float3(C.x, C.y, C.z);
+				l.n2 = make_float3(C.w, D.x, D.y);
+			}
+			else
+			{
+				const int64_t t0 = e * R + min(lane, R - 1);
+				A = a.rays[2 * t0];
+				B = a.rays[2 * t0 + 1];
+			}
 			l.frame = __float_as_int(B.w);
 			l.tof = B.z;
 		}
 		else
+		{
 			l = fetchLor(a.sc, a.L, e, a.im.nt);
+			if (DD)
+			{
+				l.p1 = make_float3(l.p1.x - a.im.ox, l.p1.y - a.im.oy, l.p1.z - a.im.oz);
+				l.p2 = make_float3(l.p2.x - a.im.ox, l.p2.y - a.im.oy, l.p2.z - a.im.oz);
+			}
+		}
 		const float m = a.c.meas ? a.c.meas[e] : 1.0f;
 		const float sf = a.c.sens ? a.c.sens[e] : 1.0f;
 		const float af = a.c.att ? a.c.att[e] : 1.0f;
@@ -461,29 +608,37 @@ __global__ void __launch_bounds__(256) kEmRatio(KArgs a, float* __restrict__ v)
 		if (!skip)
 		{
 			const float* img = a.x + (int64_t)l.frame * a.im.nvox;
-			for (int r = lane; r < R; r += LPE)
+			if constexpr (DD)
 			{
-				float3 q1, q2;
-				if (a.rays != nullptr)
-				{
-					if (r != lane)
-					{
-						A = a.rays[2 * (e * R + r)];
-						B = a.rays[2 * (e * R + r) + 1];
-					}
-					q1 = make_float3(A.x, A.y, A.z);
-					q2 = make_float3(A.w, B.x, B.y);
-				}
-				else
-				{
-					rayLine(a.pr, l, r, q1, q2);
-					q1 = make_float3(q1.x - a.im.ox, q1.y - a.im.oy, q1.z - a.im.oz);
-					q2 = make_float3(q2.x - a.im.ox, q2.y - a.im.oy, q2.z - a.im.oz);
-				}
+				// l.p1 / l.p2 hold the end points less the image offset
 				uint32_t c = 0;
-				s += siddonRay<MODE_FWD, TOF, S64>(a.im, const_cast<float*>(img), q1,
-				                                   q2, 0.f, a.pr.tof, l.tof, c);
+				s = ddRay<MODE_FWD, TOF, S64>(a.im, const_cast<float*>(img), l.p1, l.p2, l.n1,
+				                              l.n2, 0.f, a.pr, l.tof, c);
 			}
+			else
+				for (int r = lane; r < R; r += LPE)
+				{
+					float3 q1, q2;
+					if (a.rays != nullptr)
+					{
+						if (r != lane)
+						{
+							A = a.rays[2 * (e * R + r)];
+							B = a.rays[2 * (e * R + r) + 1];
+						}
+						q1 = make_float3(A.x, A.y, A.z);
+						q2 = make_float3(A.w, B.x, B.y);
+					}
+					else
+					{
+						rayLine(a.pr, l, r, q1, q2);
+						q1 = make_float3(q1.x - a.im.ox, q1.y - a.im.oy, q1.z - a.im.oz);
+						q2 = make_float3(q2.x - a.im.ox, q2.y - a.im.oy, q2.z - a.im.oz);
+					}
+					uint32_t c = 0;
+					s += siddonRay<MODE_FWD, TOF, S64>(a.im, const_cast<float*>(img), q1,
+					                                   q2, 0.f, a.pr.tof, l.tof, c);
+				}
 		}
 #pragma unroll
 		for (int o = 1; o < LPE; o <<= 1)
@@ -542,17 +697,17 @@ __global__ void __launch_bounds__(256) kSensValue(KArgs a, float* __restrict__ v
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-template <bool TOF, bool S64>
+template <bool DD, bool TOF, bool S64>
 static hipError_t binT(int pass, const KArgs& a, const DTiles& T,
                        const DPlan& pl, int grid, hipStream_t st)
 {
 	const int nB = T.nTiles * pl.C;
 	const int64_t nRays = a.L.n * (int64_t)pl.R;
 	if (pass == 0)
-		kBin<TOF, S64, 0><<<grid, 256, (size_t)nB * sizeof(int), st>>>(a.im, a.pr.tof, nRays,
-		                                                                T, pl);
+		kBin<DD, TOF, S64, 0><<<grid, 256, (size_t)nB * sizeof(int), st>>>(a.im, a.pr, nRays,
+		                                                                    T, pl);
 	else
-		kBin<TOF, S64, 1><<<grid, 256, binLdsBytes(nB), st>>>(a.im, a.pr.tof, nRays, T, pl);
+		kBin<DD, TOF, S64, 1><<<grid, 256, binLdsBytes(nB), st>>>(a.im, a.pr, nRays, T, pl);
 	return hipGetLastError();
 }
 
@@ -562,46 +717,78 @@ hipError_t launchRayPrep(const KArgs& a, float4* tab, hipStream_t st)
 	if (total == 0)
 		return hipSuccess;
 	const unsigned g = (unsigned)std::min<int64_t>(std::max<int64_t>((total + 255) / 256, 1), 65536);
-	kRayPrep<<<g, 256, 0, st>>>(a, tab);
+	if (a.pr.type == 1)
+		kRayPrep<true><<<g, 256, 0, st>>>(a, tab);
+	else
+		kRayPrep<false><<<g, 256, 0, st>>>(a, tab);
 	return hipGetLastError();
+}
+
+template <bool DD>
+static hipError_t binKind(int pass, const KArgs& a, const DTiles& T, const DPlan& pl,
+                          int grid, hipStream_t st)
+{
+	const bool tof = a.pr.useTof != 0;
+	if (a.pr.setup64)
+		return tof ? binT<DD, true, true>(pass, a, T, pl, grid, st)
+		           : binT<DD, false, true>(pass, a, T, pl, grid, st);
+	return tof ? binT<DD, true, false>(pass, a, T, pl, grid, st)
+	           : binT<DD, false, false>(pass, a, T, pl, grid, st);
 }
 
 hipError_t launchBin(int pass, const KArgs& a, const DTiles& T, const DPlan& pl,
                      int grid, hipStream_t st)
 {
-	const bool tof = a.pr.useTof != 0;
-	if (a.pr.setup64)
-		return tof ? binT<true, true>(pass, a, T, pl, grid, st)
-		           : binT<false, true>(pass, a, T, pl, grid, st);
-	return tof ? binT<true, false>(pass, a, T, pl, grid, st)
-	           : binT<false, false>(pass, a, T, pl, grid, st);
+	return a.pr.type == 1 ? binKind<true>(pass, a, T, pl, grid, st)
+	                      : binKind<false>(pass, a, T, pl, grid, st);
 }
 
-template <bool TOF, bool S64, int SHAPE>
+template <bool DD, bool TOF, bool S64, int SHAPE>
 static void tileBackT(const KArgs& a, const DTiles& T, const DPlanRead& pl,
                       const float* v, float* y, unsigned grid, hipStream_t st)
 {
-	kTileBack<TOF, S64, SHAPE><<<grid, tileThreads(SHAPE), 0, st>>>(a, T, pl, v, y);
+	kTileBack<DD, TOF, S64, SHAPE><<<grid, tileThreads(SHAPE), 0, st>>>(a, T, pl, v, y);
 }
 
-template <bool TOF, bool S64>
+template <bool DD, bool TOF, bool S64>
 static void tileBackShape(const KArgs& a, const DTiles& T, const DPlanRead& pl,
                           const float* v, float* y, unsigned grid, hipStream_t st)
 {
 	switch (T.shape)
 	{
 	case 1:
-		tileBackT<TOF, S64, 1>(a, T, pl, v, y, grid, st);
+		tileBackT<DD, TOF, S64, 1>(a, T, pl, v, y, grid, st);
 		break;
 	case 2:
-		tileBackT<TOF, S64, 2>(a, T, pl, v, y, grid, st);
+		tileBackT<DD, TOF, S64, 2>(a, T, pl, v, y, grid, st);
 		break;
 	case 3:
-		tileBackT<TOF, S64, 3>(a, T, pl, v, y, grid, st);
+		tileBackT<DD, TOF, S64, 3>(a, T, pl, v, y, grid, st);
 		break;
 	default:
-		tileBackT<TOF, S64, 0>(a, T, pl, v, y, grid, st);
+		tileBackT<DD, TOF, S64, 0>(a, T, pl, v, y, grid, st);
 		break;
+	}
+}
+
+template <bool DD>
+static void tileBackKind(const KArgs& a, const DTiles& T, const DPlanRead& pl,
+                         const float* v, float* y, unsigned grid, hipStream_t st)
+{
+	const bool tof = a.pr.useTof != 0;
+	if (a.pr.setup64)
+	{
+		if (tof)
+			tileBackShape<DD, true, true>(a, T, pl, v, y, grid, st);
+		else
+			tileBackShape<DD, false, true>(a, T, pl, v, y, grid, st);
+	}
+	else
+	{
+		if (tof)
+			tileBackShape<DD, true, false>(a, T, pl, v, y, grid, st);
+		else
+			tileBackShape<DD, false, false>(a, T, pl, v, y, grid, st);
 	}
 }
 
@@ -679,40 +866,34 @@ hipError_t launchTileBack(const KArgs& a, const DTiles& T, const DPlanRead& pl,
 		q.queue = nullptr;
 		grid = (unsigned)pl.nChunks;
 	}
-	const bool tof = a.pr.useTof != 0;
-	if (a.pr.setup64)
-	{
-		if (tof)
-			tileBackShape<true, true>(a, T, q, v, y, grid, st);
-		else
-			tileBackShape<false, true>(a, T, q, v, y, grid, st);
-	}
+	if (a.pr.type == 1)
+		tileBackKind<true>(a, T, q, v, y, grid, st);
 	else
-	{
-		if (tof)
-			tileBackShape<true, false>(a, T, q, v, y, grid, st);
-		else
-			tileBackShape<false, false>(a, T, q, v, y, grid, st);
-	}
+		tileBackKind<false>(a, T, q, v, y, grid, st);
 	return hipGetLastError();
 }
 
 template <bool TOF, bool S64>
 static hipError_t ratioT(const KArgs& a, int lpe, dim3 g, float* v, hipStream_t st)
 {
+	if (a.pr.type == 1)
+	{
+		kEmRatio<true, TOF, 1, S64><<<g, 256, 0, st>>>(a, v);
+		return hipGetLastError();
+	}
 	switch (lpe)
 	{
 	case 1:
-		kEmRatio<TOF, 1, S64><<<g, 256, 0, st>>>(a, v);
+		kEmRatio<false, TOF, 1, S64><<<g, 256, 0, st>>>(a, v);
 		break;
 	case 2:
-		kEmRatio<TOF, 2, S64><<<g, 256, 0, st>>>(a, v);
+		kEmRatio<false, TOF, 2, S64><<<g, 256, 0, st>>>(a, v);
 		break;
 	case 4:
-		kEmRatio<TOF, 4, S64><<<g, 256, 0, st>>>(a, v);
+		kEmRatio<false, TOF, 4, S64><<<g, 256, 0, st>>>(a, v);
 		break;
 	case 8:
-		kEmRatio<TOF, 8, S64><<<g, 256, 0, st>>>(a, v);
+		kEmRatio<false, TOF, 8, S64><<<g, 256, 0, st>>>(a, v);
 		break;
 	default:
 		return hipErrorInvalidValue;
@@ -722,6 +903,8 @@ static hipError_t ratioT(const KArgs& a, int lpe, dim3 g, float* v, hipStream_t 
 
 hipError_t launchEmRatio(const KArgs& a, int lpe, float* v, hipStream_t st)
 {
+	if (a.pr.type == 1)
+		lpe = 1;
 	const int64_t threads = a.L.n * lpe;
 	int64_t nb = (threads + 255) / 256;
 	if (nb > 0x7ffffff0)

@@ -1,5 +1,5 @@
-// Tile plans: Siddon walks binned by image tile, and the kernels that build
-// and use them (host-side launchers).
+// Tile plans: Siddon walks and distance-driven footprints binned by image
+// tile, and the kernels that build and use them (host-side launchers).
 #pragma once
 
 #include "yrtmi_rays.hpp"
@@ -56,7 +56,8 @@ struct TileChunk
 };
 
 // Plan being built.  Records are binned by (tile, length class): a record's
-// class counts the cuts at or below its number of segments in the tile, and
+// class counts the cuts at or below its number of segments in the tile (DD:
+// its estimated footprint voxels in the tile), and
 // within a tile the records are stored class by class, so that the lanes of
 // a wave walk pieces of similar length.  Bin b = tile * C + class.
 struct DPlan
@@ -68,10 +69,10 @@ struct DPlan
 	unsigned int* blockCnt;     // [grid][nTiles * C] records per workgroup and bin
 	unsigned int* cursor;       // [nTiles * C] fill cursor within the bin
 	const int64_t* tileOff;     // [nTiles * C] first record of the bin
-	uint32_t* key;              // lor * R + ray
-	float* alpha;               // alpha where the walk enters the tile
-	uint32_t* vox;              // packed entry voxel (packEntry)
-	const float4* ray;          // [2 * n * R] ray table (rayEntry, kRayPrep)
+	uint32_t* key;              // lor * R + ray (DD: lor)
+	float* alpha;               // alpha where the walk enters the tile (Siddon)
+	uint32_t* vox;              // packed entry voxel (packEntry; Siddon)
+	const float4* ray;          // ray table (rayEntry / ddEntry, kRayPrep)
 	int* err;                   // set when a tile range would overflow
 };
 
@@ -86,6 +87,18 @@ __device__ __forceinline__ void rayEntry(float4* tab, uint32_t t, float3 q1, flo
 	tab[2 * (uint64_t)t + 1] = make_float4(q2.y, q2.z, tof, __int_as_float(frame));
 }
 
+// Ray table of a DD plan, four float4 per LOR: the end points less the image
+// offset (d1, d2), TOF value and frame laid out as rayEntry, then the two
+// detector orientations (n1, n2).
+__device__ __forceinline__ void ddEntry(float4* tab, uint32_t t, float3 d1, float3 d2,
+                                       float3 n1, float3 n2, float tof, int32_t frame)
+{
+	tab[4 * (uint64_t)t] = make_float4(d1.x, d1.y, d1.z, d2.x);
+	tab[4 * (uint64_t)t + 1] = make_float4(d2.y, d2.z, tof, __int_as_float(frame));
+	tab[4 * (uint64_t)t + 2] = make_float4(n1.x, n1.y, n1.z, n2.x);
+	tab[4 * (uint64_t)t + 3] = make_float4(n2.y, n2.z, 0.f, 0.f);
+}
+
 // Plan in use.  queue != null: persistent launch, workgroups take chunks
 // from queue[0] (queue[1] counts finished workgroups; the last one resets
 // both, so the next launch starts from zero without a memset).
@@ -97,7 +110,7 @@ struct DPlanRead
 	const uint32_t* key;
 	const float* alpha;
 	const uint32_t* vox;
-	const float4* ray;  // rayEntry table
+	const float4* ray;  // rayEntry (Siddon) or ddEntry (DD) table
 	unsigned int* queue;
 	// binade E of the median nonzero |v| of the launch (kVExpHist + kPickExp
 	// before each launch): sets the fixed-point scale of the tile
@@ -119,7 +132,7 @@ hipError_t launchBin(int pass, const KArgs& a, const DTiles& T, const DPlan& pl,
 // workgroup per chunk, or `persistent` workgroups on the chunk queue.
 hipError_t launchTileBack(const KArgs& a, const DTiles& T, const DPlanRead& pl,
                           const float* v, float* y, int persistent, hipStream_t st);
-// v[e] = EM ratio of LOR e (0 where skipped), Siddon.
+// v[e] = EM ratio of LOR e (0 where skipped), Siddon or DD (a.pr.type).
 hipError_t launchEmRatio(const KArgs& a, int lpe, float* v, hipStream_t st);
 // v[e] = sensitivity value of LOR e (factor x attenuation).
 hipError_t launchSensValue(const KArgs& a, float* v, hipStream_t st);
