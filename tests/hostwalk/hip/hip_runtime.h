@@ -2,12 +2,13 @@
 // ray-walk header (yrt-pet_amd/csrc/yrtmi_rays.hpp) with the host compiler,
 // so the walk's logic is checked on the CPU.  It defines the HIP function
 // qualifiers away and gives host versions of the few device intrinsics the
-// header uses (correctly rounded FP32 ops, exp, atomic add).  Nothing in the
+// header uses (correctly rounded FP32 ops, exp, bit casts, atomic add).  Nothing in the
 // library includes it.
 #pragma once
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstring>
 #define __device__
 #define __host__
 #define __global__
@@ -22,6 +23,31 @@ struct float4
 {
 	float x, y, z, w;
 };
+inline float4 make_float4(float a, float b, float c, float d) { return {a, b, c, d}; }
+inline float __int_as_float(int v)
+{
+	float f;
+	std::memcpy(&f, &v, 4);
+	return f;
+}
+inline float __uint_as_float(uint32_t v)
+{
+	float f;
+	std::memcpy(&f, &v, 4);
+	return f;
+}
+inline int __float_as_int(float f)
+{
+	int v;
+	std::memcpy(&v, &f, 4);
+	return v;
+}
+inline uint32_t __float_as_uint(float f)
+{
+	uint32_t v;
+	std::memcpy(&v, &f, 4);
+	return v;
+}
 inline float __fadd_rn(float a, float b) { return a + b; }
 inline float __fsub_rn(float a, float b) { return a - b; }
 inline float __fmul_rn(float a, float b) { return a * b; }

@@ -1222,8 +1222,8 @@ int yrtmi_plan_create(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
 	    hipMalloc(&dOff, sizeof(int64_t) * nB) != hipSuccess ||
 	    hipMalloc(&dErr, sizeof(int)) != hipSuccess)
 		return cleanup(fail(YRTMI_ENOMEM, "plan: build scratch"));
-	// ray table: two float4 per Siddon ray, four per DD LOR
-	const int64_t tabVec = dd ? 4 : 2;
+	// ray table: two float4 per Siddon ray, six per DD LOR (its set-up)
+	const int64_t tabVec = dd ? 6 : 2;
 	if (hipMalloc(&q->ray, tabVec * sizeof(float4) * rays) != hipSuccess)
 		return cleanup(fail(YRTMI_ENOMEM, "plan: ray table"));
 	q->bytes += rays * tabVec * (int64_t)sizeof(float4);
@@ -1288,13 +1288,13 @@ int yrtmi_plan_create(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
 	q->nChunks = (int64_t)chunks.size();
 	if (total > 0)
 	{
-		// DD records are the key alone
+		// DD records have no alpha
 		if (hipMalloc(&q->key, sizeof(uint32_t) * total) != hipSuccess ||
 		    (!dd && hipMalloc(&q->alpha, sizeof(float) * total) != hipSuccess) ||
-		    (!dd && hipMalloc(&q->vox, sizeof(uint32_t) * total) != hipSuccess) ||
+		    hipMalloc(&q->vox, sizeof(uint32_t) * total) != hipSuccess ||
 		    hipMalloc(&q->chunks, sizeof(TileChunk) * chunks.size()) != hipSuccess)
 			return cleanup(fail(YRTMI_ENOMEM, "plan: records"));
-		q->bytes += total * (dd ? 4 : 12) + (int64_t)(sizeof(TileChunk) * chunks.size());
+		q->bytes += total * (dd ? 8 : 12) + (int64_t)(sizeof(TileChunk) * chunks.size());
 		e = hipMemcpyAsync(dOff, off.data(), sizeof(int64_t) * nB,
 		                   hipMemcpyHostToDevice, c->stream);
 		if (e == hipSuccess)

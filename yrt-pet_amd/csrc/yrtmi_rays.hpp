@@ -649,6 +649,7 @@ __device__ __forceinline__ float siddonRay(const DImage& P,
 struct DDRay
 {
 	bool usePsf;
+	int psfRow;       // row of the PSF table for this LOR
 	const float* pk;  // PSF taps (flipped for backprojection)
 	float ext, d_norm, d1_i, d2_i;
 	int xy_i_0, xy_i_1, nyx, strideXY, strideYX;
@@ -672,6 +673,7 @@ __device__ __forceinline__ void ddSetup(const DImage& P, float3 d1, float3 d2, f
 	const bool usePsf = pr.psf.k != nullptr;
 	const float* pk = nullptr;
 	float ext = 0.f;
+	int psfRow = 0;
 	if (usePsf)
 	{
 		// ProjectionPsfManager::getKernel (ProjectionPsfManager.cpp:63-91)
@@ -688,6 +690,7 @@ __device__ __forceinline__ void ddSetup(const DImage& P, float3 d1, float3 d2, f
 			if (s > pr.psf.numS - 1)
 				s = pr.psf.numS - 1;
 		}
+		psfRow = s;
 		pk = (flipPsf ? pr.psf.kflip : pr.psf.k) + s * pr.psf.K;
 		ext = pr.psf.halfWidth;
 	}
@@ -845,6 +848,7 @@ __device__ __forceinline__ void ddSetup(const DImage& P, float3 d1, float3 d2, f
 	if (TOF)
 		pc = (float)(0.5 * (double)d_norm + (double)tofMm(tofv));
 	R.usePsf = usePsf;
+	R.psfRow = psfRow;
 	R.pk = pk;
 	R.ext = ext;
 	R.d_norm = d_norm;
@@ -884,6 +888,92 @@ __device__ __forceinline__ void ddSetup(const DImage& P, float3 d1, float3 d2, f
 	R.half_dyx = half_dyx;
 	R.hvz = hvz;
 	R.pc = pc;
+}
+
+// The per-LOR part of a DDRay in six float4 (a plan's DD ray table), and
+// back: the LOR's set-up is computed once (ddSetup) and the sweeps that use
+// it read it.  The image- and PSF-derived fields are recomputed from the
+// image and projector exactly as ddSetup computes them, so a loaded DDRay
+// equals the one ddSetup makes for the same LOR.  frame -1: skipped LOR.
+__device__ __forceinline__ void ddStore(float4* __restrict__ tab, uint64_t t, const DDRay& R,
+                                        int32_t frame)
+{
+	float4* o = tab + 6 * t;
+	o[0] = make_float4(R.d1_xy_lo, R.d1_xy_hi, R.d2_xy_lo, R.d2_xy_hi);
+	o[1] = make_float4(R.d1_yx_lo, R.d1_yx_hi, R.d2_yx_lo, R.d2_yx_hi);
+	o[2] = make_float4(R.d1zloI, R.d1zloZ, R.d1zhiI, R.d1zhiZ);
+	o[3] = make_float4(R.d2zloI, R.d2zloZ, R.d2zhiI, R.d2zhiZ);
+	o[4] = make_float4(R.dxy_cos_theta, R.d_norm, R.d1_i, R.d2_i);
+	// slice range and major axis in one word: a non-empty range lies within
+	// the image (below 2048 slices); an empty one is stored as [1, 0]
+	const bool empty = R.xy_i_0 > R.xy_i_1;
+	const uint32_t lo = empty ? 1u : (uint32_t)R.xy_i_0, hi = empty ? 0u : (uint32_t)R.xy_i_1;
+	const uint32_t rg = lo | (hi << 11) | (R.flag_y ? 1u << 22 : 0u);
+	o[5] = make_float4(R.pc, __uint_as_float(rg), __int_as_float(R.psfRow), __int_as_float(frame));
+}
+
+// An empty entry: frame -1 and an empty slice range.
+__device__ __forceinline__ void ddStoreSkip(float4* __restrict__ tab, uint64_t t)
+{
+	float4* o = tab + 6 * t;
+	const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+	o[0] = z;
+	o[1] = z;
+	o[2] = z;
+	o[3] = z;
+	o[4] = z;
+	o[5] = make_float4(0.f, __uint_as_float(1u), 0.f, __int_as_float(-1));
+}
+
+__device__ __forceinline__ int32_t ddFrame(const float4& e5) { return __float_as_int(e5.w); }
+
+__device__ __forceinline__ void ddLoad(const DImage& P, const DProj& pr, const float4 (&e)[6],
+                                       bool flipPsf, DDRay& R)
+{
+#pragma clang fp contract(off)
+	R.d1_xy_lo = e[0].x;
+	R.d1_xy_hi = e[0].y;
+	R.d2_xy_lo = e[0].z;
+	R.d2_xy_hi = e[0].w;
+	R.d1_yx_lo = e[1].x;
+	R.d1_yx_hi = e[1].y;
+	R.d2_yx_lo = e[1].z;
+	R.d2_yx_hi = e[1].w;
+	R.d1zloI = e[2].x;
+	R.d1zloZ = e[2].y;
+	R.d1zhiI = e[2].z;
+	R.d1zhiZ = e[2].w;
+	R.d2zloI = e[3].x;
+	R.d2zloZ = e[3].y;
+	R.d2zhiI = e[3].z;
+	R.d2zhiZ = e[3].w;
+	R.dxy_cos_theta = e[4].x;
+	R.d_norm = e[4].y;
+	R.d1_i = e[4].z;
+	R.d2_i = e[4].w;
+	R.pc = e[5].x;
+	const uint32_t rg = __float_as_uint(e[5].y);
+	R.xy_i_0 = (int)(rg & 2047u);
+	R.xy_i_1 = (int)((rg >> 11) & 2047u);
+	R.flag_y = ((rg >> 22) & 1u) != 0;
+	R.psfRow = __float_as_int(e[5].z);
+	R.usePsf = pr.psf.k != nullptr;
+	R.pk = R.usePsf ? (flipPsf ? pr.psf.kflip : pr.psf.k) + R.psfRow * pr.psf.K : nullptr;
+	R.ext = R.usePsf ? pr.psf.halfWidth : 0.f;
+	const bool fy = R.flag_y;
+	R.lxy = fy ? P.ly : P.lx;
+	R.dxy = fy ? P.vy : P.vx;
+	R.lyx = fy ? P.lx : P.ly;
+	R.dyx = fy ? P.vx : P.vy;
+	R.nyx = fy ? P.nx : P.ny;
+	R.strideXY = fy ? P.nx : 1;
+	R.strideYX = fy ? 1 : P.nx;
+	R.yx_off = (float)(R.nyx - 1) / 2.f;
+	R.inv_dyx = 1.0f / R.dyx;
+	R.z_off = (float)(P.nz - 1) * 0.5f;
+	R.inv_dz = 1.0f / P.vz;
+	R.half_dyx = R.dyx * 0.5f;
+	R.hvz = P.vz * 0.5f;
 }
 
 // Footprint range of slice xyi: minor-axis voxels [yx0, yx1], z voxels
@@ -993,6 +1083,17 @@ __device__ __forceinline__ void ddVisit(const DImage& P, const DDRay& R, const D
 			}
 		}
 	}
+}
+
+// Forward sweep of a set-up DD LOR over the whole image.
+template <bool TOF>
+__device__ __forceinline__ float ddSweepFwd(const DImage& P, const float* __restrict__ img,
+                                            const DDRay& R, const DProj& pr)
+{
+	float acc = 0.f;
+	ddVisit<TOF>(P, R, pr, 0, 1 << 30, 0, 1 << 30, 0, 1 << 30,
+	             [&](int io, int, int, int, float w) { acc = fmaf(w, img[io], acc); });
+	return acc;
 }
 
 template <int MODE, bool TOF, bool S64>

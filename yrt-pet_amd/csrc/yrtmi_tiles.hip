@@ -47,9 +47,8 @@ __device__ __forceinline__ int tileOf(const DTiles& T, int f, int vx, int vy,
 // Ray table of a LOR set (rayEntry for every (LOR, ray) t = e * R + r):
 // the LOR resolved once (detector pair or histogram bin, LUT, motion,
 // frame), the multi-ray line and the image offset.  Skipped LORs get frame -1.
-// Distance-driven (DD): ddEntry per LOR, which adds the detector
-// orientations.
-template <bool DD>
+// Distance-driven (DD): the LOR's footprint set-up (ddSetup, ddStore).
+template <bool DD, bool TOF, bool S64>
 __global__ void __launch_bounds__(256) kRayPrep(KArgs a, float4* __restrict__ tab)
 {
 	const int R = a.pr.numRays;
@@ -60,21 +59,23 @@ __global__ void __launch_bounds__(256) kRayPrep(KArgs a, float4* __restrict__ ta
 		const int64_t e = t / R;
 		const int r = (int)(t - e * R);
 		const Lor l = fetchLor(a.sc, a.L, e, a.im.nt);
-		const float3 z = make_float3(0.f, 0.f, 0.f);
 		if (DD)
 		{
 			if (l.frame < 0)
 			{
-				ddEntry(tab, (uint32_t)t, z, z, z, z, 0.f, -1);
+				ddStoreSkip(tab, (uint64_t)t);
 				continue;
 			}
 			const float3 d1 = make_float3(l.p1.x - a.im.ox, l.p1.y - a.im.oy, l.p1.z - a.im.oz);
 			const float3 d2 = make_float3(l.p2.x - a.im.ox, l.p2.y - a.im.oy, l.p2.z - a.im.oz);
-			ddEntry(tab, (uint32_t)t, d1, d2, l.n1, l.n2, l.tof, l.frame);
+			DDRay D;
+			ddSetup<TOF, S64>(a.im, d1, d2, l.n1, l.n2, a.pr, l.tof, false, D);
+			ddStore(tab, (uint64_t)t, D, l.frame);
 			continue;
 		}
 		if (l.frame < 0)
 		{
+			const float3 z = make_float3(0.f, 0.f, 0.f);
 			rayEntry(tab, (uint32_t)t, z, z, 0.f, -1);
 			continue;
 		}
@@ -134,32 +135,31 @@ __device__ __forceinline__ void binRay(const DImage& im, const DTof& tof, const 
 }
 
 // DD: the tiles of LOR t's footprint.  The slices of the major axis are
-// taken one tile slab at a time; a slab's records are the tiles of the
-// bounding box of its slices' footprint ranges (ddSlice, the ranges the
-// sweep itself visits), onRecord(tile, 0, 0, estimated voxels of the piece).
-// A tile of the box that the footprint misses costs its record a set-up and
-// nothing else.
-template <bool TOF, bool S64, class Fn>
+// taken one tile slab at a time, with the footprint ranges of each slice
+// (ddSlice, the ranges the sweep itself visits).  Each tile of the bounding
+// box of a slab's ranges that some slice reaches gives a record:
+// onRecord(tile, 0, first | last << 6, voxels), first and last the slices
+// (less the slab's first) that reach it — the sweep of the record goes over
+// these slices only — and voxels its number of footprint voxels in the tile.
+template <class Fn>
 __device__ __forceinline__ void binRayDD(const DImage& im, const DProj& pr, const DTiles& T,
                                          const float4* __restrict__ tab, uint32_t t,
                                          Fn&& onRecord)
 {
-	const float4 A = tab[4 * (uint64_t)t], Bq = tab[4 * (uint64_t)t + 1];
-	const int frame = __float_as_int(Bq.w);
+	const float4* te = tab + 6 * (uint64_t)t;
+	const float4 e[6] = {te[0], te[1], te[2], te[3], te[4], te[5]};
+	const int frame = ddFrame(e[5]);
 	if (frame < 0)
 		return;
-	const float4 Cq = tab[4 * (uint64_t)t + 2], Dq = tab[4 * (uint64_t)t + 3];
 	DDRay R;
-	ddSetup<TOF, S64>(im, make_float3(A.x, A.y, A.z), make_float3(A.w, Bq.x, Bq.y),
-	                  make_float3(Cq.x, Cq.y, Cq.z), make_float3(Cq.w, Dq.x, Dq.y), pr, Bq.z,
-	                  false, R);
+	ddLoad(im, pr, e, false, R);
 	const int lm = R.flag_y ? T.ly : T.lx;  // major axis
 	const int ln = R.flag_y ? T.lx : T.ly;  // minor axis
 	for (int s0 = R.xy_i_0; s0 <= R.xy_i_1;)
 	{
-		const int m = s0 >> lm;
-		const int s1 = min(R.xy_i_1, ((m + 1) << lm) - 1);
-		int nlo = 1 << 30, nhi = -1, zlo = 1 << 30, zhi = -1, vox = 0;
+		const int m = s0 >> lm, mb = m << lm;
+		const int s1 = min(R.xy_i_1, mb + (1 << lm) - 1);
+		int nlo = 1 << 30, nhi = -1, zlo = 1 << 30, zhi = -1;
 		for (int xyi = s0; xyi <= s1; xyi++)
 		{
 			DDSlice q;
@@ -170,18 +170,36 @@ __device__ __forceinline__ void binRayDD(const DImage& im, const DProj& pr, cons
 				nhi = max(nhi, q.yx1);
 				zlo = min(zlo, q.zi0);
 				zhi = max(zhi, q.zi1);
-				vox += (q.yx1 - q.yx0 + 1) * (q.zi1 - q.zi0 + 1);
 			}
 		}
 		if (nhi >= 0)
 		{
-			const int nt0 = nlo >> ln, nt1 = nhi >> ln, zt0 = zlo >> T.lz, zt1 = zhi >> T.lz;
-			const int est = vox / ((nt1 - nt0 + 1) * (zt1 - zt0 + 1));
-			for (int zt = zt0; zt <= zt1; zt++)
-				for (int nt = nt0; nt <= nt1; nt++)
+			for (int zt = zlo >> T.lz; zt <= zhi >> T.lz; zt++)
+				for (int nt = nlo >> ln; nt <= nhi >> ln; nt++)
 				{
-					const int tx = R.flag_y ? nt : m, ty = R.flag_y ? m : nt;
-					onRecord(((frame * T.ntz + zt) * T.nty + ty) * T.ntx + tx, 0.f, 0u, est);
+					const int y0 = nt << ln, y1 = y0 + (1 << ln) - 1;
+					const int z0 = zt << T.lz, z1 = z0 + (1 << T.lz) - 1;
+					int first = -1, last = -1, vox = 0;
+					for (int xyi = s0; xyi <= s1; xyi++)
+					{
+						DDSlice q;
+						ddSlice<false>(im, R, pr, xyi, q);
+						const int a0 = max(q.yx0, y0), a1 = min(q.yx1, y1);
+						const int b0 = max(q.zi0, z0), b1 = min(q.zi1, z1);
+						if (a0 <= a1 && b0 <= b1)
+						{
+							if (first < 0)
+								first = xyi;
+							last = xyi;
+							vox += (a1 - a0 + 1) * (b1 - b0 + 1);
+						}
+					}
+					if (vox > 0)
+					{
+						const int tx = R.flag_y ? nt : m, ty = R.flag_y ? m : nt;
+						onRecord(((frame * T.ntz + zt) * T.nty + ty) * T.ntx + tx, 0.f,
+						         (uint32_t)(first - mb) | ((uint32_t)(last - mb) << 6), vox);
+					}
 				}
 		}
 		s0 = s1 + 1;
@@ -201,8 +219,8 @@ __device__ __forceinline__ int lengthClass(const DPlan& pl, int segs)
 // PASS 1 (same grid, same rays per workgroup): each workgroup reserves its
 // counts in every bin from cursor[bin] and walks its rays again, writing
 // each record into its range (an LDS countdown per bin, so a record beyond
-// the reservation is caught without a global read).  DD records are the key
-// alone.
+// the reservation is caught without a global read).  DD records have no
+// alpha.
 // LDS: cnt[nB] ints, then (PASS 1) base[nB] 64-bit slots.
 template <bool DD, bool TOF, bool S64, int PASS>
 __global__ void __launch_bounds__(256) kBin(DImage im, DProj pr, int64_t nRays, DTiles T,
@@ -217,7 +235,7 @@ __global__ void __launch_bounds__(256) kBin(DImage im, DProj pr, int64_t nRays, 
 	unsigned int* myCnt = pl.blockCnt + (int64_t)blockIdx.x * nB;
 	auto binOne = [&](uint32_t t, auto&& onRecord) {
 		if constexpr (DD)
-			binRayDD<TOF, S64>(im, pr, T, pl.ray, t, onRecord);
+			binRayDD(im, pr, T, pl.ray, t, onRecord);
 		else
 			binRay<TOF, S64>(im, pr.tof, T, pl.ray, t, pl.err, onRecord);
 	};
@@ -256,10 +274,8 @@ __global__ void __launch_bounds__(256) kBin(DImage im, DProj pr, int64_t nRays, 
 				const int64_t j = base[b] + left - 1;
 				pl.key[j] = (uint32_t)t;
 				if (!DD)
-				{
 					pl.alpha[j] = a0;
-					pl.vox[j] = pv;
-				}
+				pl.vox[j] = pv;
 			}
 			else
 				atomicOr(pl.err, 1);
@@ -370,9 +386,11 @@ __device__ __forceinline__ void tileChunk(const KArgs& a, const DTiles& T,
 		uint32_t key, pv;
 		float al;
 	};
+	// Siddon: two float4 of the ray table; DD: six (ddStore)
+	constexpr int NE = DD ? 6 : 2;
 	struct Ray
 	{
-		float4 A, B, C, D;
+		float4 e[NE];
 		float val;
 	};
 	auto readRecord = [&](int k, Rec& r) {
@@ -384,59 +402,49 @@ __device__ __forceinline__ void tileChunk(const KArgs& a, const DTiles& T,
 			const int64_t j = ch.begin + k;
 			r.key = pl.key[j];
 			if (!DD)
-			{
 				r.al = pl.alpha[j];
-				r.pv = pl.vox[j];
-			}
+			r.pv = pl.vox[j];
 		}
 	};
 	auto readRay = [&](uint32_t key, Ray& q) {
-		const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
-		q.A = z;
-		q.B = z;
-		q.C = z;
-		q.D = z;
+#pragma unroll
+		for (int i = 0; i < NE; i++)
+			q.e[i] = make_float4(0.f, 0.f, 0.f, 0.f);
 		q.val = 0.f;
 		if ((uint64_t)key < nRays)
 		{
-			if (DD)
-			{
-				q.A = pl.ray[4 * (uint64_t)key];
-				q.B = pl.ray[4 * (uint64_t)key + 1];
-				q.C = pl.ray[4 * (uint64_t)key + 2];
-				q.D = pl.ray[4 * (uint64_t)key + 3];
-				q.val = v[key];
-			}
-			else
-			{
-				q.A = pl.ray[2 * (uint64_t)key];
-				q.B = pl.ray[2 * (uint64_t)key + 1];
-				q.val = v[lorOf(key)];
-			}
+#pragma unroll
+			for (int i = 0; i < NE; i++)
+				q.e[i] = pl.ray[NE * (uint64_t)key + i];
+			q.val = v[DD ? key : lorOf(key)];
 		}
 	};
 	auto walk = [&](const Rec& rc, const Ray& q, auto&& add) {
 		float val = q.val;
 		if ((uint64_t)rc.key >= nRays || val == 0.f)
 			return;
-		if (__float_as_int(q.B.w) != f)
-			return;
-		const float3 q1 = make_float3(q.A.x, q.A.y, q.A.z), q2 = make_float3(q.A.w, q.B.x, q.B.y);
 		if constexpr (DD)
 		{
+			if (ddFrame(q.e[5]) != f)
+				return;
 			DDRay D;
-			ddSetup<TOF, S64>(a.im, q1, q2, make_float3(q.C.x, q.C.y, q.C.z),
-			                  make_float3(q.C.w, q.D.x, q.D.y), a.pr, q.B.z, true, D);
-			const int xlo = D.flag_y ? B.y0 : B.x0, xhi = (D.flag_y ? B.y1 : B.x1) - 1;
+			ddLoad(a.im, a.pr, q.e, true, D);
+			// the record's slices: [first, last] of the tile's slab
+			const int mb = D.flag_y ? B.y0 : B.x0;
+			const int xlo = mb + (int)(rc.pv & 63u), xhi = mb + (int)((rc.pv >> 6) & 63u);
 			const int ylo = D.flag_y ? B.x0 : B.y0, yhi = (D.flag_y ? B.x1 : B.y1) - 1;
-			ddVisit<TOF>(a.im, D, a.pr, xlo, xhi, ylo, yhi, B.z0, B.z1 - 1,
-			             [&](int, int xyi, int yxi, int zi, float w) {
+			ddVisit<TOF>(a.im, D, a.pr, xlo, min(xhi, (D.flag_y ? B.y1 : B.x1) - 1), ylo, yhi,
+			             B.z0, B.z1 - 1, [&](int, int xyi, int yxi, int zi, float w) {
 				             const int gx = D.flag_y ? yxi : xyi, gy = D.flag_y ? xyi : yxi;
 				             add(zi * B.sz + gy * B.sy + gx + B.base, gx, gy, zi, val * w);
 			             });
 		}
 		else
 		{
+			const float4 qa = q.e[0], qb = q.e[NE - 1];
+			if (__float_as_int(qb.w) != f)
+				return;
+			const float3 q1 = make_float3(qa.x, qa.y, qa.z), q2 = make_float3(qa.w, qb.x, qb.y);
 			if (R > 1u)
 				val = val / (float)R;
 			const uint32_t pv = rc.pv;
@@ -448,7 +456,7 @@ __device__ __forceinline__ void tileChunk(const KArgs& a, const DTiles& T,
 			if (!first && (vx >= B.x1 || vy >= B.y1 || vz >= B.z1))
 				return;
 			SidRay s;
-			if (!sidSetup<TOF, S64>(a.im, q1, q2, a.pr.tof, q.B.z, s))
+			if (!sidSetup<TOF, S64>(a.im, q1, q2, a.pr.tof, qb.z, s))
 				return;
 			if (!first)
 				sidRestart<S64>(a.im, s, rc.al, vx, vy, vz, dx, dy, dz);
@@ -553,10 +561,38 @@ __device__ __forceinline__ int64_t xcdRange(int64_t b, int64_t grid)
 	return (b % kXcds) * (grid / kXcds) + b / kXcds;
 }
 
-template <bool DD, bool TOF, int LPE, bool S64>
+// The EM ratio of LOR e from its forward projection s (summed over its rays).
+__device__ __forceinline__ float emRatioOut(const KArgs& a, int64_t e, float s, float m,
+                                           float sf, float af)
+{
+	float upd = (a.pr.numRays > 1) ? s / (float)a.pr.numRays : s;
+	if (a.c.sens)
+		upd *= sf;
+	if (a.c.att)
+		upd *= af;
+	upd *= a.c.g;
+	if (a.c.randoms)
+		upd += a.c.randoms[e];
+	if (a.c.scatter)
+		upd += a.c.scatter[e];
+	float out = 0.f;
+	if (fabsf(upd) > a.c.denomThr)
+	{
+		out = m / upd;
+		if (a.c.sens)
+			out *= sf;
+		if (a.c.att)
+			out *= af;
+		out *= a.c.g;
+	}
+	return out;
+}
+
+// Siddon: LPE lanes per LOR splitting its rays; the rays read from the plan's
+// table (TAB) or made from the resolved LOR.
+template <bool TOF, int LPE, bool S64, bool TAB>
 __global__ void __launch_bounds__(256) kEmRatio(KArgs a, float* __restrict__ v)
 {
-	static_assert(!DD || LPE == 1, "DD: one lane per LOR");
 	const int64_t total = a.L.n * LPE;
 	const int64_t stride = (int64_t)gridDim.x * blockDim.x;
 	for (int64_t t = xcdRange(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; t < total;
@@ -568,38 +604,17 @@ __global__ void __launch_bounds__(256) kEmRatio(KArgs a, float* __restrict__ v)
 		// the LOR from the plan's ray table (consecutive LORs: coalesced), or
 		// resolved from the detector pair
 		Lor l;
-		float4 A, B, C, D;
-		if (a.rays != nullptr)
+		float4 A, B;
+		if (TAB)
 		{
-			if (DD)
-			{
-				A = a.rays[4 * e];
-				B = a.rays[4 * e + 1];
-				C = a.rays[4 * e + 2];
-				D = a.rays[4 * e + 3];
-				l.p1 = make_float3(A.x, A.y, A.z);
-				l.p2 = make_float3(A.w, B.x, B.y);
-				l.n1 = make_float3(C.x, C.y, C.z);
-				l.n2 = make_float3(C.w, D.x, D.y);
-			}
-			else
-			{
-				const int64_t t0 = e * R + min(lane, R - 1);
-				A = a.rays[2 * t0];
-				B = a.rays[2 * t0 + 1];
-			}
+			const int64_t t0 = e * R + min(lane, R - 1);
+			A = a.rays[2 * t0];
+			B = a.rays[2 * t0 + 1];
 			l.frame = __float_as_int(B.w);
 			l.tof = B.z;
 		}
 		else
-		{
 			l = fetchLor(a.sc, a.L, e, a.im.nt);
-			if (DD)
-			{
-				l.p1 = make_float3(l.p1.x - a.im.ox, l.p1.y - a.im.oy, l.p1.z - a.im.oz);
-				l.p2 = make_float3(l.p2.x - a.im.ox, l.p2.y - a.im.oy, l.p2.z - a.im.oz);
-			}
-		}
 		const float m = a.c.meas ? a.c.meas[e] : 1.0f;
 		const float sf = a.c.sens ? a.c.sens[e] : 1.0f;
 		const float af = a.c.att ? a.c.att[e] : 1.0f;
@@ -608,65 +623,84 @@ __global__ void __launch_bounds__(256) kEmRatio(KArgs a, float* __restrict__ v)
 		if (!skip)
 		{
 			const float* img = a.x + (int64_t)l.frame * a.im.nvox;
-			if constexpr (DD)
+			for (int r = lane; r < R; r += LPE)
 			{
-				// l.p1 / l.p2 hold the end points less the image offset
-				uint32_t c = 0;
-				s = ddRay<MODE_FWD, TOF, S64>(a.im, const_cast<float*>(img), l.p1, l.p2, l.n1,
-				                              l.n2, 0.f, a.pr, l.tof, c);
-			}
-			else
-				for (int r = lane; r < R; r += LPE)
+				float3 q1, q2;
+				if (TAB)
 				{
-					float3 q1, q2;
-					if (a.rays != nullptr)
+					if (r != lane)
 					{
-						if (r != lane)
-						{
-							A = a.rays[2 * (e * R + r)];
-							B = a.rays[2 * (e * R + r) + 1];
-						}
-						q1 = make_float3(A.x, A.y, A.z);
-						q2 = make_float3(A.w, B.x, B.y);
+						A = a.rays[2 * (e * R + r)];
+						B = a.rays[2 * (e * R + r) + 1];
 					}
-					else
-					{
-						rayLine(a.pr, l, r, q1, q2);
-						q1 = make_float3(q1.x - a.im.ox, q1.y - a.im.oy, q1.z - a.im.oz);
-						q2 = make_float3(q2.x - a.im.ox, q2.y - a.im.oy, q2.z - a.im.oz);
-					}
-					uint32_t c = 0;
-					s += siddonRay<MODE_FWD, TOF, S64>(a.im, const_cast<float*>(img), q1,
-					                                   q2, 0.f, a.pr.tof, l.tof, c);
+					q1 = make_float3(A.x, A.y, A.z);
+					q2 = make_float3(A.w, B.x, B.y);
 				}
+				else
+				{
+					rayLine(a.pr, l, r, q1, q2);
+					q1 = make_float3(q1.x - a.im.ox, q1.y - a.im.oy, q1.z - a.im.oz);
+					q2 = make_float3(q2.x - a.im.ox, q2.y - a.im.oy, q2.z - a.im.oz);
+				}
+				uint32_t c = 0;
+				s += siddonRay<MODE_FWD, TOF, S64>(a.im, const_cast<float*>(img), q1, q2,
+				                                   0.f, a.pr.tof, l.tof, c);
+			}
 		}
 #pragma unroll
 		for (int o = 1; o < LPE; o <<= 1)
 			s += __shfl_xor(s, o, LPE);
-		if (lane != 0)
-			continue;
+		if (lane == 0)
+			v[e] = skip ? 0.f : emRatioOut(a, e, s, m, sf, af);
+	}
+}
+
+// DD: one lane per LOR, its set-up read from the plan's table (TAB) or made
+// from the resolved LOR.
+template <bool TOF, bool S64, bool TAB>
+__global__ void __launch_bounds__(256) kEmRatioDD(KArgs a, float* __restrict__ v)
+{
+	const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+	for (int64_t e = xcdRange(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; e < a.L.n;
+	     e += stride)
+	{
+		const float m = a.c.meas ? a.c.meas[e] : 1.0f;
+		const float sf = a.c.sens ? a.c.sens[e] : 1.0f;
+		const float af = a.c.att ? a.c.att[e] : 1.0f;
+		int frame;
+		Lor l;
+		float4 de[6];
+		if (TAB)
+		{
+			de[5] = a.rays[6 * e + 5];
+			frame = ddFrame(de[5]);
+		}
+		else
+		{
+			l = fetchLor(a.sc, a.L, e, a.im.nt);
+			frame = l.frame;
+		}
+		const bool skip = (m == 0.f || sf == 0.f || af == 0.f || frame < 0);
 		float out = 0.f;
 		if (!skip)
 		{
-			float upd = (a.pr.numRays > 1) ? s / (float)a.pr.numRays : s;
-			if (a.c.sens)
-				upd *= sf;
-			if (a.c.att)
-				upd *= af;
-			upd *= a.c.g;
-			if (a.c.randoms)
-				upd += a.c.randoms[e];
-			if (a.c.scatter)
-				upd += a.c.scatter[e];
-			if (fabsf(upd) > a.c.denomThr)
+			DDRay D;
+			if (TAB)
 			{
-				out = m / upd;
-				if (a.c.sens)
-					out *= sf;
-				if (a.c.att)
-					out *= af;
-				out *= a.c.g;
+#pragma unroll
+				for (int i = 0; i < 5; i++)
+					de[i] = a.rays[6 * e + i];
+				ddLoad(a.im, a.pr, de, false, D);
 			}
+			else
+			{
+				const float3 d1 = make_float3(l.p1.x - a.im.ox, l.p1.y - a.im.oy, l.p1.z - a.im.oz);
+				const float3 d2 = make_float3(l.p2.x - a.im.ox, l.p2.y - a.im.oy, l.p2.z - a.im.oz);
+				ddSetup<TOF, S64>(a.im, d1, d2, l.n1, l.n2, a.pr, l.tof, false, D);
+			}
+			const float s =
+			    ddSweepFwd<TOF>(a.im, a.x + (int64_t)frame * a.im.nvox, D, a.pr);
+			out = emRatioOut(a, e, s, m, sf, af);
 		}
 		v[e] = out;
 	}
@@ -717,10 +751,22 @@ hipError_t launchRayPrep(const KArgs& a, float4* tab, hipStream_t st)
 	if (total == 0)
 		return hipSuccess;
 	const unsigned g = (unsigned)std::min<int64_t>(std::max<int64_t>((total + 255) / 256, 1), 65536);
-	if (a.pr.type == 1)
-		kRayPrep<true><<<g, 256, 0, st>>>(a, tab);
+	if (a.pr.type != 1)
+		kRayPrep<false, false, false><<<g, 256, 0, st>>>(a, tab);
+	else if (a.pr.useTof)
+	{
+		if (a.pr.setup64)
+			kRayPrep<true, true, true><<<g, 256, 0, st>>>(a, tab);
+		else
+			kRayPrep<true, true, false><<<g, 256, 0, st>>>(a, tab);
+	}
 	else
-		kRayPrep<false><<<g, 256, 0, st>>>(a, tab);
+	{
+		if (a.pr.setup64)
+			kRayPrep<true, false, true><<<g, 256, 0, st>>>(a, tab);
+		else
+			kRayPrep<true, false, false><<<g, 256, 0, st>>>(a, tab);
+	}
 	return hipGetLastError();
 }
 
@@ -873,32 +919,42 @@ hipError_t launchTileBack(const KArgs& a, const DTiles& T, const DPlanRead& pl,
 	return hipGetLastError();
 }
 
-template <bool TOF, bool S64>
-static hipError_t ratioT(const KArgs& a, int lpe, dim3 g, float* v, hipStream_t st)
+template <bool TOF, bool S64, bool TAB>
+static hipError_t ratioLpe(const KArgs& a, int lpe, dim3 g, float* v, hipStream_t st)
 {
-	if (a.pr.type == 1)
-	{
-		kEmRatio<true, TOF, 1, S64><<<g, 256, 0, st>>>(a, v);
-		return hipGetLastError();
-	}
 	switch (lpe)
 	{
 	case 1:
-		kEmRatio<false, TOF, 1, S64><<<g, 256, 0, st>>>(a, v);
+		kEmRatio<TOF, 1, S64, TAB><<<g, 256, 0, st>>>(a, v);
 		break;
 	case 2:
-		kEmRatio<false, TOF, 2, S64><<<g, 256, 0, st>>>(a, v);
+		kEmRatio<TOF, 2, S64, TAB><<<g, 256, 0, st>>>(a, v);
 		break;
 	case 4:
-		kEmRatio<false, TOF, 4, S64><<<g, 256, 0, st>>>(a, v);
+		kEmRatio<TOF, 4, S64, TAB><<<g, 256, 0, st>>>(a, v);
 		break;
 	case 8:
-		kEmRatio<false, TOF, 8, S64><<<g, 256, 0, st>>>(a, v);
+		kEmRatio<TOF, 8, S64, TAB><<<g, 256, 0, st>>>(a, v);
 		break;
 	default:
 		return hipErrorInvalidValue;
 	}
 	return hipGetLastError();
+}
+
+template <bool TOF, bool S64>
+static hipError_t ratioT(const KArgs& a, int lpe, dim3 g, float* v, hipStream_t st)
+{
+	if (a.pr.type == 1)
+	{
+		if (a.rays != nullptr)
+			kEmRatioDD<TOF, S64, true><<<g, 256, 0, st>>>(a, v);
+		else
+			kEmRatioDD<TOF, S64, false><<<g, 256, 0, st>>>(a, v);
+		return hipGetLastError();
+	}
+	return a.rays != nullptr ? ratioLpe<TOF, S64, true>(a, lpe, g, v, st)
+	                         : ratioLpe<TOF, S64, false>(a, lpe, g, v, st);
 }
 
 hipError_t launchEmRatio(const KArgs& a, int lpe, float* v, hipStream_t st)

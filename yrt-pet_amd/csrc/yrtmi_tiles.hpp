@@ -57,7 +57,7 @@ struct TileChunk
 
 // Plan being built.  Records are binned by (tile, length class): a record's
 // class counts the cuts at or below its number of segments in the tile (DD:
-// its estimated footprint voxels in the tile), and
+// its footprint voxels in the tile), and
 // within a tile the records are stored class by class, so that the lanes of
 // a wave walk pieces of similar length.  Bin b = tile * C + class.
 struct DPlan
@@ -71,8 +71,9 @@ struct DPlan
 	const int64_t* tileOff;     // [nTiles * C] first record of the bin
 	uint32_t* key;              // lor * R + ray (DD: lor)
 	float* alpha;               // alpha where the walk enters the tile (Siddon)
-	uint32_t* vox;              // packed entry voxel (packEntry; Siddon)
-	const float4* ray;          // ray table (rayEntry / ddEntry, kRayPrep)
+	uint32_t* vox;              // packed entry voxel (packEntry); DD: the
+	                            // record's slices in the slab (binRayDD)
+	const float4* ray;          // ray table (rayEntry / ddStore, kRayPrep)
 	int* err;                   // set when a tile range would overflow
 };
 
@@ -87,18 +88,6 @@ __device__ __forceinline__ void rayEntry(float4* tab, uint32_t t, float3 q1, flo
 	tab[2 * (uint64_t)t + 1] = make_float4(q2.y, q2.z, tof, __int_as_float(frame));
 }
 
-// Ray table of a DD plan, four float4 per LOR: the end points less the image
-// offset (d1, d2), TOF value and frame laid out as rayEntry, then the two
-// detector orientations (n1, n2).
-__device__ __forceinline__ void ddEntry(float4* tab, uint32_t t, float3 d1, float3 d2,
-                                       float3 n1, float3 n2, float tof, int32_t frame)
-{
-	tab[4 * (uint64_t)t] = make_float4(d1.x, d1.y, d1.z, d2.x);
-	tab[4 * (uint64_t)t + 1] = make_float4(d2.y, d2.z, tof, __int_as_float(frame));
-	tab[4 * (uint64_t)t + 2] = make_float4(n1.x, n1.y, n1.z, n2.x);
-	tab[4 * (uint64_t)t + 3] = make_float4(n2.y, n2.z, 0.f, 0.f);
-}
-
 // Plan in use.  queue != null: persistent launch, workgroups take chunks
 // from queue[0] (queue[1] counts finished workgroups; the last one resets
 // both, so the next launch starts from zero without a memset).
@@ -110,7 +99,7 @@ struct DPlanRead
 	const uint32_t* key;
 	const float* alpha;
 	const uint32_t* vox;
-	const float4* ray;  // rayEntry (Siddon) or ddEntry (DD) table
+	const float4* ray;  // rayEntry (Siddon) or ddStore (DD) table
 	unsigned int* queue;
 	// binade E of the median nonzero |v| of the launch (kVExpHist + kPickExp
 	// before each launch): sets the fixed-point scale of the tile
