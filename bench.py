@@ -493,7 +493,8 @@ def bench_frames(args, torch, dev, stream):
 
 def bench_dd(args, torch, dev, stream):
     """Batched distance-driven OSEM with projection-space PSF over dynamic
-    frames (the 48-ring configuration), plain per-frame vs one launch."""
+    frames (the 48-ring configuration): plain per-frame vs one launch over all
+    frames, direct and with tile plans (the value: batched with plans)."""
     import yrtmi
     from yrtmi import synth
 
@@ -579,8 +580,33 @@ def bench_dd(args, torch, dev, stream):
     ctx.synchronize()
     with ctx.graph() as g:
         batched()
-    res["batched_s"] = timed(g.launch, args.steps)
+    res["batched_direct_s"] = timed(g.launch, args.steps)
     log(args, f"DD batched (frames in one launch, locality order, graph): "
+              f"{res['batched_direct_s']:.3f} s/iter")
+    del g
+    # tile plans of the batched subsets (all frames' tiles: the 32x32x8 shape
+    # keeps 16 frames within the bin limit)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    plans = [ctx.plan(proj, imF, bsub[s], tile_shape=yrtmi.capi.TILE_32x32x8) for s in range(S)]
+    ctx.synchronize()
+    res["plan_s"] = time.perf_counter() - t0
+    plan_bytes = sum(p_.info()["bytes"] for p_ in plans)
+    plan_records = sum(p_.info()["records"] for p_ in plans)
+
+    def batched_planned():
+        for s in range(S):
+            ctx.fill(uF, 0.0)
+            ctx.em_subset(proj, imF, xF, uF, bsub[s], plan=plans[s])
+            ctx.em_update(xF, uF, sens, im1.nvox)
+
+    batched_planned()
+    ctx.synchronize()
+    with ctx.graph() as g:
+        batched_planned()
+    res["batched_s"] = timed(g.launch, args.steps)
+    log(args, f"DD batched + tile plans ({plan_records / n:.1f} records/LOR, "
+              f"{plan_bytes / 2**30:.2f} GiB, built in {res['plan_s']:.2f}s): "
               f"{res['batched_s']:.3f} s/iter")
     return {
         "metric": "fwd+back event-projections/s, batched DD+PSF dynamic frames vs plain per-frame",
@@ -598,6 +624,7 @@ def bench_dd(args, torch, dev, stream):
         "plain_value": 2.0 * n / res["plain_s"],
         "speedup_vs_plain": res["plain_s"] / res["batched_s"],
         "footprint_voxels_per_s": 2.0 * fp / res["batched_s"],
+        "plan_bytes": plan_bytes, "plan_records": plan_records,
         "seconds": res,
     }
 
