@@ -297,13 +297,14 @@ __global__ void __launch_bounds__(256) kBin(DImage im, DProj pr, int64_t nRays, 
 // rounded once to a multiple of 2^-k and added as an integer (exact and
 // order-free, so the tile sum is deterministic); the flush converts back
 // (int64 -> float, times 2^-k: exact scaling).  A voxel takes at most one
-// contribution per record, so with every term below 2^(62 - e2) in
-// magnitude (records < 2^e2) no partial sum can overflow; a term that does
-// not fit — and any non-finite one — is added to the image directly with a
-// global FP32 atomic, as the direct backprojection does.  k is chosen from
-// the launch's values: E, the binade of the median nonzero |v| (kVExpHist,
-// kPickExp), 2^10 of headroom, and wmax, the largest segment weight (voxel
-// diagonal, times the TOF norm): k = 62 - e2 - (E + 10 + ew), wmax < 2^ew.
+// contribution per record, so with every term below 2^lim in magnitude,
+// lim = min(62 - e2, 51) (records < 2^e2; 2^51 for the conversion), no
+// partial sum can overflow; a term that does not fit — and any non-finite
+// one — is added to the image directly with a global FP32 atomic, as the
+// direct backprojection does.  k is chosen from the launch's values: E, the
+// binade of the median nonzero |v| (kVExpHist, kPickExp), 2^10 of headroom,
+// and wmax, the largest segment weight (voxel diagonal, times the TOF norm):
+// k = lim - (E + 10 + ew), wmax < 2^ew.
 // Typical contributions then keep ~35 bits below their magnitude; values
 // more than ~1000 times the median (the heavy tail of EM ratios m / Ax on
 // LORs with tiny projections) cost a global atomic instead of precision.
@@ -346,7 +347,7 @@ __device__ __forceinline__ void tileChunk(const KArgs& a, const DTiles& T,
 		acc64[i] = 0ull;
 	// fixed-point scale of this chunk (uniform)
 	int fixed = 0, k = 0;
-	float limit = 0.f;
+	float climit = 0.f;
 	if constexpr (SHAPE != 2)
 	{
 		if (pl.vexp != nullptr)
@@ -362,9 +363,15 @@ __device__ __forceinline__ void tileChunk(const KArgs& a, const DTiles& T,
 				wmax *= 1.01f * a.pr.tof.norm;
 			int ew = 0;
 			(void)frexpf(wmax, &ew);
+			// terms below 2^lim, lim = min(62 - e2, 51): no partial sum of
+			// the chunk's at most 2^e2 terms overflows, and every term
+			// converts exactly through the 2^52 bias below
 			const int e2 = 32 - __clz(ch.count);
-			k = 62 - e2 - (*pl.vexp + 10 + ew);
-			limit = ldexpf(1.f, 62 - e2);
+			const int lim = min(62 - e2, 51);
+			k = lim - (*pl.vexp + 10 + ew);
+			// |c| < 2^(lim - k) <=> |c * 2^k| < 2^lim (inf when above the
+			// float range: then every finite c qualifies)
+			climit = ldexpf(1.f, lim - k);
 			fixed = 1;
 		}
 	}
@@ -485,11 +492,22 @@ __device__ __forceinline__ void tileChunk(const KArgs& a, const DTiles& T,
 			r1 = r2;
 		}
 	};
+	// c * 2^k rounded to an integer: one FP64 fma with the bias 1.5 * 2^52
+	// (|c * 2^k| < 2^51, so the sum stays in [2^52, 2^53), whose ulp is 1:
+	// the fma's rounding is the round-to-nearest-even of c * 2^k, which is
+	// exact in FP64), the integer being the sum's bits less the bias's.
+	// The bias's low word is zero, so that is one 32-bit subtract.
+	const double scale = ldexp(1.0, k);
+	constexpr double kBias = 6755399441055744.0;  // 1.5 * 2^52
+	constexpr unsigned long long kBiasBits = 0x4338000000000000ull;
 	if (fixed)
 		pipeline([&](int idx, int gx, int gy, int gz, float c) {
-			const float xs = ldexpf(c, k);
-			if (fabsf(xs) < limit)
-				ldsAdd64(&acc64[idx], (long long)__builtin_rintf(xs));
+			if (fabsf(c) < climit)
+			{
+				const double d = fma((double)c, scale, kBias);
+				ldsAdd64(&acc64[idx], (long long)((unsigned long long)__double_as_longlong(d) -
+				                                  kBiasBits));
+			}
 			else  // out of the fixed-point range, or not finite
 				unsafeAtomicAdd(yf + (int64_t)gz * a.im.nxy + (int64_t)gy * a.im.nx + gx, c);
 		});
