@@ -516,10 +516,13 @@ __device__ __forceinline__ void sidWalk(const DImage& P, const SidRay& r,
 	float axn = r.axn, ayn = r.ayn, azn = r.azn;
 	float xc = r.xc, yc = r.yc, zc = r.zc;
 	float a_cur = r.a_cur, a_next = 0.f;
-	int dn = 0;
+	// planes crossed by the last step (ties advance together)
+	bool mx = false, my = false, mz = false;
 	auto step = [&]() {
 		const float m = fminf(axn, fminf(ayn, azn));
-		const bool mx = axn == m, my = ayn == m, mz = azn == m;
+		mx = axn == m;
+		my = ayn == m;
+		mz = azn == m;
 		const float xn = xc + sdx, yn = yc + sdy, zn = zc + sdz;
 		xc = mx ? xn : xc;
 		yc = my ? yn : yc;
@@ -527,7 +530,6 @@ __device__ __forceinline__ void sidWalk(const DImage& P, const SidRay& r,
 		axn = mx ? (xn - r.p1.x) * r.invx : axn;
 		ayn = my ? (yn - r.p1.y) * r.invy : ayn;
 		azn = mz ? (zn - r.p1.z) * r.invz : azn;
-		dn = (mx ? 1 : 0) | (my ? 2 : 0) | (mz ? 4 : 0);
 		a_next = fminf(m, amaxf);
 	};
 	float acd = 0.f;  // a_cur * dnorm, carried from the previous segment's end
@@ -556,7 +558,6 @@ __device__ __forceinline__ void sidWalk(const DImage& P, const SidRay& r,
 		a_cur = a_next;
 	};
 	int off;
-	int dprev = 0;
 	if (first)
 	{
 		// planes at or before the ray's start are consumed without a voxel
@@ -575,7 +576,6 @@ __device__ __forceinline__ void sidWalk(const DImage& P, const SidRay& r,
 		if (TOF && !S64)
 			acd = a_cur * dnorm;
 		emit(off);
-		dprev = dn;
 	}
 	else
 	{
@@ -586,12 +586,14 @@ __device__ __forceinline__ void sidWalk(const DImage& P, const SidRay& r,
 	const int ys = r.diry * B.sy, zs = r.dirz * B.sz;
 	const unsigned bx = (unsigned)(B.x1 - B.x0), by = (unsigned)(B.y1 - B.y0),
 	               bz = (unsigned)(B.z1 - B.z0);
+	// each pass moves the voxel across the planes of the previous step (none
+	// before the first: the masks start false)
 	while (a_cur < amaxf)
 	{
-		vx += (dprev & 1) ? r.dirx : 0;
-		vy += (dprev & 2) ? r.diry : 0;
-		vz += (dprev & 4) ? r.dirz : 0;
-		off += ((dprev & 2) ? ys : 0) + ((dprev & 4) ? zs : 0);
+		vx += mx ? r.dirx : 0;
+		vy += my ? r.diry : 0;
+		vz += mz ? r.dirz : 0;
+		off += (my ? ys : 0) + (mz ? zs : 0);
 		if (((unsigned)(vx - B.x0) >= bx) | ((unsigned)(vy - B.y0) >= by) |
 		    ((unsigned)(vz - B.z0) >= bz))
 			break;
@@ -601,7 +603,6 @@ __device__ __forceinline__ void sidWalk(const DImage& P, const SidRay& r,
 		// which is still a crossing: an empty segment, then the move
 		a_next = fmaxf(a_next, a_cur);
 		emit(off);
-		dprev = dn;
 	}
 }
 
