@@ -532,6 +532,10 @@ __device__ __forceinline__ void sidWalk(const DImage& P, const SidRay& r,
 		azn = mz ? (zn - r.p1.z) * r.invz : azn;
 		a_next = fminf(m, amaxf);
 	};
+	// the voxel in box coordinates (lx, ly, lz) = (vx - x0, vy - y0, vz - z0);
+	// its index base + vz * sz + vy * sy + vx is off + lx, off = base +
+	// vz * sz + vy * sy + x0
+	int lx = 0, ly = 0, lz = 0, off = 0;
 	float acd = 0.f;  // a_cur * dnorm, carried from the previous segment's end
 	auto emit = [&](int off) {
 		float w;
@@ -554,10 +558,9 @@ __device__ __forceinline__ void sidWalk(const DImage& P, const SidRay& r,
 		}
 		else
 			w = (a_next - a_cur) * dnorm;
-		fn(vx, vy, vz, off + vx, a_cur, w);
+		fn(lx + B.x0, ly + B.y0, lz + B.z0, off + lx, a_cur, w);
 		a_cur = a_next;
 	};
-	int off;
 	if (first)
 	{
 		// planes at or before the ray's start are consumed without a voxel
@@ -572,14 +575,20 @@ __device__ __forceinline__ void sidWalk(const DImage& P, const SidRay& r,
 		sidMidVoxel<S64>(P, r, a_cur, a_next, vx, vy, vz);
 		if (vx < B.x0 || vx >= B.x1 || vy < B.y0 || vy >= B.y1 || vz < B.z0 || vz >= B.z1)
 			return;
-		off = B.base + vz * B.sz + vy * B.sy;
+		lx = vx - B.x0;
+		ly = vy - B.y0;
+		lz = vz - B.z0;
+		off = B.base + vz * B.sz + vy * B.sy + B.x0;
 		if (TOF && !S64)
 			acd = a_cur * dnorm;
 		emit(off);
 	}
 	else
 	{
-		off = B.base + vz * B.sz + vy * B.sy;
+		lx = vx - B.x0;
+		ly = vy - B.y0;
+		lz = vz - B.z0;
+		off = B.base + vz * B.sz + vy * B.sy + B.x0;
 		if (TOF && !S64)
 			acd = a_cur * dnorm;
 	}
@@ -590,12 +599,11 @@ __device__ __forceinline__ void sidWalk(const DImage& P, const SidRay& r,
 	// before the first: the masks start false)
 	while (a_cur < amaxf)
 	{
-		vx += mx ? r.dirx : 0;
-		vy += my ? r.diry : 0;
-		vz += mz ? r.dirz : 0;
+		lx += mx ? r.dirx : 0;
+		ly += my ? r.diry : 0;
+		lz += mz ? r.dirz : 0;
 		off += (my ? ys : 0) + (mz ? zs : 0);
-		if (((unsigned)(vx - B.x0) >= bx) | ((unsigned)(vy - B.y0) >= by) |
-		    ((unsigned)(vz - B.z0) >= bz))
+		if (((unsigned)lx >= bx) | ((unsigned)ly >= by) | ((unsigned)lz >= bz))
 			break;
 		step();
 		// after the first voxel the planes only come after a_cur; a walk
