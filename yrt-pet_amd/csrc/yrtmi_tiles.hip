@@ -541,13 +541,15 @@ __global__ void __launch_bounds__(tileThreads(SHAPE))
 		tileChunk<DD, TOF, S64, SHAPE>(a, T, pl, pl.chunks[blockIdx.x], v, y, acc);
 		return;
 	}
-	__shared__ int64_t next;
+	// the next chunk's index is passed through the tile's first word (read
+	// between two barriers, before the chunk clears the tile), so the
+	// workgroup's LDS is the tile alone: 32 KB for 16^3, five per CU
 	for (;;)
 	{
 		if (threadIdx.x == 0)
-			next = (int64_t)atomicAdd(&pl.queue[0], 1u);
+			acc[0] = (unsigned long long)atomicAdd(&pl.queue[0], 1u);
 		__syncthreads();
-		const int64_t c = next;
+		const int64_t c = (int64_t)acc[0];
 		__syncthreads();
 		if (c >= pl.nChunks)
 			break;
