@@ -426,13 +426,23 @@ int yrtmi_em_subset_planned(yrtmi_ctx* ctx, const yrtmi_proj* p,
                             const yrtmi_lors* lors,
                             const yrtmi_corrections* corr,
                             const yrtmi_plan* plan);
-/* The first half of yrtmi_em_subset on its own (Siddon): v[i] = the EM
- * ratio of LOR i (m / corrected forward projection, times the
+/* The first half of yrtmi_em_subset on its own (Siddon or DD): v[i] = the
+ * EM ratio of LOR i (m / corrected forward projection, times the
  * multiplicative corrections), 0 where the reference skips the LOR.
- * yrtmi_em_subset_planned = yrtmi_em_ratio + yrtmi_back_planned(v). */
+ * yrtmi_em_subset_planned = yrtmi_em_ratio_planned + yrtmi_back_planned(v).
+ * Replaces the forward half of OSEM_CPU::computeEMUpdateImage
+ * (src/recon/OSEM_CPU.cpp:329-374). */
 int yrtmi_em_ratio(yrtmi_ctx* ctx, const yrtmi_proj* p, const yrtmi_image* im,
                    const float* x, const yrtmi_lors* lors,
                    const yrtmi_corrections* corr, float* v);
+/* yrtmi_em_ratio with the plan of (p, im, lors) (same checks as the
+ * *_planned calls): a DD projector reads each LOR's footprint set-up from
+ * the plan instead of computing it; Siddon resolves its LORs as
+ * yrtmi_em_ratio does.  Results identical to yrtmi_em_ratio. */
+int yrtmi_em_ratio_planned(yrtmi_ctx* ctx, const yrtmi_proj* p, const yrtmi_image* im,
+                           const float* x, const yrtmi_lors* lors,
+                           const yrtmi_corrections* corr, float* v,
+                           const yrtmi_plan* plan);
 int yrtmi_sensitivity_planned(yrtmi_ctx* ctx, const yrtmi_proj* p,
                               const yrtmi_image* im, float* img,
                               const yrtmi_lors* lors,

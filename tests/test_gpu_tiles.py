@@ -222,6 +222,28 @@ def test_em_subset_planned(gpu, rays, tof):
     check(f"em_r{rays}_planned_vs_oracle", u1.cpu().numpy(), ref, TOL_ORACLE)
 
 
+@pytest.mark.parametrize("rays,tof,lpe,s64", [(1, None, 1, 0), (4, 250.0, 4, 0),
+                                              (4, 250.0, 2, 1)])
+def test_em_ratio_planned_matches(gpu, rays, tof, lpe, s64):
+    """em_ratio with the plan equals em_ratio without it, bit for bit."""
+    sc, im, pp, ctx, p = setup(gpu, "dyadic", rays, tof, s64, lpe, nt=2)
+    n = 6000
+    ev = events(sc, n, 106, tof, nt=2)
+    rng = np.random.default_rng(11)
+    x = torch.from_numpy(rng.uniform(0.5, 2.0, im.shape).astype(np.float32)).to(gpu)
+    meas = torch.from_numpy(rng.poisson(3.0, n).astype(np.float32)).to(gpu)
+    L = dev_lors(ev, gpu)
+    plan = ctx.plan(p, im, L)
+    v0 = torch.zeros(n, device=gpu)
+    v1 = torch.zeros(n, device=gpu)
+    ctx.em_ratio(p, im, x, L, v0, meas=meas, global_scale=0.9)
+    ctx.em_ratio(p, im, x, L, v1, meas=meas, global_scale=0.9, plan=plan)
+    ctx.synchronize()
+    # a third of the events are in frame -1 (skipped); many LORs miss the image
+    assert torch.count_nonzero(v0).item() > n // 5
+    assert torch.equal(v0, v1)
+
+
 @pytest.mark.parametrize("att", [False, True])
 def test_sensitivity_planned_histogram(gpu, att):
     sc = yrtmi.Scanner("h", 48.0, 4.0, 4.0, 10.0, 110.0, 64, 12, maxRingDiff=7)

@@ -206,11 +206,16 @@ def test_dd_em_ratio_matches_forward(gpu):
     fwd = torch.zeros(n, device=gpu)
     ctx.forward(p, im, xg, L, fwd)
     v = torch.zeros(n, device=gpu)
-    ctx.em_ratio(p, im, xg, L, v, meas=torch.from_numpy(meas).to(gpu))
+    mg = torch.from_numpy(meas).to(gpu)
+    ctx.em_ratio(p, im, xg, L, v, meas=mg)
+    # from the plan's table of stored set-ups: the same values
+    vp = torch.zeros(n, device=gpu)
+    ctx.em_ratio(p, im, xg, L, vp, meas=mg, plan=ctx.plan(p, im, L))
     ctx.synchronize()
     f = fwd.cpu().numpy()
     want = np.where((meas != 0) & (np.abs(f) > 1e-35), meas / np.where(f == 0, 1, f), 0.0)
     np.testing.assert_array_equal(v.cpu().numpy(), want.astype(np.float32))
+    np.testing.assert_array_equal(vp.cpu().numpy(), want.astype(np.float32))
 
 
 def test_dd_sensitivity_planned_histogram(gpu):

@@ -1066,6 +1066,15 @@ int checkPlan(const yrtmi_plan* q, const yrtmi_proj* p, const yrtmi_image* im,
 	return YRTMI_OK;
 }
 
+// Ray table for the EM ratio's forward projection: DD LORs read their
+// stored footprint set-up; Siddon LORs are resolved from their detector
+// pairs (the LUT stays in L2; reading the table's 32 bytes per ray from HBM
+// measured slower).
+const float4* planRays(const yrtmi_plan* q)
+{
+	return q->proj->d.type == YRTMI_PROJ_DD ? q->ray : nullptr;
+}
+
 DPlanRead planRead(const yrtmi_plan* q)
 {
 	DPlanRead r{};
@@ -1411,6 +1420,28 @@ int yrtmi_em_ratio(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
 	return YRTMI_OK;
 }
 
+int yrtmi_em_ratio_planned(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
+                           const float* x, const yrtmi_lors* l,
+                           const yrtmi_corrections* corr, float* v, const yrtmi_plan* q)
+{
+	KArgs a;
+	int r = prepare(c, p, im, l, a);
+	if (r)
+		return r;
+	r = checkPlan(q, p, im, l);
+	if (r)
+		return r;
+	if (l->n == 0)
+		return YRTMI_OK;
+	if (x == nullptr || v == nullptr)
+		return fail(YRTMI_EINVAL, "em_ratio: null image or output");
+	a.x = x;
+	setCorr(a, corr);
+	a.rays = planRays(q);
+	HIP_TRY(launchEmRatio(a, p->lpe, v, c->stream));
+	return YRTMI_OK;
+}
+
 int yrtmi_em_subset_planned(yrtmi_ctx* c, const yrtmi_proj* p,
                             const yrtmi_image* im, const float* x, float* u,
                             const yrtmi_lors* l, const yrtmi_corrections* corr,
@@ -1429,7 +1460,7 @@ int yrtmi_em_subset_planned(yrtmi_ctx* c, const yrtmi_proj* p,
 		return fail(YRTMI_EINVAL, "em_subset: null image");
 	a.x = x;
 	setCorr(a, corr);
-	a.rays = q->ray;
+	a.rays = planRays(q);
 	HIP_TRY(launchEmRatio(a, p->lpe, q->v, c->stream));
 	HIP_TRY(launchTileBack(a, q->T, planRead(q), q->v, u, q->persistent, c->stream));
 	return YRTMI_OK;

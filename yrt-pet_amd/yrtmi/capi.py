@@ -150,6 +150,8 @@ _SIGS = {
                                           C.POINTER(Lors), C.POINTER(Corrections), _P]),
     "yrtmi_em_ratio": (C.c_int, [_P, _P, C.POINTER(Image), _P, C.POINTER(Lors),
                                  C.POINTER(Corrections), _P]),
+    "yrtmi_em_ratio_planned": (C.c_int, [_P, _P, C.POINTER(Image), _P, C.POINTER(Lors),
+                                         C.POINTER(Corrections), _P, _P]),
     "yrtmi_sensitivity_planned": (C.c_int, [_P, _P, C.POINTER(Image), _P, C.POINTER(Lors),
                                             C.POINTER(Image), _P, _P, _P]),
 }

@@ -292,9 +292,9 @@ class Projector:
 
 class Plan:
     """Tile plan of a LOR set (yrtmi_plan_create): the Siddon walks of its
-    rays binned by 32x32x16 image tiles.  Geometry only; holds device
-    memory until dropped.  Keeps the LOR set alive (the plan refers to its
-    arrays)."""
+    rays (or its DD footprints) binned by image tiles of `tile_shape`, and
+    its ray table.  Geometry only; holds device memory until dropped.  Keeps
+    the LOR set alive (the plan refers to its arrays)."""
 
     def __init__(self, ctx, proj, im, lors, chunk_records=0, tile_shape=0, persistent=0):
         self.ctx, self.proj, self.im, self.lors = ctx, proj, im, lors
@@ -468,8 +468,10 @@ class Context:
         return u
 
     def em_ratio(self, proj, im, x, lors, v, meas=None, sens=None, att=None,
-                 randoms=None, scatter=None, global_scale=1.0, denom_threshold=1e-35):
-        """v[i] = EM ratio of LOR i (the forward half of em_subset)."""
+                 randoms=None, scatter=None, global_scale=1.0, denom_threshold=1e-35,
+                 plan=None):
+        """v[i] = EM ratio of LOR i (the forward half of em_subset); with a
+        plan of (proj, im, lors) the LORs' rays come from its ray table."""
         for t, nm in ((x, "x"), (v, "v"), (meas, "meas"), (sens, "sens"),
                       (att, "att"), (randoms, "randoms"), (scatter, "scatter")):
             _check_dtype(t, torch.float32, nm)
@@ -481,8 +483,13 @@ class Context:
         corr = capi.Corrections(_ptr(meas), _ptr(sens), _ptr(att), _ptr(randoms),
                                 _ptr(scatter), global_scale, denom_threshold)
         ci, cl = im.c(), lors.c()
-        capi.check(capi.lib().yrtmi_em_ratio(self.h, proj.h, C.byref(ci), _ptr(x),
-                                             C.byref(cl), C.byref(corr), _ptr(v)))
+        if plan is not None:
+            capi.check(capi.lib().yrtmi_em_ratio_planned(self.h, proj.h, C.byref(ci), _ptr(x),
+                                                         C.byref(cl), C.byref(corr), _ptr(v),
+                                                         plan.h))
+        else:
+            capi.check(capi.lib().yrtmi_em_ratio(self.h, proj.h, C.byref(ci), _ptr(x),
+                                                 C.byref(cl), C.byref(corr), _ptr(v)))
         return v
 
     def sensitivity(self, proj, im, img, lors, att_im=None, att_img=None, sens=None,
