@@ -339,30 +339,37 @@ int yrtmi_lor_order(yrtmi_ctx* ctx, const yrtmi_image* im,
 int yrtmi_permute(yrtmi_ctx* ctx, const void* src, int elem_bytes,
                   const uint32_t* perm, int64_t n, void* dst);
 
-/* ---- tile plans (Siddon backprojection through LDS image tiles) --------- */
+/* ---- tile plans (backprojection through LDS image tiles) ---------------- */
 /* No counterpart in the reference: a performance extension of the
  * backprojection behind yrtmi_back / yrtmi_em_subset / yrtmi_sensitivity
- * (OperatorProjectorSiddon_GPU::applyAH, OSEM_GPU::computeEMUpdateImage,
+ * (OperatorProjectorSiddon_GPU::applyAH, OperatorProjectorDD_GPU::applyAH,
+ * OSEM_GPU::computeEMUpdateImage,
  * OSEM_GPU::generateSensitivityImageForCurrentSubset).
  *
- * A plan bins the Siddon walk of every (LOR, ray) of `lors` by the image
+ * Siddon: a plan bins the walk of every (LOR, ray) of `lors` by the image
  * tiles it crosses, recording where it enters each (12 bytes per record,
  * records of a tile ordered by the length of their piece), and keeps each
  * ray's end points, TOF value and frame (32 bytes per LOR and ray), so that
  * the backprojection reads one table entry per record instead of resolving
  * the LOR.
- * It depends on geometry only (detector pairs or bins, TOF values, frames,
- * motion frames, the projector, the image geometry), not on image or LOR
- * values: a list-mode or histogram subset is planned once per
+ * DD: a plan keeps each LOR's footprint set-up (96 bytes per LOR) and one
+ * record per tile the footprint reaches (8 bytes: the LOR and the slices of
+ * the tile's slab that reach it, records of a tile ordered by their voxel
+ * count); the backprojection sweeps each record's slices over the tile with
+ * the unplanned sweep's arithmetic.
+ * A plan depends on geometry only (detector pairs or bins, TOF values,
+ * frames, motion frames, the projector, the image geometry), not on image
+ * or LOR values: a list-mode or histogram subset is planned once per
  * reconstruction.  The *_planned calls then accumulate each tile in LDS and
  * add it to the image with coalesced atomics; results equal the unplanned
- * calls' up to FP32 summation order (and, on voxel sizes that are not short
- * binary fractions, ulp-level shifts of segment ends at tile boundaries).
+ * calls' up to FP32 summation order (and, for Siddon on voxel sizes that
+ * are not short binary fractions, ulp-level shifts of segment ends at tile
+ * boundaries).
  *
- * Limits: Siddon only; nx, ny, nz <= 1024; n * num_rays < 2^32; tiles
- * (frames included) at most the device's LDS per workgroup / 12 bytes
- * (5459 with 64 KB).  The call synchronises the context's stream and
- * allocates device memory (yrtmi_plan_info's bytes); the plan owns it.
+ * Limits: nx, ny, nz <= 1024; n * num_rays < 2^32; tiles (frames included)
+ * at most the device's LDS per workgroup / 12 bytes (5459 with 64 KB).  The
+ * call synchronises the context's stream and allocates device memory
+ * (yrtmi_plan_info's bytes); the plan owns it.
  *
  * Options (NULL: defaults): chunk_records, records per workgroup (0: 4096);
  * tile_shape, YRTMI_TILE_16x16x16 (default), YRTMI_TILE_32x32x16,

@@ -550,10 +550,21 @@ struct PlanHandle
 std::unique_ptr<PlanHandle> makePlan(const Context& ctx, const OperatorProjectorMI& proj,
                                      const yrtmi_image& im, const yrtmi_lors& l)
 {
-	// 32 x 32 x 8 tiles: flat tiles suit the mostly transaxial rays (fewer
-	// tile pieces per ray); the fastest shape measured on MI355X for list-mode
-	// TOF subsets and for sensitivity images
-	const yrtmi_plan_options o{0, YRTMI_TILE_32x32x8, 0};
+	// Siddon: 32 x 32 x 8 tiles: flat tiles suit the mostly transaxial rays
+	// (fewer tile pieces per ray); the fastest shape measured on MI355X for
+	// list-mode TOF subsets and for sensitivity images.  DD: 16^3 tiles (five
+	// workgroups per CU) measured fastest, while the tile count (frames
+	// included) stays within the plan's bin limit (5459 tiles with 64 KB of
+	// LDS per workgroup); 32 x 32 x 8 beyond it.
+	int shape = YRTMI_TILE_32x32x8;
+	if (!proj.isSiddon())
+	{
+		const int64_t t16 = ((im.nx + 15) / 16) * (int64_t)((im.ny + 15) / 16) *
+		                    ((im.nz + 15) / 16) * im.nt;
+		if (t16 <= 5459)
+			shape = YRTMI_TILE_16x16x16;
+	}
+	const yrtmi_plan_options o{0, shape, 0};
 	yrtmi_plan* q = nullptr;
 	check(yrtmi_plan_create(ctx.get(), proj.get(), &im, &l, &o, &q), "tile plan");
 	return std::unique_ptr<PlanHandle>(new PlanHandle(q));
@@ -605,7 +616,7 @@ std::unique_ptr<OperatorProjectorMI> OSEM_MI::makeProjector(bool withTOF) const
 
 bool OSEM_MI::planned() const
 {
-	return useTilePlans && projectorParams.projectorType == ProjectorType::SIDDON;
+	return useTilePlans;
 }
 
 ImageParams OSEM_MI::singleFrame() const

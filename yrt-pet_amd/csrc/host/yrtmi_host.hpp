@@ -337,7 +337,8 @@ public:
 	float hardThreshold = DEFAULT_HARD_THRESHOLD;
 	float denomThreshold = DEFAULT_DENOM_THRESHOLD;
 	ProjectorParams projectorParams;
-	// Backprojection through tile plans (Siddon): built once per subset.
+	// Backprojection through tile plans (Siddon and DD): built once per
+	// subset.
 	bool useTilePlans = true;
 	// Project each list-mode subset's events in locality order
 	// (yrtmi_lor_order); the EM update does not depend on the order.
