@@ -1,4 +1,4 @@
-// pybind11 module `pyyrtmi`: the C++ host layer with the names of the
+// pybind11 module `_pyyrtmi` (imported as `pyyrtmi`): the C++ host layer with the names of the
 // reference's pyyrtpet bindings for this path (src/pybind11.cpp:95-177;
 // OSEM: src/recon/OSEM.cpp:37-162; ProjectorParams:
 // src/operators/ProjectorParams.cpp:19-80; OperatorProjector:
@@ -57,12 +57,17 @@ py::array_t<float> imageToNumpy(const ImageDevice& img)
 
 }  // namespace
 
-PYBIND11_MODULE(pyyrtmi, m)
+// file formats (pyyrtmi_io.cpp)
+void bindIO(py::module_& m, py::class_<Scanner>& scanner, py::class_<ImageParams>& imageParams,
+            py::class_<ImageDevice>& image, py::class_<Histogram3DDevice>& histo);
+
+PYBIND11_MODULE(_pyyrtmi, m)
 {
 	m.doc() = "yrtmi: MI355X OSEM projection path (C++ host layer over the yrtmi C ABI)";
 	py::register_exception<Error>(m, "YrtmiError", PyExc_RuntimeError);
 
-	py::class_<Scanner>(m, "Scanner")
+	py::class_<Scanner> scannerCls(m, "Scanner");
+	scannerCls
 	    .def(py::init<std::string, float, float, float, float, float, size_t, size_t,
 	                  size_t, size_t, size_t, size_t>(),
 	         "scannerName"_a, "axialFOV"_a, "crystalSize_z"_a, "crystalSize_trans"_a,
@@ -82,7 +87,8 @@ PYBIND11_MODULE(pyyrtmi, m)
 	    .def_readonly("minAngDiff", &Scanner::minAngDiff)
 	    .def_readonly("detsPerBlock", &Scanner::detsPerBlock);
 
-	py::class_<ImageParams>(m, "ImageParams")
+	py::class_<ImageParams> imageParamsCls(m, "ImageParams");
+	imageParamsCls
 	    .def(py::init<int64_t, int64_t, int64_t, float, float, float, float, float, float,
 	                  int64_t>(),
 	         "nx"_a, "ny"_a, "nz"_a, "length_x"_a, "length_y"_a, "length_z"_a,
@@ -153,8 +159,8 @@ PYBIND11_MODULE(pyyrtmi, m)
 	    .def("histogramShape",
 	         [](const Context& c) { return py::make_tuple(c.numZBin(), c.numPhi(), c.numR()); });
 
-	py::class_<ImageDevice>(m, "ImageDevice")
-	    .def("getParams", &ImageDevice::getParams)
+	py::class_<ImageDevice> imageCls(m, "ImageDevice");
+	imageCls.def("getParams", &ImageDevice::getParams)
 	    .def("getDevicePointer",
 	         [](const ImageDevice& i) { return reinterpret_cast<uintptr_t>(i.getDevicePointer()); })
 	    .def("isAlias", &ImageDevice::isAlias)
@@ -243,8 +249,8 @@ PYBIND11_MODULE(pyyrtmi, m)
 	    py::keep_alive<0, 3>(), py::keep_alive<0, 4>(), py::keep_alive<0, 5>(),
 	    py::keep_alive<0, 6>(), py::keep_alive<0, 7>(), py::keep_alive<0, 8>());
 
-	py::class_<Histogram3DDevice>(m, "Histogram3DDevice")
-	    .def("count", &Histogram3DDevice::count)
+	py::class_<Histogram3DDevice> histoCls(m, "Histogram3DDevice");
+	histoCls.def("count", &Histogram3DDevice::count)
 	    .def("getDevicePointer",
 	         [](const Histogram3DDevice& h) {
 		         return reinterpret_cast<uintptr_t>(h.getDevicePointer());
@@ -370,4 +376,6 @@ PYBIND11_MODULE(pyyrtmi, m)
 	      static_cast<void (*)(const Context&, const ProjectorParams&, ImageDevice&,
 	                           const Histogram3DDevice&)>(&backProject),
 	      "ctx"_a, "proj_params"_a, "img"_a, "proj"_a);
+
+	bindIO(m, scannerCls, imageParamsCls, imageCls, histoCls);
 }

@@ -74,6 +74,15 @@ const std::vector<float>& Scanner::lutOrientations() const
 	return m_ori;
 }
 
+void Scanner::setDetectorLUT(std::vector<float> pos, std::vector<float> ori)
+{
+	if (pos.size() != 3 * getNumDets() || ori.size() != pos.size())
+		throw Error("Scanner: the detector table does not hold num_doi * num_rings * "
+		            "dets_per_ring detectors");
+	m_pos = std::move(pos);
+	m_ori = std::move(ori);
+}
+
 // ---------------------------------------------------------------------------
 // ImageParams
 // ---------------------------------------------------------------------------
@@ -261,7 +270,7 @@ void Context::synchronize() const
 	check(yrtmi_ctx_synchronize(m_ctx), "synchronize");
 }
 
-void Context::uploadMotion(const std::vector<float>& xf)
+void Context::uploadMotion(const std::vector<float>& xf) const
 {
 	if (xf.size() % 12 != 0)
 		throw Error("motion transforms: 12 floats each");
@@ -496,6 +505,10 @@ static void needFrames(const ImageDevice& img, int numFrames)
 void OperatorProjectorMI::applyA(const ImageDevice& img, ListModeDevice& proj)
 {
 	needFrames(img, proj.numFrames);
+	// LORs moved by the list-mode's motion (ProjectionData::getLOR,
+	// ProjectionData.cpp:302-317)
+	if (!proj.motionTransforms.empty())
+		m_ctx.uploadMotion(proj.motionTransforms);
 	const yrtmi_image im = img.getParams().desc();
 	const yrtmi_lors l = proj.lors();
 	check(yrtmi_forward(m_ctx.get(), m_proj, &im, img.getDevicePointer(), &l, proj.values()),
@@ -505,6 +518,8 @@ void OperatorProjectorMI::applyA(const ImageDevice& img, ListModeDevice& proj)
 void OperatorProjectorMI::applyAH(const ListModeDevice& proj, ImageDevice& img)
 {
 	needFrames(img, proj.numFrames);
+	if (!proj.motionTransforms.empty())
+		m_ctx.uploadMotion(proj.motionTransforms);
 	const yrtmi_image im = img.getParams().desc();
 	const yrtmi_lors l = proj.lors();
 	check(yrtmi_back(m_ctx.get(), m_proj, &im, img.getDevicePointer(), &l, proj.values()),

@@ -44,9 +44,12 @@ public:
 	        size_t detsPerRing, size_t numRings, size_t numDOI,
 	        size_t maxRingDiff, size_t minAngDiff, size_t detsPerBlock);
 	size_t getNumDets() const;
-	// Detector LUT (DetRegular::generateLUT), n x 3 each, computed on first use.
+	// Detector LUT, n x 3 each: DetRegular::generateLUT computed on first
+	// use, or a table given by setDetectorLUT (a detector coordinate file,
+	// DetCoord.cpp:263-320).
 	const std::vector<float>& lutPositions() const;
 	const std::vector<float>& lutOrientations() const;
+	void setDetectorLUT(std::vector<float> pos, std::vector<float> ori);
 
 	std::string scannerName;
 	float axialFOV, crystalSize_z, crystalSize_trans, crystalDepth, scannerRadius;
@@ -142,7 +145,9 @@ public:
 	int64_t histogramSize() const { return m_numZBin * m_numPhi * m_numR; }
 	void setStream(void* stream);
 	void synchronize() const;
-	void uploadMotion(const std::vector<float>& transforms);  // k x 12
+	// the motion table of the context's scanner tables (device state, not
+	// this object's)
+	void uploadMotion(const std::vector<float>& transforms) const;  // k x 12
 
 private:
 	const Scanner& m_scanner;
@@ -253,6 +258,7 @@ public:
 	void copyFromHost(const float* src);
 	void copyToHost(float* dst) const;
 	yrtmi_lors lors() const;  // every bin, in bin order
+	const Context& context() const { return m_ctx; }
 
 private:
 	const Context& m_ctx;

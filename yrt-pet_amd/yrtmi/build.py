@@ -3,7 +3,8 @@
     python -m yrtmi.build          (from yrt-pet_amd/)
 
 Produces ``yrtmi/libyrtmi.so`` (HIP kernels + C ABI) and the C++ host layer
-with its pybind11 module ``pyyrtmi`` (``yrt-pet_amd/pyyrtmi*.so``, host code
+with its pybind11 module ``_pyyrtmi`` (``yrt-pet_amd/_pyyrtmi*.so``, imported
+through ``pyyrtmi.py``; host code
 only, linked against libyrtmi.so).  Everything is rebuilt when this runs;
 loaders never build on their own.
 """
@@ -61,7 +62,8 @@ def build_library(verbose=False):
     return out
 
 
-HOST_SOURCES = ["host/yrtmi_host.cpp", "host/pyyrtmi.cpp"]
+HOST_SOURCES = ["host/yrtmi_host.cpp", "host/yrtmi_io.cpp", "host/pyyrtmi.cpp",
+                "host/pyyrtmi_io.cpp"]
 
 
 def build_host_module(verbose=False):
@@ -69,14 +71,14 @@ def build_host_module(verbose=False):
     import sysconfig
     import pybind11
     suffix = sysconfig.get_config_var("EXT_SUFFIX") or ".so"
-    out = os.path.join(PKG_ROOT, "pyyrtmi" + suffix)
+    out = os.path.join(PKG_ROOT, "_pyyrtmi" + suffix)
     tmp = out + ".tmp"
     py_inc = sysconfig.get_paths()["include"]
     cmd = [shutil.which("g++") or "g++", "-O2", "-std=c++17", "-shared", "-fPIC",
            "-fvisibility=hidden", "-Wall", "-Wno-unused-function",
            f"-I{INCLUDE}", f"-I{os.path.join(CSRC, 'host')}",
            f"-I{pybind11.get_include()}", f"-I{py_inc}",
-           *HOST_SOURCES, f"-L{HERE}", "-lyrtmi", "-Wl,-rpath,$ORIGIN/yrtmi",
+           *HOST_SOURCES, f"-L{HERE}", "-lyrtmi", "-lz", "-Wl,-rpath,$ORIGIN/yrtmi",
            "-o", tmp]
     log = _run(cmd)
     os.replace(tmp, out)
