@@ -257,6 +257,19 @@ int yrtmi_threshold(yrtmi_ctx* ctx, float* x, const float* mask, int64_t nvox,
                     int64_t nt, int64_t mask_frames, float thr, float le_scale,
                     float le_off, float gt_scale, float gt_off);
 int yrtmi_fill(yrtmi_ctx* ctx, float* x, int64_t n, float value);
+/* Image-space PSF (OperatorPsf::applyA / applyAH,
+ * src/operators/OperatorPsf.cpp:254-375; the device operator
+ * OperatorPsfDevice): out = the separable circular convolution of `in`
+ * (im->nt frames) with kx along x, then ky along y, then kz along z, each
+ * output sum taken over the taps in order in FP32 (bitwise the reference
+ * CPU loop); transpose != 0 convolves with the flipped kernels (the
+ * adjoint, applyAH).  Kernels (host arrays) of odd length <= 63 whose half
+ * length is below the image size on their axis; `tmp` is a device image
+ * buffer of the same size; in, out and tmp distinct; fewer than 2^31 voxels
+ * in all frames. */
+int yrtmi_psf_apply(yrtmi_ctx* ctx, const yrtmi_image* im, const float* in, float* out,
+                    float* tmp, const float* kx, int nkx, const float* ky, int nky,
+                    const float* kz, int nkz, int transpose);
 
 /* FP64 Siddon projector (no counterpart in the reference, which computes in
  * FP32): the same LORs and rays with every ray quantity in FP64 (detector

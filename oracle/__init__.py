@@ -530,11 +530,53 @@ def attenuation_factors(att_im: ImageDef, att_img, lors: Lors, nthreads=1):
 # --------------------------------------------------------------------------
 # OSEM driver (restated)
 # --------------------------------------------------------------------------
+def psf_convolve(img, kx, ky, kz, transpose=False):
+    """OperatorPsf::convolve (src/operators/OperatorPsf.cpp:290-375): per
+    frame, the separable convolution along x, then y, then z, each output
+    out[i] = sum over kk = -h..h of k[kk + h] * in[circular(i - kk)] in FP32,
+    kk ascending (circular: i - kk wrapped once, Tools.cpp:160-167).
+    transpose: the flipped kernels (applyAH, OperatorPsf.cpp:268-276).
+    np.roll(a, kk)[i] = a[(i - kk) mod n], which is circular(n, i - kk) for
+    |kk| < n.  Pinned by: the identity kernel, a brute-force loop on small
+    images and the adjoint identity (tests/test_oracle_psf.py)."""
+    x = np.asarray(img, np.float32)
+    shape = x.shape
+    x = x.reshape((-1,) + shape[-3:])  # frames, nz, ny, nx
+    ks = [np.asarray(k, np.float32) for k in (kx, ky, kz)]
+    if transpose:
+        ks = [k[::-1].copy() for k in ks]
+    for k, axis in zip(ks, (3, 2, 1)):
+        h = len(k) // 2
+        if len(k) % 2 == 0 or h >= x.shape[axis]:
+            raise ValueError("kernel length must be odd and its half below the image size")
+        acc = np.zeros_like(x)
+        for kk in range(-h, h + 1):
+            acc = acc + k[kk + h] * np.roll(x, kk, axis=axis)
+        x = acc.astype(np.float32)
+    return x.reshape(shape)
+
+
+def gaussian_kernel_1d(sigma, voxel_size, size=0):
+    """OperatorPsf::createGaussianKernel1D (OperatorPsf.cpp:158-185): size
+    (0: rint(5 sigma / v)) taps exp(-0.5 ((i - size/2) v / sigma)^2), FP32,
+    normalised by their FP32 sum."""
+    sigma, v = np.float32(sigma), np.float32(voxel_size)
+    if size == 0:
+        size = int(np.rint(np.float32(5) * sigma / v))
+    x = (np.arange(size) - size // 2).astype(np.float32)
+    k = np.exp(np.float32(-0.5) * ((x * v / sigma) ** 2)).astype(np.float32)
+    tot = np.float32(0)
+    for t in k:
+        tot = np.float32(tot + t)
+    return (k / tot).astype(np.float32)
+
+
 def sensitivity_images(sc: ScannerDef, im: ImageDef, proj: ProjDef,
                        num_subsets: int, listmode: bool, att_im=None,
                        att_img=None, hard_threshold=DEFAULT_HARD_THRESHOLD,
                        global_scale=1.0, dtype=np.float32, nthreads=1,
-                       pos_ori=None, sens_histo=None, invert_sens=False):
+                       pos_ori=None, sens_histo=None, invert_sens=False,
+                       image_psf=None):
     """OSEM::generateSensitivityImagesCore (src/recon/OSEM.cpp:926-986) with
     OSEM_CPU::generateSensitivityImageForCurrentSubset (OSEM_CPU.cpp:127-188):
     backproject the multiplicative correction (acf * sens, 1.0 without
@@ -544,7 +586,10 @@ def sensitivity_images(sc: ScannerDef, im: ImageDef, proj: ProjDef,
     then the hard threshold (voxels <= thr -> 0, others * global scale).
     sens_histo: per-bin normalisation (histogram order), inverted where
     nonzero when invert_sens (Corrector::getSensitivity, Corrector.cpp:
-    295-308); the bin's factor is acf * sens (Corrector_CPU.cpp:17-45)."""
+    295-308); the bin's factor is acf * sens (Corrector_CPU.cpp:17-45).
+    image_psf (kx, ky, kz): the backprojection goes through the PSF's
+    adjoint, then voxels whose unblurred value is <= EPS_FLT are zeroed
+    (OSEM_CPU.cpp:158-165), before the hard threshold."""
     pos, ori = pos_ori if pos_ori is not None else lut(sc)
     S = 1 if listmode else num_subsets
     p = proj.without_tof()
@@ -564,6 +609,9 @@ def sensitivity_images(sc: ScannerDef, im: ImageDef, proj: ProjDef,
             vals = (vals * sf).astype(np.float32)
         img = back(im1, p, L, vals, dtype=dtype, nthreads=nthreads)
         img = img.astype(np.float32) if dtype == np.float32 else img
+        if image_psf is not None:
+            blurred = psf_convolve(img, *image_psf, transpose=True)
+            img = np.where(img <= EPS_FLT, blurred * 0, blurred).astype(np.float32)
         thr_img = img.copy()
         img = np.where(thr_img <= hard_threshold, img * 0, img * global_scale).astype(img.dtype)
         out.append(img)
@@ -585,7 +633,7 @@ def reconstruct(sc: ScannerDef, im: ImageDef, proj: ProjDef, lors: Lors,
                 sens_images, num_iterations: int, num_subsets: int,
                 listmode: bool, meas=None, att=None, rnd=None, scat=None,
                 global_scale=1.0, subset_indices=None, dtype=np.float32,
-                nthreads=1, initial=None, norm=None):
+                nthreads=1, initial=None, norm=None, image_psf=None):
     """OSEM::reconstruct / iterate (src/recon/OSEM.cpp:315-365,788-811) with
     the CPU hooks of OSEM_CPU.cpp:217-403:
       - list-mode sensitivity scaled by 1/S (OSEM.cpp:1007-1035);
@@ -595,7 +643,10 @@ def reconstruct(sc: ScannerDef, im: ImageDef, proj: ProjDef, lors: Lors,
     `lors` holds every event/bin; `subset_indices[s]` selects the subset
     (default: interleaved for list-mode, the caller supplies histogram
     subsets).  `att` is the per-LOR ACF (precomputeAttenuationFactors);
-    `norm` the per-LOR normalisation factor (sensitivity_factors)."""
+    `norm` the per-LOR normalisation factor (sensitivity_factors).
+    image_psf (kx, ky, kz): the forward projection reads PSF(x) and the
+    update image is PSF^T of the backprojected ratios
+    (OSEM_CPU::computeEMUpdateImage, OSEM_CPU.cpp:284-295,377-392)."""
     dt = _dt(dtype)
     S = num_subsets
     sens = [np.asarray(s, dt).reshape(im.nz, im.ny, im.nx) for s in sens_images]
@@ -618,9 +669,12 @@ def reconstruct(sc: ScannerDef, im: ImageDef, proj: ProjDef, lors: Lors,
         for s in range(S):
             idx = subset_indices[s]
             u = np.zeros(im.shape, dt)
-            em_subset(im, x, u, proj, sub_lors[s], take(meas, idx), take(norm, idx),
+            xf = x if image_psf is None else psf_convolve(x, *image_psf)
+            em_subset(im, xf, u, proj, sub_lors[s], take(meas, idx), take(norm, idx),
                       take(att, idx), take(rnd, idx), take(scat, idx),
                       global_scale, DEFAULT_DENOM_THRESHOLD, dt, nthreads)
+            if image_psf is not None:
+                u = psf_convolve(u, *image_psf, transpose=True)
             sidx = 0 if listmode else s
             em_update(x, u, sens[sidx], dt(EPS_FLT))
     return x

@@ -369,7 +369,7 @@ def test_event_frames(tmp_path):
 def test_cli_refuses_options_outside_the_path(tmp_path):
     from yrtmi import cli
     f = scanner_json(str(tmp_path / "s.json"))
-    for argv in (["reconstruct", "-s", f, "-o", "x.nii", "--psf", "k.csv"],
+    for argv in (["reconstruct", "-s", f, "-o", "x.nii", "--varpsf", "k.csv"],
                  ["forward_project", "-s", f, "-i", "x.nii", "-o", "h.rawd", "--sparse", "1"],
                  ["backproject", "-s", f, "-i", "lm", "-o", "x.nii", "-p", "p.json",
                   "--detmask", "m"]):

@@ -314,6 +314,26 @@ PYBIND11_MODULE(_pyyrtmi, m)
 	    .def("setNumRays", &OSEM_MI::setNumRays, "num_rays"_a)
 	    .def("addTOF", &OSEM_MI::addTOF, "tof_width_ps"_a, "tof_num_std"_a)
 	    .def("addProjPSF", &OSEM_MI::addProjPSF, "proj_psf_fname"_a)
+	    .def(
+	        "addImagePSF",
+	        [](OSEM_MI& o, const std::string& f, const std::string& mode) {
+		        if (mode != "UNIFORM")
+			        throw Error("Image-space PSF: only the UNIFORM mode is supported");
+		        o.addImagePSF(f);
+	        },
+	        "image_psf_fname"_a, "image_psf_mode"_a = "UNIFORM")
+	    .def("addUniformGaussianImagePSFFromFWHM", &OSEM_MI::addUniformGaussianImagePSFFromFWHM,
+	         "fwhm_x"_a, "fwhm_y"_a, "fwhm_z"_a, "kernel_size_x"_a = 0, "kernel_size_y"_a = 0,
+	         "kernel_size_z"_a = 0)
+	    .def("addUniformGaussianImagePSFFromSigma",
+	         &OSEM_MI::addUniformGaussianImagePSFFromSigma, "sigma_x"_a, "sigma_y"_a,
+	         "sigma_z"_a, "kernel_size_x"_a = 0, "kernel_size_y"_a = 0, "kernel_size_z"_a = 0)
+	    .def("hasImagePSF", &OSEM_MI::hasImagePSF)
+	    .def("getImagePSFKernels",
+	         [](const OSEM_MI& o) {
+		         return py::make_tuple(o.imagePSFKernel(0), o.imagePSFKernel(1),
+		                               o.imagePSFKernel(2));
+	         })
 	    .def("setGlobalScalingFactor", &OSEM_MI::setGlobalScalingFactor, "global_scale"_a)
 	    .def("setAttenuationImage", &OSEM_MI::setAttenuationImage, "att_image"_a,
 	         py::keep_alive<1, 2>())

@@ -281,6 +281,33 @@ void bindIO(py::module_& m, py::class_<Scanner>& scanner, py::class_<ImageParams
 	    },
 	    "fname"_a);
 
+	// ---- image-space PSF
+	m.def(
+	    "readImagePSF",
+	    [](const std::string& fname) {
+		    std::vector<float> kx, ky, kz;
+		    readImagePsf(fname, kx, ky, kz);
+		    return py::make_tuple(toNumpy(kx), toNumpy(ky), toNumpy(kz));
+	    },
+	    "fname"_a);
+	m.def(
+	    "applyImagePSF",
+	    [](const Context& c, const ImageDevice& in, ImageDevice& out, CArray<float> kx,
+	       CArray<float> ky, CArray<float> kz, bool transpose) {
+		    const ImageParams& p = in.getParams();
+		    if (!p.isSameAs(out.getParams()) || p.nt != out.getParams().nt)
+			    throw Error("applyImagePSF: images of different geometry");
+		    ImageDevice tmp(c, p);
+		    const yrtmi_image ci = p.desc();
+		    check(yrtmi_psf_apply(c.get(), &ci, in.getDevicePointer(), out.getDevicePointer(),
+		                          tmp.getDevicePointer(), kx.data(), (int)kx.size(), ky.data(),
+		                          (int)ky.size(), kz.data(), (int)kz.size(), transpose ? 1 : 0),
+		          "image PSF");
+		    c.synchronize();
+	    },
+	    "ctx"_a, "img_in"_a, "img_out"_a, "kernel_x"_a, "kernel_y"_a, "kernel_z"_a,
+	    "transpose"_a = false);
+
 	// ---- histograms (raw-data array numZBin x numPhi x numR)
 	histo.def(
 	    "readFromFile",

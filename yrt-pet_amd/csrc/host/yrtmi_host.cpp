@@ -1,6 +1,7 @@
 // C++ host layer over the yrtmi C ABI (see yrtmi_host.hpp).  Host code only:
 // every device operation goes through include/yrtmi.h.
 #include "yrtmi_host.hpp"
+#include "yrtmi_io.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -629,6 +630,51 @@ std::unique_ptr<OperatorProjectorMI> OSEM_MI::makeProjector(bool withTOF) const
 	return p;
 }
 
+std::vector<float> gaussianKernel1D(float sigma, float voxSize, size_t size)
+{
+	if (!(sigma >= 0.f))
+		throw Error("Gaussian Sigma or FWHM cannot be null or negative");
+	if (!(voxSize >= 0.f))
+		throw Error("Voxel size cannot be null or negative");
+	if (size == 0)
+		size = (size_t)std::rintf(5.f * sigma / voxSize);
+	std::vector<float> k;
+	for (size_t i = 0; i < size; i++)
+	{
+		const float x = (float)((int)i - (int)(size / 2));
+		k.push_back(std::exp(-0.5f * std::pow(x * voxSize / sigma, 2.f)));
+	}
+	float sum = 0.f;
+	for (float v : k)
+		sum += v;
+	for (float& v : k)
+		v /= sum;
+	return k;
+}
+
+void OSEM_MI::addImagePSF(const std::string& fname)
+{
+	readImagePsf(fname, m_psf[0], m_psf[1], m_psf[2]);
+}
+
+void OSEM_MI::addUniformGaussianImagePSFFromSigma(float sx, float sy, float sz, size_t kx,
+                                                  size_t ky, size_t kz)
+{
+	if (!m_hasImageParams)
+		throw Error("Image parameters not set");
+	m_psf[0] = gaussianKernel1D(sx, m_imageParams.vx, kx);
+	m_psf[1] = gaussianKernel1D(sy, m_imageParams.vy, ky);
+	m_psf[2] = gaussianKernel1D(sz, m_imageParams.vz, kz);
+}
+
+void OSEM_MI::addUniformGaussianImagePSFFromFWHM(float fx, float fy, float fz, size_t kx,
+                                                 size_t ky, size_t kz)
+{
+	// SIGMA_TO_FWHM = 2 sqrt(2 ln 2) (geometry/Constants.hpp)
+	const float s2f = (float)(2.0 * std::sqrt(2.0 * std::log(2.0)));
+	addUniformGaussianImagePSFFromSigma(fx / s2f, fy / s2f, fz / s2f, kx, ky, kz);
+}
+
 bool OSEM_MI::planned() const
 {
 	return useTilePlans;
@@ -705,6 +751,23 @@ const std::vector<std::unique_ptr<ImageDevice>>& OSEM_MI::generateSensitivityIma
 			                        &l, m_att ? &ca : nullptr,
 			                        m_att ? m_att->getDevicePointer() : nullptr, sensPtr),
 			      "sensitivity");
+		}
+		if (hasImagePSF())
+		{
+			// PSF^T of the sensitivity image, zero where it was <= EPS
+			// (OSEM_CPU.cpp:158-165)
+			ImageDevice blurred(*m_ctx, im1), tmp(*m_ctx, im1);
+			check(yrtmi_psf_apply(m_ctx->get(), &ci, img->getDevicePointer(),
+			                      blurred.getDevicePointer(), tmp.getDevicePointer(),
+			                      m_psf[0].data(), (int)m_psf[0].size(), m_psf[1].data(),
+			                      (int)m_psf[1].size(), m_psf[2].data(), (int)m_psf[2].size(),
+			                      1),
+			      "image PSF");
+			check(yrtmi_threshold(m_ctx->get(), blurred.getDevicePointer(),
+			                      img->getDevicePointer(), (int64_t)im1.numVoxels(), 1, 1,
+			                      EPS_FLT, 0.f, 0.f, 1.f, 0.f),
+			      "threshold");
+			img->copyFrom(blurred);
 		}
 		// hard threshold: <= thr -> 0, else * global scale
 		// (applyThreshold, OSEM_CPU.cpp:168-172); the mask is a copy
@@ -906,12 +969,34 @@ std::unique_ptr<ImageDevice> OSEM_MI::reconstruct()
 	prepareSubsets(*proj, imt);
 	ImageDevice u(*m_ctx, imt);
 	const yrtmi_image ci = imt.desc();
+	// image-space PSF buffers: PSF(x), the blurred update image, scratch
+	std::unique_ptr<ImageDevice> xPsf, uPsf, psfTmp;
+	if (hasImagePSF())
+	{
+		xPsf.reset(new ImageDevice(*m_ctx, imt));
+		uPsf.reset(new ImageDevice(*m_ctx, imt));
+		psfTmp.reset(new ImageDevice(*m_ctx, imt));
+	}
+	auto psf = [&](const ImageDevice& in, ImageDevice& out, int transpose) {
+		check(yrtmi_psf_apply(c, &ci, in.getDevicePointer(), out.getDevicePointer(),
+		                      psfTmp->getDevicePointer(), m_psf[0].data(), (int)m_psf[0].size(),
+		                      m_psf[1].data(), (int)m_psf[1].size(), m_psf[2].data(),
+		                      (int)m_psf[2].size(), transpose),
+		      "image PSF");
+	};
 	for (int it = 0; it < num_MLEM_iterations; it++)
 	{
 		for (int s = 0; s < S; s++)
 		{
 			Subset& q = *m_subsets[s];
 			u.setValue(0.f);
+			// the forward projection reads PSF(x) (OSEM_CPU.cpp:284-295)
+			const ImageDevice* xFwd = x.get();
+			if (hasImagePSF())
+			{
+				psf(*x, *xPsf, 0);
+				xFwd = xPsf.get();
+			}
 			yrtmi_corrections cr{};
 			cr.meas = q.meas ? q.meas->data() : nullptr;
 			cr.sens = q.sens ? q.sens->data() : nullptr;
@@ -923,16 +1008,24 @@ std::unique_ptr<ImageDevice> OSEM_MI::reconstruct()
 			if (q.lors.n > 0)
 			{
 				if (q.plan)
-					check(yrtmi_em_subset_planned(c, proj->get(), &ci, x->getDevicePointer(),
+					check(yrtmi_em_subset_planned(c, proj->get(), &ci, xFwd->getDevicePointer(),
 					                              u.getDevicePointer(), &q.lors, &cr, q.plan->p),
 					      "EM subset");
 				else
-					check(yrtmi_em_subset(c, proj->get(), &ci, x->getDevicePointer(),
+					check(yrtmi_em_subset(c, proj->get(), &ci, xFwd->getDevicePointer(),
 					                      u.getDevicePointer(), &q.lors, &cr),
 					      "EM subset");
 			}
+			// the update image is PSF^T of the backprojection
+			// (OSEM_CPU.cpp:377-392)
+			const ImageDevice* uImg = &u;
+			if (hasImagePSF())
+			{
+				psf(u, *uPsf, 1);
+				uImg = uPsf.get();
+			}
 			const ImageDevice& sImg = *sens[isListModeEnabled() ? 0 : s];
-			check(yrtmi_em_update(c, x->getDevicePointer(), u.getDevicePointer(),
+			check(yrtmi_em_update(c, x->getDevicePointer(), uImg->getDevicePointer(),
 			                      sImg.getDevicePointer(), nvox, imt.nt, 1, EPS_FLT),
 			      "EM update");
 		}

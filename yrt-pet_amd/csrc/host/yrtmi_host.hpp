@@ -310,6 +310,23 @@ public:
 	// configuration (OSEM.hpp:57-134)
 	void setImageParams(const ImageParams& p) { m_imageParams = p; m_hasImageParams = true; }
 	const ImageParams& getImageParams() const { return m_imageParams; }
+	// Image-space PSF (OSEM::addImagePSF and the uniform Gaussian forms,
+	// OSEM.cpp:86-101, OSEM_CPU.cpp:47-92; OperatorPsf.cpp:158-255): the
+	// forward projection reads PSF(x), the update image is PSF^T of the
+	// backprojected ratios, and each sensitivity image is PSF^T of the
+	// backprojection (zero where that was <= EPS).  The file holds the x, y
+	// and z taps on its first three rows and their lengths on the fourth.
+	// The Gaussian forms need the image parameters set; a kernel size of 0
+	// takes rint(5 sigma / voxel size) taps.
+	void addImagePSF(const std::string& fname);
+	void addUniformGaussianImagePSFFromFWHM(float fwhmX, float fwhmY, float fwhmZ,
+	                                        size_t kerSizeX = 0, size_t kerSizeY = 0,
+	                                        size_t kerSizeZ = 0);
+	void addUniformGaussianImagePSFFromSigma(float sigmaX, float sigmaY, float sigmaZ,
+	                                         size_t kerSizeX = 0, size_t kerSizeY = 0,
+	                                         size_t kerSizeZ = 0);
+	bool hasImagePSF() const { return !m_psf[0].empty(); }
+	const std::vector<float>& imagePSFKernel(int axis) const { return m_psf[axis]; }
 	void setDataInput(const ListModeDevice* lm);
 	void setDataInput(const Histogram3DDevice* h);
 	bool isListModeEnabled() const { return m_lm != nullptr; }
@@ -361,6 +378,7 @@ private:
 	ImageParams singleFrame() const;
 
 	std::unique_ptr<Context> m_ctx;
+	std::vector<float> m_psf[3];  // image-space PSF taps along x, y, z
 	ImageParams m_imageParams;
 	bool m_hasImageParams = false;
 	const ListModeDevice* m_lm = nullptr;
@@ -375,6 +393,11 @@ private:
 	std::vector<std::unique_ptr<ImageDevice>> m_sens;
 	std::vector<std::unique_ptr<Subset>> m_subsets;
 };
+
+// OperatorPsf::createGaussianKernel1D (OperatorPsf.cpp:158-185): `size`
+// taps (0: rint(5 sigma / voxel size)) exp(-0.5 ((i - size/2) v / sigma)^2),
+// normalised by their sum.
+std::vector<float> gaussianKernel1D(float sigma, float voxelSize, size_t size = 0);
 
 // util::createOSEM(scanner, useGPU) (ReconstructionUtils.cpp:915-933): this
 // library provides the GPU OSEM only; useGPU = false throws.

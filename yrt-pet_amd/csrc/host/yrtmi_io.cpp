@@ -909,6 +909,41 @@ void writeDetCoord(const std::string& fname, const std::vector<float>& pos,
 }
 
 // ---------------------------------------------------------------------------
+// image-space PSF file
+// ---------------------------------------------------------------------------
+void readImagePsf(const std::string& fname, std::vector<float>& kx, std::vector<float>& ky,
+                  std::vector<float>& kz)
+{
+	std::ifstream f(fname);
+	if (!f)
+		throw Error("Error opening file '" + fname + "'");
+	std::vector<std::vector<float>> rows;
+	std::string line;
+	while (std::getline(f, line))
+	{
+		std::replace(line.begin(), line.end(), ',', ' ');
+		std::stringstream ss(line);
+		std::vector<float> row;
+		float v;
+		while (ss >> v)
+			row.push_back(v);
+		rows.push_back(row);
+	}
+	if (rows.size() < 4 || rows[3].size() < 3)
+		throw Error("Image-space PSF file: three kernel rows and a row of sizes expected");
+	std::vector<float>* out[3] = {&kx, &ky, &kz};
+	for (int a = 0; a < 3; a++)
+	{
+		const int n = (int)rows[3][a];
+		if (n <= 0 || n % 2 == 0)
+			throw Error("Kernel size must be odd");
+		if ((int)rows[a].size() < n)
+			throw Error("Image-space PSF file: a kernel row is shorter than its size");
+		out[a]->assign(rows[a].begin(), rows[a].begin() + n);
+	}
+}
+
+// ---------------------------------------------------------------------------
 // motion and dynamic framing
 // ---------------------------------------------------------------------------
 std::vector<MotionRecord> readMotionCsv(const std::string& fname)

@@ -90,6 +90,13 @@ void readDetCoord(const std::string& fname, std::vector<float>& pos, std::vector
 void writeDetCoord(const std::string& fname, const std::vector<float>& pos,
                    const std::vector<float>& ori);
 
+// ---- image-space PSF file (OperatorPsf::readFromFileInternal,
+// OperatorPsf.cpp:197-253) ----------------------------------------------------
+// Rows of comma- or space-separated values: the x, y and z taps, then their
+// three (odd) lengths.
+void readImagePsf(const std::string& fname, std::vector<float>& kx, std::vector<float>& ky,
+                  std::vector<float>& kz);
+
 // ---- motion (LORMotion.cpp:110-190; motion_file.md) -----------------------
 // One line per motion frame: t, r00, r01, r02, tx, r10, r11, r12, ty, r20,
 // r21, r22, tz, e (comma- or space-separated); t the frame's starting

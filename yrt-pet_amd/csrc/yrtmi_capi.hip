@@ -957,6 +957,44 @@ int yrtmi_threshold(yrtmi_ctx* c, float* x, const float* m, int64_t nvox,
 	return YRTMI_OK;
 }
 
+int yrtmi_psf_apply(yrtmi_ctx* c, const yrtmi_image* im, const float* in, float* out,
+                    float* tmp, const float* kx, int nkx, const float* ky, int nky,
+                    const float* kz, int nkz, int transpose)
+{
+	int r = setDev(c);
+	if (r)
+		return r;
+	r = checkImage(im, "image");
+	if (r)
+		return r;
+	if (in == nullptr || out == nullptr || tmp == nullptr || in == out || in == tmp ||
+	    out == tmp)
+		return fail(YRTMI_EINVAL, "psf: in, out and tmp must be distinct device buffers");
+	const int64_t total = im->nx * im->ny * im->nz * im->nt;
+	if (total >= (int64_t(1) << 31))
+		return fail(YRTMI_EINVAL, "psf: 2^31 voxels or more");
+	const float* ks[3] = {kx, ky, kz};
+	const int nk[3] = {nkx, nky, nkz};
+	const int64_t dims[3] = {im->nx, im->ny, im->nz};
+	PsfTaps taps[3];
+	for (int a = 0; a < 3; a++)
+	{
+		if (ks[a] == nullptr || nk[a] <= 0 || nk[a] % 2 == 0 || nk[a] > kMaxPsfTaps)
+			return fail(YRTMI_EINVAL, "psf: kernels must have an odd length <= 63");
+		if (nk[a] / 2 >= dims[a])
+			return fail(YRTMI_EINVAL, "psf: kernel half length must be below the image size");
+		taps[a].n = nk[a];
+		for (int i = 0; i < nk[a]; i++)
+			taps[a].k[i] = transpose ? ks[a][nk[a] - 1 - i] : ks[a][i];
+	}
+	const int nx = (int)im->nx, ny = (int)im->ny, nz = (int)im->nz;
+	// x: in -> out, y: out -> tmp, z: tmp -> out
+	HIP_TRY(launchPsfAxis(0, in, out, taps[0], nx, ny, nz, (int)total, c->stream));
+	HIP_TRY(launchPsfAxis(1, out, tmp, taps[1], nx, ny, nz, (int)total, c->stream));
+	HIP_TRY(launchPsfAxis(2, tmp, out, taps[2], nx, ny, nz, (int)total, c->stream));
+	return YRTMI_OK;
+}
+
 int yrtmi_fill(yrtmi_ctx* c, float* x, int64_t n, float v)
 {
 	int r = setDev(c);

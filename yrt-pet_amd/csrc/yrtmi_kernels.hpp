@@ -1,6 +1,8 @@
 // Launchers of the yrtmi kernels (host side).
 #pragma once
 
+#include <algorithm>
+
 #include "yrtmi_rays.hpp"
 
 namespace yrtmi
@@ -19,6 +21,18 @@ hipError_t launchThreshold(float* x, const float* m, int64_t nvox, int64_t nt,
                            int64_t mFrames, float thr, float les, float leo,
                            float gts, float gto, hipStream_t st);
 hipError_t launchFill(float* x, int64_t n, float v, hipStream_t st);
+
+// Image-space PSF: one axis (0 x, 1 y, 2 z) of the separable circular
+// convolution of every frame (nx * ny * nz * frames = total < 2^31 voxels),
+// taps passed by value.
+constexpr int kMaxPsfTaps = 63;
+struct PsfTaps
+{
+	float k[kMaxPsfTaps];
+	int n;  // odd, <= kMaxPsfTaps
+};
+hipError_t launchPsfAxis(int axis, const float* in, float* out, const PsfTaps& t, int nx,
+                         int ny, int nz, int total, hipStream_t st);
 
 // Histogram subset geometry for the bin-list kernels.
 struct SubsetGeom

@@ -31,7 +31,7 @@ HIP_FLAGS = [
 ]
 
 LIB_SOURCES = ["yrtmi_kernels.hip", "yrtmi_tiles.hip", "yrtmi_f64.hip", "yrtmi_order.hip",
-               "yrtmi_capi.hip"]
+               "yrtmi_image.hip", "yrtmi_capi.hip"]
 
 
 def hipcc():

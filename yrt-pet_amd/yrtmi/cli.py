@@ -10,8 +10,8 @@ Files: scanner JSON (--scanner), image-parameters JSON (--params), NIfTI
 images (.nii / .nii.gz), raw-data histograms (format H), list-mode files
 (format LM with --flag_tof / --flag_randoms), motion CSV (--lor_motion) and
 dynamic framing (--dyn).  Options of the reference outside this path
-(image-space PSF, detector masks, sparse histograms, DOI list-mode, in-vivo
-or hardware attenuation, ACF histograms, LR updaters) are refused with a
+(variant image-space PSF, detector masks, sparse histograms, DOI list-mode,
+in-vivo or hardware attenuation, ACF histograms, LR updaters) are refused with a
 message rather than ignored; --gpu and --num_threads are accepted (the
 projections always run on the GPU).
 """
@@ -20,7 +20,7 @@ import sys
 
 import numpy as np
 
-UNSUPPORTED = ["psf", "varpsf", "detmask", "att_invivo", "acf_invivo", "att_hardware",
+UNSUPPORTED = ["varpsf", "detmask", "att_invivo", "acf_invivo", "att_hardware",
                "acf_hardware", "acf", "projector_updater", "sparse", "move_sens",
                "save_iter_step", "save_iter_ranges"]
 
@@ -43,6 +43,7 @@ def _common(p, need_params=False):
     p.add_argument("--tof_width_ps", type=float, default=0.0, help="TOF width in picoseconds")
     p.add_argument("--tof_n_std", type=int, default=5,
                    help="Number of standard deviations for TOF's Gaussian. Default: 5")
+    p.add_argument("--psf", default="", help="Image-space PSF kernel file")
     for name in UNSUPPORTED:
         p.add_argument("--" + name, default=None, help=argparse.SUPPRESS)
 
@@ -170,6 +171,8 @@ def reconstruct(argv):
     else:
         raise SystemExit("--params is required when no sensitivity image is given")
     osem.setImageParams(params)
+    if a.psf:
+        osem.addImagePSF(a.psf)
     if a.input:
         data, lm = _load_input(Y, ctx, sc, a)
         keep += [data, lm]
@@ -227,6 +230,11 @@ def forward_project(argv):
     ctx = Y.Context(sc)
     pp = _projector_params(Y, sc, a)
     img = Y.ImageDeviceFromFile(ctx, a.input)
+    if a.psf:
+        # the image-space PSF applied to the image first (ForwardProject.cpp)
+        blurred = Y.ImageDeviceOwned(ctx, img.getParams())
+        Y.applyImagePSF(ctx, img, blurred, *Y.readImagePSF(a.psf))
+        img = blurred
     h = Y.Histogram3DDeviceOwned(ctx)
     Y.forwProject(ctx, pp, img, h)
     ctx.synchronize()
@@ -270,6 +278,12 @@ def backproject(argv):
     img.setValue(0.0)
     Y.backProject(ctx, pp, img, data)
     ctx.synchronize()
+    if a.psf:
+        # the adjoint of the image-space PSF after the backprojection
+        # (Backproject.cpp)
+        blurred = Y.ImageDeviceOwned(ctx, img.getParams())
+        Y.applyImagePSF(ctx, img, blurred, *Y.readImagePSF(a.psf), transpose=True)
+        img = blurred
     img.writeToFile(a.out)
     return 0
 

@@ -539,6 +539,22 @@ class Context:
         capi.check(capi.lib().yrtmi_fill(self.h, _ptr(x), x.numel(), float(v)))
         return x
 
+    def psf(self, im, x, out, tmp, kx, ky, kz, transpose=False):
+        """out = image-space PSF of x (separable circular convolution with
+        kx, ky, kz; the adjoint with transpose).  tmp: a scratch image."""
+        for t, nm in ((x, "x"), (out, "out"), (tmp, "tmp")):
+            _check_dtype(t, torch.float32, nm)
+            if t.numel() != im.nvox * im.nt:
+                raise ValueError(f"psf: {nm} size mismatch")
+        ks = [np.ascontiguousarray(k, np.float32) for k in (kx, ky, kz)]
+        ci = im.c()
+        f = C.POINTER(C.c_float)
+        capi.check(capi.lib().yrtmi_psf_apply(
+            self.h, C.byref(ci), _ptr(x), _ptr(out), _ptr(tmp),
+            ks[0].ctypes.data_as(f), len(ks[0]), ks[1].ctypes.data_as(f), len(ks[1]),
+            ks[2].ctypes.data_as(f), len(ks[2]), 1 if transpose else 0))
+        return out
+
     def lor_order(self, im, lors):
         """Locality order of `lors` (int32 tensor of the indices 0..n-1):
         sorted by frame, axial midpoint, tilt, transaxial angle and offset,
