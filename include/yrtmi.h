@@ -270,6 +270,21 @@ int yrtmi_fill(yrtmi_ctx* ctx, float* x, int64_t n, float value);
 int yrtmi_psf_apply(yrtmi_ctx* ctx, const yrtmi_image* im, const float* in, float* out,
                     float* tmp, const float* kx, int nkx, const float* ky, int nky,
                     const float* kz, int nkz, int transpose);
+/* Time-averaged moved image: one launch of timeAverageMoveImage_kernel<true>
+ * (src/datastruct/image/ImageSpaceKernels.cu:191-264) as
+ * util::timeAverageMoveImageDevice drives it (src/utils/
+ * ReconstructionUtilsDevice.cu:146-225).  For every voxel v of one frame,
+ * out[v] += sum over f of weights[f] * trilinear(in, inverse(xf[f]) *
+ * centre(v)), the frames in order.  in: frame 0 of an image of `im`'s
+ * geometry; out: one frame (offset by the caller to its dynamic frame),
+ * accumulated into; in and out distinct device buffers.  xf: n host
+ * transforms (12 floats each, transform_t's r00 r01 r02 tx r10 ... tz), the
+ * motion itself: the library inverts them as util::invertTransform;
+ * weights: n host floats (duration / scan duration in the reference
+ * driver).  Stages the frames into a context buffer after synchronising
+ * the context stream; not under graph capture. */
+int yrtmi_time_average_move(yrtmi_ctx* ctx, const yrtmi_image* im, const float* in,
+                            float* out, const float* xf, const float* weights, int n);
 
 /* FP64 Siddon projector (no counterpart in the reference, which computes in
  * FP32): the same LORs and rays with every ray quantity in FP64 (detector

@@ -571,6 +571,158 @@ def gaussian_kernel_1d(sigma, voxel_size, size=0):
     return (k / tot).astype(np.float32)
 
 
+# --------------------------------------------------------------------------
+# Motion: time-averaged (moved) images
+# --------------------------------------------------------------------------
+def _fmaf(a, b, c):
+    """fmaf in float32 through float64: a*b of two floats is exact in
+    float64, the sum is rounded to float64 and then to float32 (a double
+    rounding that can differ from one true fmaf by one ulp, rarely)."""
+    return (np.float64(1.0) * np.asarray(a, np.float32) * np.asarray(b, np.float32)
+            + np.asarray(c, np.float32)).astype(np.float32)
+
+
+def invert_transform(xf):
+    """util::invertTransform (include/yrt-pet/geometry/TransformUtils.hpp:
+    16-40): rigid inverse [R^T | -R^T t] of transform_t rows (r00 r01 r02 tx
+    r10 ... tz), in FP32, each dot product summed left to right.
+    Pinned by: composition with the forward transform is the identity on
+    points (tests/test_oracle_motion.py)."""
+    f = np.float32
+    a = np.asarray(xf, f).reshape(-1, 12)
+    R = a[:, [0, 1, 2, 4, 5, 6, 8, 9, 10]].reshape(-1, 3, 3)
+    t = a[:, [3, 7, 11]]
+    RT = np.transpose(R, (0, 2, 1))
+    out = np.empty_like(a)
+    for i in range(3):
+        ti = -((RT[:, i, 0] * t[:, 0] + RT[:, i, 1] * t[:, 1]) + RT[:, i, 2] * t[:, 2])
+        out[:, 4 * i:4 * i + 3] = RT[:, i, :]
+        out[:, 4 * i + 3] = ti.astype(f)
+    return out.reshape(np.shape(xf))
+
+
+def full_time_range(timestamps):
+    """util::getFullTimeRange (src/utils/ReconstructionUtils.cpp:468-495):
+    [first start, first start + 1] for one frame, else [first start, last
+    start + the second-to-last frame's duration]."""
+    t = [int(v) for v in timestamps]
+    if not t:
+        raise ValueError("no motion frames")
+    if len(t) == 1:
+        return t[0], t[0] + 1
+    return t[0], t[-1] + (t[-1] - t[-2])
+
+
+def motion_duration(timestamps, frame):
+    """LORMotion::getDuration (src/datastruct/projection/LORMotion.cpp:
+    73-88): the next frame's start minus this one's; the last frame takes
+    the second-to-last frame's duration.  As float."""
+    t = [int(v) for v in timestamps]
+    last = len(t) - 1
+    if frame < last:
+        return np.float32(t[frame + 1] - t[frame])
+    if frame == last:
+        return np.float32(t[last] - t[last - 1])
+    raise IndexError("Frame index out of range")
+
+
+def time_average_frames(timestamps, xf, t0, t1):
+    """The frames and weights of a time average over [t0, t1]
+    (timeAverageMoveImageDevice, src/utils/ReconstructionUtilsDevice.cu:
+    146-190): the frames whose start lies in [t0, t1] (both ends included),
+    each weighted duration / float(t1 - t0), with its inverted transform.
+    Returns (frame indices, inverse transforms (k,12), weights (k,))."""
+    if not t1 > t0:
+        raise ValueError("Time stop must be greater than time start")
+    t = [int(v) for v in timestamps]
+    frames = [f for f in range(len(t)) if t0 <= t[f] <= t1]
+    # the loop stops at the first start beyond t1; starts are increasing
+    stop = next((f for f in range(len(t)) if t[f] >= t0 and t[f] > t1), len(t))
+    frames = [f for f in frames if f < stop]
+    if not frames:
+        raise ValueError("no motion frame starts inside the time range")
+    scan = np.float32(t1 - t0)
+    w = np.array([motion_duration(t, f) / scan for f in frames], np.float32)
+    inv = invert_transform(np.asarray(xf, np.float32).reshape(-1, 12)[frames])
+    return np.array(frames), inv, w
+
+
+def _image_geometry(im: ImageDef):
+    f = np.float32
+    n = (im.nx, im.ny, im.nz)
+    L = im.lengths
+    off = (f(im.ox), f(im.oy), f(im.oz))
+    v = tuple(f(L[a] / f(n[a])) for a in range(3))
+    inv = tuple(f(f(1) / v[a]) for a in range(3))
+    origin = tuple(f(f(off[a] - f(L[a] * f(0.5))) + f(v[a] * f(0.5))) for a in range(3))
+    # util::indexToPosition (ImageUtils.cuh:96-101): ((i v - L/2) + off) + v/2
+    pos = tuple(((np.arange(n[a], dtype=f) * v[a] - f(f(0.5) * L[a])) + off[a])
+                + f(f(0.5) * v[a]) for a in range(3))
+    return n, inv, origin, pos
+
+
+def move_accumulate(img, im: ImageDef, inv_xf, weights, out):
+    """timeAverageMoveImage_kernel<WEIGHED_AVG=true> (src/datastruct/image/
+    ImageSpaceKernels.cu:191-264) with util::trilinearInterpolateCore
+    (include/yrt-pet/datastruct/image/ImageUtils.cuh:15-70): for every
+    output voxel centre p and every frame, q = inv * p (nested fmaf as the
+    kernel), c = (q - origin) * inv_v, i0 = floor(c), d = c - i0, the 8
+    corner weights (1-dx)(1-dy)(1-dz) ... (corners outside the image get
+    weight 0), and out += in[corner] * w * frame_weight (one fmaf), corners
+    in the kernel's order, frames in order.  img: (nz, ny, nx) frame 0 of
+    the input; out: (nz, ny, nx), accumulated in place.
+    Pinned by: identity and whole-voxel shifts reproduce the image, a 90
+    degree turn is rot90, a random rigid motion equals scipy's order-1
+    map_coordinates with zero padding, linearity in the frame weights
+    (tests/test_oracle_motion.py)."""
+    f = np.float32
+    src = np.ascontiguousarray(img, f).reshape(-1)
+    (nx, ny, nz), inv_v, origin, pos = _image_geometry(im)
+    Z, Y, X = np.meshgrid(pos[2], pos[1], pos[0], indexing="ij")
+    acc = np.ascontiguousarray(out, f).reshape(nz, ny, nx).copy()
+    for r, fw in zip(np.asarray(inv_xf, f).reshape(-1, 12), np.asarray(weights, f)):
+        q = [_fmaf(r[4 * a], X, _fmaf(r[4 * a + 1], Y, _fmaf(r[4 * a + 2], Z, r[4 * a + 3])))
+             for a in range(3)]
+        c = [((q[a] - origin[a]) * inv_v[a]).astype(f) for a in range(3)]
+        i0 = [np.floor(c[a]).astype(np.int64) for a in range(3)]
+        d = [(c[a] - i0[a].astype(f)).astype(f) for a in range(3)]
+        one = f(1)
+        for k in range(8):
+            ox, oy, oz = k & 1, (k >> 1) & 1, (k >> 2) & 1
+            wx = d[0] if ox else one - d[0]
+            wy = d[1] if oy else one - d[1]
+            wz = d[2] if oz else one - d[2]
+            w = ((wx * wy) * wz).astype(f)
+            x, y, z = i0[0] + ox, i0[1] + oy, i0[2] + oz
+            ok = (x >= 0) & (x < nx) & (y >= 0) & (y < ny) & (z >= 0) & (z < nz)
+            idx = np.where(ok, x + y * nx + z * nx * ny, 0)
+            w = np.where(ok, w, f(0))
+            acc = _fmaf((src[idx] * w).astype(f), fw, acc)
+    return acc
+
+
+def time_average_move(img, im: ImageDef, timestamps, xf, t0=None, t1=None):
+    """util::timeAverageMoveImage(Device) (src/utils/ReconstructionUtils.cpp:
+    497-592, ReconstructionUtilsDevice.cu:78-225): a zeroed output plus the
+    weighted moved copies of frame 0 of `img` over [t0, t1] (default: the
+    motion's full time range).  Returns (nz, ny, nx)."""
+    if t0 is None:
+        t0, t1 = full_time_range(timestamps)
+    _, inv, w = time_average_frames(timestamps, xf, t0, t1)
+    img = np.asarray(img, np.float32)
+    base = img.reshape((-1,) + img.shape[-3:])[0]
+    return move_accumulate(base, im, inv, w, np.zeros_like(base))
+
+
+def time_average_move_dynamic(img, im: ImageDef, timestamps, xf, bounds):
+    """util::timeAverageMoveImageDynamic (ReconstructionUtils.cpp:600-652):
+    one time average per dynamic frame f over [bounds[f], bounds[f+1]].
+    Returns (len(bounds) - 1, nz, ny, nx)."""
+    b = [int(v) for v in bounds]
+    return np.stack([time_average_move(img, im, timestamps, xf, b[f], b[f + 1])
+                     for f in range(len(b) - 1)])
+
+
 def sensitivity_images(sc: ScannerDef, im: ImageDef, proj: ProjDef,
                        num_subsets: int, listmode: bool, att_im=None,
                        att_img=None, hard_threshold=DEFAULT_HARD_THRESHOLD,

@@ -217,6 +217,48 @@ void bindIO(py::module_& m, py::class_<Scanner>& scanner, py::class_<ImageParams
 		    return toNumpy(motionFrames(toVector(ts), records(t, xf, err)));
 	    },
 	    "event_timestamps"_a, "timestamps"_a, "transforms"_a, "errors"_a);
+	// ---- time-averaged (moved) images: the motion as its timestamps and
+	// k x 12 transforms (readLORMotion's first two arrays)
+	auto xfVec = [](CArray<int64_t> t, CArray<float> xf) {
+		if ((size_t)xf.size() != 12 * (size_t)t.size())
+			throw py::value_error("motion: need k timestamps and k x 12 transforms");
+		return std::make_pair(toVector(t), toVector(xf));
+	};
+	m.def(
+	    "getFullTimeRange", [](CArray<int64_t> t) { return getFullTimeRange(toVector(t)); },
+	    "timestamps"_a);
+	m.def(
+	    "timeAverageMoveImage",
+	    [xfVec](CArray<int64_t> t, CArray<float> xf, const ImageDevice& img, py::object t0,
+	            py::object t1) {
+		    auto [ts, x] = xfVec(t, xf);
+		    if (t0.is_none() != t1.is_none())
+			    throw py::value_error("give both time_start and time_stop, or neither");
+		    if (t0.is_none())
+			    return timeAverageMoveImageDevice(ts, x, img).release();
+		    return timeAverageMoveImageDevice(ts, x, img, t0.cast<int64_t>(), t1.cast<int64_t>())
+		        .release();
+	    },
+	    "timestamps"_a, "transforms"_a, "unmoved_image"_a, "time_start"_a = py::none(),
+	    "time_stop"_a = py::none(), py::keep_alive<0, 3>());
+	m.def(
+	    "timeAverageMoveImageInto",
+	    [xfVec](CArray<int64_t> t, CArray<float> xf, const ImageDevice& img, ImageDevice& out,
+	            int64_t t0, int64_t t1, int64_t frame) {
+		    auto [ts, x] = xfVec(t, xf);
+		    timeAverageMoveImageDevice(ts, x, img, out, t0, t1, frame);
+	    },
+	    "timestamps"_a, "transforms"_a, "unmoved_image"_a, "out_image"_a, "time_start"_a,
+	    "time_stop"_a, "out_dynamic_frame"_a = 0);
+	m.def(
+	    "timeAverageMoveImageDynamic",
+	    [xfVec](CArray<int64_t> t, CArray<float> xf, const ImageDevice& img,
+	            CArray<uint32_t> bounds) {
+		    auto [ts, x] = xfVec(t, xf);
+		    return timeAverageMoveImageDynamicDevice(ts, x, img, toVector(bounds)).release();
+	    },
+	    "timestamps"_a, "transforms"_a, "unmoved_image"_a, "dynamic_framing"_a,
+	    py::keep_alive<0, 3>());
 	m.def(
 	    "readDynamicFraming", [](const std::string& f) { return toNumpy(readDynamicFraming(f)); },
 	    "fname"_a);

@@ -630,6 +630,105 @@ std::unique_ptr<OperatorProjectorMI> OSEM_MI::makeProjector(bool withTOF) const
 	return p;
 }
 
+// ---- motion: time-averaged (moved) images ---------------------------------
+std::pair<int64_t, int64_t> getFullTimeRange(const std::vector<int64_t>& t)
+{
+	if (t.empty())
+		throw Error("getFullTimeRange: no motion frames");
+	if (t.size() == 1)
+		return {t[0], t[0] + 1};
+	const size_t last = t.size() - 1;
+	return {t[0], t[last] + (t[last] - t[last - 1])};
+}
+
+void timeAverageMoveImageDevice(const std::vector<int64_t>& t, const std::vector<float>& xf,
+                                const ImageDevice& unmoved, ImageDevice& out,
+                                int64_t timeStart, int64_t timeStop, int64_t outFrame)
+{
+	const size_t k = t.size();
+	if (xf.size() != 12 * k)
+		throw Error("timeAverageMoveImage: need 12 transform values per motion frame");
+	if (k == 0)
+		throw Error("timeAverageMoveImage: no motion frames");
+	if (!(timeStop > timeStart))
+		throw Error("Time stop must be greater than time start");
+	const ImageParams& p = unmoved.getParams();
+	const ImageParams& q = out.getParams();
+	if (p.nx != q.nx || p.ny != q.ny || p.nz != q.nz)
+		throw Error("timeAverageMoveImage: input and output image sizes differ");
+	if (outFrame < 0 || outFrame >= q.nt)
+		throw Error("timeAverageMoveImage: output frame out of range");
+	// ReconstructionUtilsDevice.cu:146-166: the frames from the first start
+	// >= timeStart up to (excluding) the first start > timeStop
+	int64_t f0 = -1, f1 = (int64_t)k;
+	for (size_t f = 0; f < k; f++)
+	{
+		if (t[f] >= timeStart)
+		{
+			if (f0 < 0)
+				f0 = (int64_t)f;
+			if (t[f] > timeStop)
+			{
+				f1 = (int64_t)f;
+				break;
+			}
+		}
+	}
+	if (f0 < 0 || f1 <= f0)
+		throw Error("timeAverageMoveImage: no motion frame starts inside the time range");
+	const float scan = (float)(timeStop - timeStart);
+	const int n = (int)(f1 - f0);
+	std::vector<float> w(n);
+	for (int i = 0; i < n; i++)
+	{
+		// LORMotion::getDuration (LORMotion.cpp:73-88)
+		const size_t f = (size_t)(f0 + i);
+		const int64_t d = f + 1 < k ? t[f + 1] - t[f] : t[f] - t[f - 1];
+		w[i] = (float)d / scan;
+	}
+	yrtmi_image d = p.desc(1);
+	check(yrtmi_time_average_move(unmoved.context().get(), &d, unmoved.getDevicePointer(),
+	                              out.getDevicePointer() + outFrame * (int64_t)q.numVoxels(),
+	                              xf.data() + 12 * f0, w.data(), n),
+	      "timeAverageMoveImage");
+}
+
+std::unique_ptr<ImageDevice> timeAverageMoveImageDevice(const std::vector<int64_t>& t,
+                                                        const std::vector<float>& xf,
+                                                        const ImageDevice& unmoved,
+                                                        int64_t timeStart, int64_t timeStop)
+{
+	ImageParams p = unmoved.getParams();
+	p.nt = 1;
+	auto out = std::make_unique<ImageDevice>(unmoved.context(), p);
+	out->setValue(0.f);
+	timeAverageMoveImageDevice(t, xf, unmoved, *out, timeStart, timeStop, 0);
+	return out;
+}
+
+std::unique_ptr<ImageDevice> timeAverageMoveImageDevice(const std::vector<int64_t>& t,
+                                                        const std::vector<float>& xf,
+                                                        const ImageDevice& unmoved)
+{
+	const auto r = getFullTimeRange(t);
+	return timeAverageMoveImageDevice(t, xf, unmoved, r.first, r.second);
+}
+
+std::unique_ptr<ImageDevice> timeAverageMoveImageDynamicDevice(
+    const std::vector<int64_t>& t, const std::vector<float>& xf, const ImageDevice& unmoved,
+    const std::vector<uint32_t>& bounds)
+{
+	if (bounds.size() < 2)
+		throw Error("timeAverageMoveImageDynamic: the framing needs at least one frame");
+	ImageParams p = unmoved.getParams();
+	p.nt = (int64_t)bounds.size() - 1;
+	auto out = std::make_unique<ImageDevice>(unmoved.context(), p);
+	out->setValue(0.f);
+	for (int64_t f = 0; f < p.nt; f++)
+		timeAverageMoveImageDevice(t, xf, unmoved, *out, bounds[f], bounds[f + 1], f);
+	return out;
+}
+
 std::vector<float> gaussianKernel1D(float sigma, float voxSize, size_t size)
 {
 	if (!(sigma >= 0.f))

@@ -21,6 +21,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace yrtmi
@@ -188,6 +189,7 @@ public:
 	// ImageDeviceAlias: wraps `devicePointer` (nt * nz * ny * nx floats).
 	ImageDevice(const Context& ctx, const ImageParams& params, float* devicePointer);
 	const ImageParams& getParams() const { return m_params; }
+	const Context& context() const { return m_ctx; }
 	float* getDevicePointer() const { return m_buf->data(); }
 	bool isAlias() const { return m_buf->isAlias(); }
 	size_t size() const { return m_buf->size(); }
@@ -398,6 +400,38 @@ private:
 // taps (0: rint(5 sigma / voxel size)) exp(-0.5 ((i - size/2) v / sigma)^2),
 // normalised by their sum.
 std::vector<float> gaussianKernel1D(float sigma, float voxelSize, size_t size = 0);
+
+// ---- motion: time-averaged (moved) images --------------------------------
+// The motion is given as its frames' starting timestamps (ms, increasing)
+// and transforms (k x 12, transform_t rows r00 r01 r02 tx r10 ... tz), the
+// fields of a LORMotion.
+// util::getFullTimeRange (ReconstructionUtils.cpp:468-495): [first start,
+// first start + 1] for one frame, else [first start, last start + the
+// second-to-last frame's duration].
+std::pair<int64_t, int64_t> getFullTimeRange(const std::vector<int64_t>& timestamps);
+// util::timeAverageMoveImageDevice (ReconstructionUtilsDevice.cu:120-225):
+// adds to frame `outFrame` of `out` the copies of frame 0 of `unmoved`
+// moved by every motion frame starting in [timeStart, timeStop] (both
+// included), weighted by LORMotion::getDuration / (timeStop - timeStart).
+void timeAverageMoveImageDevice(const std::vector<int64_t>& timestamps,
+                                const std::vector<float>& transforms,
+                                const ImageDevice& unmoved, ImageDevice& out,
+                                int64_t timeStart, int64_t timeStop, int64_t outFrame = 0);
+// The same into a new zeroed one-frame image (ReconstructionUtilsDevice.cu:
+// 97-118); without a time range over getFullTimeRange.
+std::unique_ptr<ImageDevice> timeAverageMoveImageDevice(const std::vector<int64_t>& timestamps,
+                                                        const std::vector<float>& transforms,
+                                                        const ImageDevice& unmoved);
+std::unique_ptr<ImageDevice> timeAverageMoveImageDevice(const std::vector<int64_t>& timestamps,
+                                                        const std::vector<float>& transforms,
+                                                        const ImageDevice& unmoved,
+                                                        int64_t timeStart, int64_t timeStop);
+// util::timeAverageMoveImageDynamicDevice (ReconstructionUtilsDevice.cu:
+// 227-300): one average per dynamic frame f over [bounds[f], bounds[f+1]]
+// into frame f of a new image of bounds.size() - 1 frames.
+std::unique_ptr<ImageDevice> timeAverageMoveImageDynamicDevice(
+    const std::vector<int64_t>& timestamps, const std::vector<float>& transforms,
+    const ImageDevice& unmoved, const std::vector<uint32_t>& bounds);
 
 // util::createOSEM(scanner, useGPU) (ReconstructionUtils.cpp:915-933): this
 // library provides the GPU OSEM only; useGPU = false throws.

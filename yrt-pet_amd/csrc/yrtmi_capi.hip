@@ -54,6 +54,9 @@ struct yrtmi_ctx
 	uint32_t* zbin = nullptr;
 	float* motion = nullptr;
 	int nmotion = 0;
+	// frames of yrtmi_time_average_move (4 float4 each), grown on demand
+	float4* moveFrames = nullptr;
+	int moveCap = 0;
 	unsigned long long* counter = nullptr;
 	// per-XCD partial images for backprojection (nxcc x partStride floats),
 	// kept zeroed between uses
@@ -300,6 +303,7 @@ int yrtmi_ctx_destroy(yrtmi_ctx* c)
 	(void)hipFree(c->lut);
 	(void)hipFree(c->zbin);
 	(void)hipFree(c->motion);
+	(void)hipFree(c->moveFrames);
 	(void)hipFree(c->counter);
 	(void)hipFree(c->part);
 	if (c->owned != nullptr)
@@ -992,6 +996,82 @@ int yrtmi_psf_apply(yrtmi_ctx* c, const yrtmi_image* im, const float* in, float*
 	HIP_TRY(launchPsfAxis(0, in, out, taps[0], nx, ny, nz, (int)total, c->stream));
 	HIP_TRY(launchPsfAxis(1, out, tmp, taps[1], nx, ny, nz, (int)total, c->stream));
 	HIP_TRY(launchPsfAxis(2, tmp, out, taps[2], nx, ny, nz, (int)total, c->stream));
+	return YRTMI_OK;
+}
+
+// util::invertTransform (TransformUtils.hpp:16-40): [R^T | -R^T t], each
+// dot product summed left to right in FP32.
+static void invertRigid(const float* t, float* o)
+{
+#pragma clang fp contract(off)
+	const float R[9] = {t[0], t[1], t[2], t[4], t[5], t[6], t[8], t[9], t[10]};
+	const float tr[3] = {t[3], t[7], t[11]};
+	for (int i = 0; i < 3; i++)
+	{
+		const float a = R[i], b = R[3 + i], c = R[6 + i];  // row i of R^T
+		o[4 * i] = a;
+		o[4 * i + 1] = b;
+		o[4 * i + 2] = c;
+		o[4 * i + 3] = -(a * tr[0] + b * tr[1] + c * tr[2]);
+	}
+}
+
+int yrtmi_time_average_move(yrtmi_ctx* c, const yrtmi_image* im, const float* in,
+                            float* out, const float* xf, const float* weights, int n)
+{
+	int r = setDev(c);
+	if (r)
+		return r;
+	r = checkImage(im, "image");
+	if (r)
+		return r;
+	r = notCapturing(c, "time_average_move");
+	if (r)
+		return r;
+	if (in == nullptr || out == nullptr || in == out)
+		return fail(YRTMI_EINVAL, "time_average_move: in and out must be distinct device buffers");
+	if (xf == nullptr || weights == nullptr || n <= 0 || n > (1 << 24))
+		return fail(YRTMI_EINVAL, "time_average_move: need 1..2^24 transforms and weights");
+	if (im->nx * im->ny * im->nz >= (int64_t(1) << 31))
+		return fail(YRTMI_EINVAL, "time_average_move: 2^31 voxels or more");
+	std::vector<float4> fr((size_t)4 * n);
+	for (int f = 0; f < n; f++)
+	{
+		float inv[12];
+		invertRigid(xf + 12 * (size_t)f, inv);
+		for (int i = 0; i < 3; i++)
+			fr[4 * f + i] = make_float4(inv[4 * i], inv[4 * i + 1], inv[4 * i + 2], inv[4 * i + 3]);
+		fr[4 * f + 3] = make_float4(weights[f], 0.f, 0.f, 0.f);
+	}
+	// the buffer may still be read by the previous call's kernel
+	HIP_TRY(hipStreamSynchronize(c->stream));
+	if (n > c->moveCap)
+	{
+		HIP_TRY(hipFree(c->moveFrames));
+		c->moveFrames = nullptr;
+		c->moveCap = 0;
+		HIP_TRY(hipMalloc(&c->moveFrames, sizeof(float4) * 4 * (size_t)n));
+		c->moveCap = n;
+	}
+	HIP_TRY(hipMemcpy(c->moveFrames, fr.data(), sizeof(float4) * fr.size(),
+	                  hipMemcpyHostToDevice));
+	MoveGeom g{};
+	const int64_t dims[3] = {im->nx, im->ny, im->nz};
+	const float len[3] = {im->lx, im->ly, im->lz};
+	const float off[3] = {im->ox, im->oy, im->oz};
+	for (int a = 0; a < 3; a++)
+	{
+		// timeAverageMoveImage_kernel: v = length / n, origin = off - L/2 + v/2
+		g.v[a] = len[a] / (float)dims[a];
+		g.len[a] = len[a];
+		g.off[a] = off[a];
+		g.inv[a] = 1.f / g.v[a];
+		g.origin[a] = (off[a] - len[a] * 0.5f) + g.v[a] * 0.5f;
+	}
+	g.nx = (int)im->nx;
+	g.ny = (int)im->ny;
+	g.nz = (int)im->nz;
+	HIP_TRY(launchTimeAverageMove(in, out, c->moveFrames, n, g, c->stream));
 	return YRTMI_OK;
 }
 

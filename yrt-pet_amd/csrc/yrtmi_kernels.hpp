@@ -34,6 +34,19 @@ struct PsfTaps
 hipError_t launchPsfAxis(int axis, const float* in, float* out, const PsfTaps& t, int nx,
                          int ny, int nz, int total, hipStream_t st);
 
+// Time-averaged moved image: out[v] += sum over frames of weight *
+// trilinear(in, inverse transform * centre of v), for one frame of
+// nx * ny * nz < 2^31 voxels.  `frames` holds 4 float4 per frame: the rows
+// (r_i0, r_i1, r_i2, t_i) of the inverse transform, then {weight, 0, 0, 0}.
+struct MoveGeom
+{
+	float v[3], len[3], off[3];  // voxel size (length / n), length, offset
+	float origin[3], inv[3];     // centre of voxel 0, 1 / voxel size
+	int nx, ny, nz;
+};
+hipError_t launchTimeAverageMove(const float* in, float* out, const float4* frames, int n,
+                                 const MoveGeom& g, hipStream_t st);
+
 // Histogram subset geometry for the bin-list kernels.
 struct SubsetGeom
 {

@@ -21,7 +21,7 @@ import sys
 import numpy as np
 
 UNSUPPORTED = ["varpsf", "detmask", "att_invivo", "acf_invivo", "att_hardware",
-               "acf_hardware", "acf", "projector_updater", "sparse", "move_sens",
+               "acf_hardware", "acf", "projector_updater", "sparse",
                "save_iter_step", "save_iter_ranges"]
 
 
@@ -123,6 +123,8 @@ def reconstruct(argv):
     p.add_argument("--out_sens", default="", help="Sensitivity image output filename")
     p.add_argument("--sens", default="", help="Sensitivity image files (comma-separated)")
     p.add_argument("--sens_only", action="store_true", help="Only generate sensitivity image(s)")
+    p.add_argument("--move_sens", action="store_true",
+                   help="Move the provided sensitivity image based on motion")
     p.add_argument("--sensitivity", default="", help="Sensitivity histogram file")
     p.add_argument("--invert_sensitivity", action="store_true",
                    help="Invert the sensitivity histogram values")
@@ -200,13 +202,38 @@ def reconstruct(argv):
         x0 = Y.ImageDeviceFromFile(ctx, a.initial_estimate)
         keep.append(x0)
         osem.setInitialEstimate(x0)
-    if sens_imgs:
-        osem.setSensitivityImages(sens_imgs)
+    given = bool(sens_imgs)
+    if given:
+        move = a.move_sens
     else:
-        imgs = osem.generateSensitivityImages()
+        sens_imgs = osem.generateSensitivityImages()
+        move = True
         if a.out_sens:
-            for i, img in enumerate(imgs):
-                img.writeToFile(_sens_name(a.out_sens, i, len(imgs)))
+            for i, img in enumerate(sens_imgs):
+                img.writeToFile(_sens_name(a.out_sens, i, len(sens_imgs)))
+    if a.lor_motion and move:
+        # Reconstruct.cpp:556-660: the sensitivity image time-averaged over
+        # the motion: per dynamic frame with --dyn, else over the list-mode's
+        # first to last timestamp, else over the motion's full time range
+        if len(sens_imgs) != 1:
+            raise SystemExit("moving the sensitivity needs exactly one sensitivity image")
+        t, xf, _ = Y.readLORMotion(a.lor_motion)
+        if a.dyn:
+            moved = Y.timeAverageMoveImageDynamic(t, xf, sens_imgs[0],
+                                                  Y.readDynamicFraming(a.dyn))
+        elif a.input and a.format.upper() == "LM":
+            ts = keep[1].getTimestampArray()
+            moved = Y.timeAverageMoveImage(t, xf, sens_imgs[0], int(ts[0]), int(ts[-1]))
+        else:
+            moved = Y.timeAverageMoveImage(t, xf, sens_imgs[0])
+        sens_imgs = [moved]
+        keep.append(moved)
+        if a.out_sens:
+            moved.writeToFile(a.out_sens)
+        given = True
+    if given:
+        # setSensitivityImages copies; the generated images are the OSEM's own
+        osem.setSensitivityImages(sens_imgs)
     if a.sens_only:
         return 0
     x = osem.reconstruct()

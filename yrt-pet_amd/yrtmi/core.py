@@ -555,6 +555,25 @@ class Context:
             ks[2].ctypes.data_as(f), len(ks[2]), 1 if transpose else 0))
         return out
 
+    def time_average_move(self, im, x, out, transforms, weights):
+        """out (one frame of im) += sum over frames f of weights[f] * x
+        (its frame 0) moved by transforms[f] (k x 12 transform_t rows; the
+        library inverts them) with trilinear interpolation."""
+        _check_dtype(x, torch.float32, "x")
+        _check_dtype(out, torch.float32, "out")
+        if x.numel() < im.nvox or out.numel() < im.nvox:
+            raise ValueError("time_average_move: x and out need one frame each")
+        xf = np.ascontiguousarray(transforms, np.float32).reshape(-1, 12)
+        w = np.ascontiguousarray(weights, np.float32).reshape(-1)
+        if len(w) != len(xf):
+            raise ValueError("time_average_move: one weight per transform")
+        ci = im.c()
+        f = C.POINTER(C.c_float)
+        capi.check(capi.lib().yrtmi_time_average_move(
+            self.h, C.byref(ci), _ptr(x), _ptr(out), xf.ctypes.data_as(f),
+            w.ctypes.data_as(f), len(w)))
+        return out
+
     def lor_order(self, im, lors):
         """Locality order of `lors` (int32 tensor of the indices 0..n-1):
         sorted by frame, axial midpoint, tilt, transaxial angle and offset,
