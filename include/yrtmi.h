@@ -102,7 +102,11 @@ typedef struct
 
 /* Per-LOR correction factors for the fused EM kernel (OSEM_CPU.cpp:329-374,
  * OSEM_GPU.cu:1159-1341).  DEVICE pointers, all nullable:
- * meas NULL = list-mode measurement 1.0 (ListMode.cpp:52-55). */
+ * meas NULL = list-mode measurement 1.0 (ListMode.cpp:52-55).  The
+ * denominator is ((A x) * sens * att * global_scale + randoms + scatter) *
+ * precorr, precorr the in-vivo attenuation precorrection factor
+ * (ATTENUATION_PRECORRECTION, OSEM_GPU.cu:1303-1308); the backprojected
+ * ratio is multiplied by sens * att * global_scale. */
 typedef struct
 {
 	const float* meas;
@@ -112,6 +116,7 @@ typedef struct
 	const float* scatter;
 	float global_scale;
 	float denom_threshold;
+	const float* precorr;
 } yrtmi_corrections;
 
 /* Regular cylindrical scanner (Scanner ctor, src/datastruct/scanner/

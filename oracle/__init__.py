@@ -83,8 +83,8 @@ def lib():
         _LIB.yo_tof_alpha_range.argtypes = [C.c_float, C.c_int, C.c_float,
                                             C.c_float, C.c_void_p, C.c_void_p]
         _LIB.yo_tof_helper.argtypes = [C.c_float, C.c_int] + [C.c_void_p] * 3
-        _LIB.yo_em_subset_f32.argtypes = [C.c_void_p] * 10 + [C.c_float, C.c_float, C.c_int]
-        _LIB.yo_em_subset_f64.argtypes = [C.c_void_p] * 10 + [C.c_double, C.c_double, C.c_int]
+        _LIB.yo_em_subset_f32.argtypes = [C.c_void_p] * 11 + [C.c_float, C.c_float, C.c_int]
+        _LIB.yo_em_subset_f64.argtypes = [C.c_void_p] * 11 + [C.c_double, C.c_double, C.c_int]
         _LIB.yo_em_update_f32.argtypes = [C.c_void_p] * 3 + [C.c_int64, C.c_float]
         _LIB.yo_em_update_f64.argtypes = [C.c_void_p] * 3 + [C.c_int64, C.c_double]
         _LIB.yo_apply_threshold_f32.argtypes = [C.c_void_p] * 2 + [C.c_int64] + [C.c_float] * 5
@@ -477,17 +477,20 @@ def count_segments(im: ImageDef, proj: ProjDef, lors: Lors, nthreads=1):
 
 def em_subset(im, x, u, proj, lors, meas=None, sens=None, att=None, rnd=None,
               scat=None, global_scale=1.0, denom_thr=DEFAULT_DENOM_THRESHOLD,
-              dtype=np.float32, nthreads=1):
+              dtype=np.float32, nthreads=1, precorr=None):
     """OSEM_CPU::computeEMUpdateImage (src/recon/OSEM_CPU.cpp:284-393),
-    accumulating A^T(m / (A x)) into u (in place)."""
+    accumulating A^T(m / (A x)) into u (in place).  precorr: the in-vivo
+    attenuation precorrection, multiplying the denominator after the
+    additive terms (OSEM_GPU.cu:1303-1308).  Pinned by: precorr p equals
+    measurements m / p (tests/test_oracle_attenuation.py)."""
     dt = _dt(dtype)
     assert x.dtype == dt and u.dtype == dt and u.flags.c_contiguous
     cv = lambda a: None if a is None else np.ascontiguousarray(a, dt)
-    meas, sens, att, rnd, scat = map(cv, (meas, sens, att, rnd, scat))
+    meas, sens, att, rnd, scat, precorr = map(cv, (meas, sens, att, rnd, scat, precorr))
     ci, cp, cl = im.c(), proj.c(), lors.c()
     f = lib().yo_em_subset_f64 if dt == np.float64 else lib().yo_em_subset_f32
     f(C.byref(ci), _p(x), _p(u), C.byref(cp), C.byref(cl), _p(meas),
-      _p(sens), _p(att), _p(rnd), _p(scat), global_scale, denom_thr, nthreads)
+      _p(sens), _p(att), _p(rnd), _p(scat), _p(precorr), global_scale, denom_thr, nthreads)
     return u
 
 
@@ -728,7 +731,7 @@ def sensitivity_images(sc: ScannerDef, im: ImageDef, proj: ProjDef,
                        att_img=None, hard_threshold=DEFAULT_HARD_THRESHOLD,
                        global_scale=1.0, dtype=np.float32, nthreads=1,
                        pos_ori=None, sens_histo=None, invert_sens=False,
-                       image_psf=None):
+                       image_psf=None, acf_histo=None):
     """OSEM::generateSensitivityImagesCore (src/recon/OSEM.cpp:926-986) with
     OSEM_CPU::generateSensitivityImageForCurrentSubset (OSEM_CPU.cpp:127-188):
     backproject the multiplicative correction (acf * sens, 1.0 without
@@ -741,7 +744,9 @@ def sensitivity_images(sc: ScannerDef, im: ImageDef, proj: ProjDef,
     295-308); the bin's factor is acf * sens (Corrector_CPU.cpp:17-45).
     image_psf (kx, ky, kz): the backprojection goes through the PSF's
     adjoint, then voxels whose unblurred value is <= EPS_FLT are zeroed
-    (OSEM_CPU.cpp:158-165), before the hard threshold."""
+    (OSEM_CPU.cpp:158-165), before the hard threshold.  acf_histo: a
+    hardware ACF histogram (histogram order), used per bin in place of the
+    attenuation image (generateSensImage_kernel, OSEM_GPU.cu:1060-1090)."""
     pos, ori = pos_ori if pos_ori is not None else lut(sc)
     S = 1 if listmode else num_subsets
     p = proj.without_tof()
@@ -752,7 +757,9 @@ def sensitivity_images(sc: ScannerDef, im: ImageDef, proj: ProjDef,
         bins = histo_subset_bins(sc, S, s)
         d1, d2 = histo_pairs(sc, bins)
         L = lors_from_pairs(pos, ori, d1, d2)
-        if att_img is not None:
+        if acf_histo is not None:
+            vals = np.ascontiguousarray(np.asarray(acf_histo, np.float32).reshape(-1)[bins])
+        elif att_img is not None:
             vals = attenuation_factors(att_im, att_img, L, nthreads)
         else:
             vals = np.ones(L.n, np.float32)
@@ -785,7 +792,7 @@ def reconstruct(sc: ScannerDef, im: ImageDef, proj: ProjDef, lors: Lors,
                 sens_images, num_iterations: int, num_subsets: int,
                 listmode: bool, meas=None, att=None, rnd=None, scat=None,
                 global_scale=1.0, subset_indices=None, dtype=np.float32,
-                nthreads=1, initial=None, norm=None, image_psf=None):
+                nthreads=1, initial=None, norm=None, image_psf=None, precorr=None):
     """OSEM::reconstruct / iterate (src/recon/OSEM.cpp:315-365,788-811) with
     the CPU hooks of OSEM_CPU.cpp:217-403:
       - list-mode sensitivity scaled by 1/S (OSEM.cpp:1007-1035);
@@ -824,7 +831,8 @@ def reconstruct(sc: ScannerDef, im: ImageDef, proj: ProjDef, lors: Lors,
             xf = x if image_psf is None else psf_convolve(x, *image_psf)
             em_subset(im, xf, u, proj, sub_lors[s], take(meas, idx), take(norm, idx),
                       take(att, idx), take(rnd, idx), take(scat, idx),
-                      global_scale, DEFAULT_DENOM_THRESHOLD, dt, nthreads)
+                      global_scale, DEFAULT_DENOM_THRESHOLD, dt, nthreads,
+                      precorr=take(precorr, idx))
             if image_psf is not None:
                 u = psf_convolve(u, *image_psf, transpose=True)
             sidx = 0 if listmode else s

@@ -917,7 +917,7 @@ int backImpl(const yo_image* im, T* img, const yo_projector* pr,
 template <class T>
 int emSubsetImpl(const yo_image* im, const T* x, T* u, const yo_projector* pr,
                  const yo_lors* L, const T* meas, const T* sens, const T* att,
-                 const T* rnd, const T* scat, T g, T thr, int nthreads)
+                 const T* rnd, const T* scat, const T* pre, T g, T thr, int nthreads)
 {
 	Projector<T> P(im, pr);
 	const bool atomic = nthreads > 1;
@@ -937,6 +937,9 @@ int emSubsetImpl(const yo_image* im, const T* x, T* u, const yo_projector* pr,
 			            update += rnd[i];
 		            if (scat)
 			            update += scat[i];
+		            // in-vivo attenuation precorrection (OSEM_GPU.cu:1303-1308)
+		            if (pre)
+			            update *= pre[i];
 		            if (std::abs(update) > thr)
 		            {
 			            const T m = meas ? meas[i] : T(1);
@@ -1337,19 +1340,19 @@ int yo_count_segments(const yo_image* im, const yo_projector* pr,
 int yo_em_subset_f32(const yo_image* im, const float* x, float* u,
                      const yo_projector* pr, const yo_lors* lors,
                      const float* meas, const float* sens, const float* att,
-                     const float* rnd, const float* scat, float g, float thr,
-                     int nthreads)
+                     const float* rnd, const float* scat, const float* pre, float g,
+                     float thr, int nthreads)
 {
-	return emSubsetImpl<float>(im, x, u, pr, lors, meas, sens, att, rnd, scat,
+	return emSubsetImpl<float>(im, x, u, pr, lors, meas, sens, att, rnd, scat, pre,
 	                           g, thr, nthreads);
 }
 int yo_em_subset_f64(const yo_image* im, const double* x, double* u,
                      const yo_projector* pr, const yo_lors* lors,
                      const double* meas, const double* sens, const double* att,
-                     const double* rnd, const double* scat, double g,
-                     double thr, int nthreads)
+                     const double* rnd, const double* scat, const double* pre,
+                     double g, double thr, int nthreads)
 {
-	return emSubsetImpl<double>(im, x, u, pr, lors, meas, sens, att, rnd, scat,
+	return emSubsetImpl<double>(im, x, u, pr, lors, meas, sens, att, rnd, scat, pre,
 	                            g, thr, nthreads);
 }
 

@@ -122,16 +122,18 @@ int yo_count_segments(const yo_image* im, const yo_projector* pr,
 /* ---- OSEM pieces ---- */
 /* EM update image for one subset (OSEM_CPU::computeEMUpdateImage,
  * src/recon/OSEM_CPU.cpp:284-393).  meas/sens/att/rand/scat are per-LOR,
- * nullable (meas == NULL means list-mode measurement 1.0). */
+ * nullable (meas == NULL means list-mode measurement 1.0); pre (nullable)
+ * the in-vivo attenuation precorrection multiplying the denominator after
+ * the additive terms (OSEM_GPU.cu:1303-1308). */
 int yo_em_subset_f32(const yo_image* im, const float* x, float* u,
                      const yo_projector* pr, const yo_lors* lors,
                      const float* meas, const float* sens, const float* att,
-                     const float* rand, const float* scat, float global_scale,
-                     float denom_thr, int nthreads);
+                     const float* rand, const float* scat, const float* pre,
+                     float global_scale, float denom_thr, int nthreads);
 int yo_em_subset_f64(const yo_image* im, const double* x, double* u,
                      const yo_projector* pr, const yo_lors* lors,
                      const double* meas, const double* sens, const double* att,
-                     const double* rand, const double* scat,
+                     const double* rand, const double* scat, const double* pre,
                      double global_scale, double denom_thr, int nthreads);
 /* Image::updateEMThresholdStatic (src/datastruct/image/Image.cpp:808-833). */
 int yo_em_update_f32(float* x, const float* u, const float* s, int64_t n,

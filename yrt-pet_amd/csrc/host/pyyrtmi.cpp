@@ -337,6 +337,16 @@ PYBIND11_MODULE(_pyyrtmi, m)
 	    .def("setGlobalScalingFactor", &OSEM_MI::setGlobalScalingFactor, "global_scale"_a)
 	    .def("setAttenuationImage", &OSEM_MI::setAttenuationImage, "att_image"_a,
 	         py::keep_alive<1, 2>())
+	    .def("setHardwareAttenuationImage", &OSEM_MI::setHardwareAttenuationImage,
+	         "att_hardware"_a, py::keep_alive<1, 2>())
+	    .def("setInVivoAttenuationImage", &OSEM_MI::setInVivoAttenuationImage,
+	         "att_invivo"_a, py::keep_alive<1, 2>())
+	    .def("setACFHistogram", &OSEM_MI::setACFHistogram, "acf_his"_a,
+	         py::keep_alive<1, 2>())
+	    .def("setHardwareACFHistogram", &OSEM_MI::setHardwareACFHistogram,
+	         "acf_hardware_his"_a, py::keep_alive<1, 2>())
+	    .def("setInVivoACFHistogram", &OSEM_MI::setInVivoACFHistogram, "acf_invivo_his"_a,
+	         py::keep_alive<1, 2>())
 	    .def("setRandomsHistogram", &OSEM_MI::setRandomsHistogram, "randoms_his"_a,
 	         py::keep_alive<1, 2>())
 	    .def("setScatterHistogram", &OSEM_MI::setScatterHistogram, "scatter_his"_a,

@@ -592,7 +592,7 @@ struct OSEM_MI::Subset
 {
 	yrtmi_lors lors{};
 	std::unique_ptr<DeviceArray<uint32_t>> d1, d2, bins;
-	std::unique_ptr<DeviceArray<float>> tof, meas, att, sens, randoms, scatter;
+	std::unique_ptr<DeviceArray<float>> tof, meas, att, sens, randoms, scatter, pre;
 	std::unique_ptr<DeviceArray<int32_t>> frame;
 	std::unique_ptr<DeviceArray<uint8_t>> motion;
 	std::unique_ptr<PlanHandle> plan;
@@ -799,6 +799,51 @@ void OSEM_MI::setSensitivityImages(const std::vector<const ImageDevice*>& imgs)
 	}
 }
 
+// y[i] *= x[i]: an EM update with a denominator of ones, y * (x / 1)
+static void multiplyInto(const Context& ctx, const float* x, float* y, int64_t n)
+{
+	if (n <= 0)
+		return;
+	DeviceArray<float> ones(ctx, (size_t)n);
+	check(yrtmi_fill(ctx.get(), ones.data(), n, 1.f), "fill");
+	check(yrtmi_em_update(ctx.get(), y, x, ones.data(), n, 1, 1, 0.f), "product");
+}
+
+// Corrector::setup (Corrector.cpp:34-166), without changing the setters'
+// values: which histogram or image gives the total, hardware and in-vivo
+// attenuation.
+OSEM_MI::Attenuation OSEM_MI::resolveAttenuation()
+{
+	Attenuation a{m_att, m_hwAtt, m_ivAtt, m_acf, m_hwAcf, m_ivAcf, false};
+	if (!a.acf && !a.ivAcf && a.hwAcf)
+		a.acf = a.hwAcf;  // all ACF is hardware ACF
+	else if (!a.acf && a.ivAcf && !a.hwAcf)
+		a.acf = a.ivAcf;  // all ACF is in-vivo ACF
+	else if (a.acf && !a.ivAcf && !a.hwAcf)
+		a.hwAcf = a.acf;  // a total alone: all hardware (no motion)
+	if (a.inVivo && a.hardware && !a.total)
+	{
+		// the total is the sum of the two images
+		if (!a.hardware->getParams().isSameAs(a.inVivo->getParams()))
+			throw Error("Parameters mismatch between attenuation images");
+		const ImageParams& p = a.hardware->getParams();
+		m_impliedAtt.reset(new ImageDevice(*m_ctx, p));
+		m_impliedAtt->copyFrom(*a.hardware);
+		check(yrtmi_axpby(m_ctx->get(), (int64_t)(p.numVoxels() * (size_t)p.nt), 1.f,
+		                  a.inVivo->getDevicePointer(), 1.f, m_impliedAtt->getDevicePointer()),
+		      "implied attenuation");
+		a.total = m_impliedAtt.get();
+	}
+	else if (!a.inVivo && a.hardware && !a.total)
+		a.total = a.hardware;
+	else if (a.inVivo && !a.hardware && !a.total)
+		a.total = a.inVivo;
+	else if (!a.inVivo && !a.hardware && a.total)
+		a.hardware = a.total;
+	a.totalFromHisto = a.acf != nullptr || (a.hwAcf != nullptr && a.ivAcf != nullptr);
+	return a;
+}
+
 const std::vector<std::unique_ptr<ImageDevice>>& OSEM_MI::generateSensitivityImages()
 {
 	if (!m_hasImageParams)
@@ -808,9 +853,13 @@ const std::vector<std::unique_ptr<ImageDevice>>& OSEM_MI::generateSensitivityIma
 	const int S = getExpectedSensImagesAmount();
 	// sensitivity projector: TOF off (OSEM_CPU.cpp:94-108)
 	auto proj = makeProjector(false);
+	// hardware attenuation: its ACF histogram, else its image
+	// (generateSensImage_kernel, OSEM_GPU.cu:1048-1110)
+	const Attenuation at = resolveAttenuation();
+	const ImageDevice* hw = at.hwAcf ? nullptr : at.hardware;
 	yrtmi_image ca{};
-	if (m_att != nullptr)
-		ca = m_att->getParams().desc(1);
+	if (hw != nullptr)
+		ca = hw->getParams().desc(1);
 	m_sens.clear();
 	for (int s = 0; s < S; s++)
 	{
@@ -834,21 +883,34 @@ const std::vector<std::unique_ptr<ImageDevice>>& OSEM_MI::generateSensitivityIma
 			      "sensitivity factors");
 			sensPtr = sensv->data();
 		}
+		std::unique_ptr<DeviceArray<float>> acfv;
+		if (at.hwAcf != nullptr && l.n > 0)
+		{
+			// sensitivity x hardware ACF per bin
+			acfv.reset(new DeviceArray<float>(*m_ctx, (size_t)l.n));
+			check(yrtmi_histo_subset_gather(m_ctx->get(), at.hwAcf->getDevicePointer(), S, s, 0,
+			                                acfv->data()),
+			      "hardware ACF");
+			if (sensv)
+				multiplyInto(*m_ctx, acfv->data(), sensv->data(), l.n);
+			else
+				sensPtr = acfv->data();
+		}
 		if (planned())
 		{
 			auto plan = makePlan(*m_ctx, *proj, ci, l);
 			check(yrtmi_sensitivity_planned(m_ctx->get(), proj->get(), &ci,
 			                                img->getDevicePointer(), &l,
-			                                m_att ? &ca : nullptr,
-			                                m_att ? m_att->getDevicePointer() : nullptr,
+			                                hw ? &ca : nullptr,
+			                                hw ? hw->getDevicePointer() : nullptr,
 			                                sensPtr, plan->p),
 			      "sensitivity");
 		}
 		else
 		{
 			check(yrtmi_sensitivity(m_ctx->get(), proj->get(), &ci, img->getDevicePointer(),
-			                        &l, m_att ? &ca : nullptr,
-			                        m_att ? m_att->getDevicePointer() : nullptr, sensPtr),
+			                        &l, hw ? &ca : nullptr,
+			                        hw ? hw->getDevicePointer() : nullptr, sensPtr),
 			      "sensitivity");
 		}
 		if (hasImagePSF())
@@ -886,12 +948,34 @@ void OSEM_MI::prepareSubsets(OperatorProjectorMI& proj, const ImageParams& imt)
 	m_subsets.clear();
 	const int S = num_OSEM_subsets;
 	const yrtmi_image ci = imt.desc();
-	yrtmi_image ca{};
-	if (m_att != nullptr)
-		ca = m_att->getParams().desc(1);
+	// total attenuation in the model, in-vivo attenuation as precorrection
+	// (precomputeAttenuationFactors / precomputeInVivoAttenuationFactors,
+	// Corrector_GPU.cu:19-116)
+	const Attenuation at = resolveAttenuation();
+	const ImageDevice* totImg = at.totalFromHisto ? nullptr : at.total;
+	const ImageDevice* ivImg = at.ivAcf ? nullptr : at.inVivo;
+	yrtmi_image ca{}, cv{};
+	if (totImg != nullptr)
+		ca = totImg->getParams().desc(1);
+	if (ivImg != nullptr)
+		cv = ivImg->getParams().desc(1);
 	yrtmi_ctx* c = m_ctx->get();
+	auto imageFactors = [&](std::unique_ptr<DeviceArray<float>>& dst, const ImageDevice* img,
+	                        const yrtmi_image& d, const yrtmi_lors& lors, int64_t n) {
+		if (img == nullptr || n == 0)
+			return;
+		dst.reset(new DeviceArray<float>(*m_ctx, (size_t)n));
+		yrtmi_lors la = lors;
+		la.tof = nullptr;
+		la.frame = nullptr;
+		check(yrtmi_attenuation_factors(c, &d, img->getDevicePointer(), &la, dst->data()),
+		      "attenuation factors");
+	};
 	if (m_lm != nullptr)
 	{
+		if (at.totalFromHisto || at.ivAcf != nullptr)
+			throw Error("ACF histograms apply to histogram inputs only; give attenuation "
+			            "images for list-mode data");
 		if (!m_lm->motionTransforms.empty())
 			m_ctx->uploadMotion(m_lm->motionTransforms);
 		const int64_t N = (int64_t)m_lm->count();
@@ -949,17 +1033,9 @@ void OSEM_MI::prepareSubsets(OperatorProjectorMI& proj, const ImageParams& imt)
 				q->lors.frame = q->frame ? q->frame->data() : nullptr;
 				q->lors.motion_frame = q->motion ? q->motion->data() : nullptr;
 			}
-			if (m_att != nullptr && n > 0)
-			{
-				// per-event attenuation factors, once (precomputeAttenuationFactors)
-				q->att.reset(new DeviceArray<float>(*m_ctx, (size_t)n));
-				yrtmi_lors la = q->lors;
-				la.tof = nullptr;
-				la.frame = nullptr;
-				check(yrtmi_attenuation_factors(c, &ca, m_att->getDevicePointer(), &la,
-				                                q->att->data()),
-				      "attenuation factors");
-			}
+			// per-event attenuation factors, once (precomputeAttenuationFactors)
+			imageFactors(q->att, totImg, ca, q->lors, n);
+			imageFactors(q->pre, ivImg, cv, q->lors, n);
 			m_subsets.push_back(std::move(q));
 		}
 	}
@@ -994,13 +1070,26 @@ void OSEM_MI::prepareSubsets(OperatorProjectorMI& proj, const ImageParams& imt)
 			gatherHisto(q->sens, m_sensHisto, m_invertSens);
 			gatherHisto(q->randoms, m_randomsHisto, false);
 			gatherHisto(q->scatter, m_scatterHisto, false);
-			if (m_att != nullptr && n > 0)
+			if (at.totalFromHisto && n > 0)
 			{
-				q->att.reset(new DeviceArray<float>(*m_ctx, (size_t)n));
-				check(yrtmi_attenuation_factors(c, &ca, m_att->getDevicePointer(), &q->lors,
-				                                q->att->data()),
-				      "attenuation factors");
+				// Corrector::getTotalACFFromHistogram: the total ACF, or
+				// in-vivo x hardware (Corrector.cpp:585-599)
+				if (at.acf != nullptr)
+					gatherHisto(q->att, at.acf, false);
+				else
+				{
+					std::unique_ptr<DeviceArray<float>> hwv;
+					gatherHisto(q->att, at.ivAcf, false);
+					gatherHisto(hwv, at.hwAcf, false);
+					multiplyInto(*m_ctx, hwv->data(), q->att->data(), n);
+				}
 			}
+			else
+				imageFactors(q->att, totImg, ca, q->lors, n);
+			if (at.ivAcf != nullptr)
+				gatherHisto(q->pre, at.ivAcf, false);
+			else
+				imageFactors(q->pre, ivImg, cv, q->lors, n);
 			m_subsets.push_back(std::move(q));
 		}
 	}
@@ -1102,6 +1191,7 @@ std::unique_ptr<ImageDevice> OSEM_MI::reconstruct()
 			cr.att = q.att ? q.att->data() : nullptr;
 			cr.randoms = q.randoms ? q.randoms->data() : nullptr;
 			cr.scatter = q.scatter ? q.scatter->data() : nullptr;
+			cr.precorr = q.pre ? q.pre->data() : nullptr;
 			cr.global_scale = m_globalScale;
 			cr.denom_threshold = denomThreshold;
 			if (q.lors.n > 0)

@@ -338,6 +338,19 @@ public:
 	void addProjPSF(const std::string& fname) { projectorParams.setProjPsf(fname); }
 	void setGlobalScalingFactor(float g) { m_globalScale = g; }
 	void setAttenuationImage(const ImageDevice* mu) { m_att = mu; }
+	// The attenuation split (Corrector.cpp:34-166, 191-221): hardware
+	// attenuation weighs the sensitivity images, the total attenuation the
+	// EM model, the in-vivo attenuation precorrects the EM denominator.  Each
+	// comes from an image (exp(-0.1 * Siddon(mu)) per LOR) or from an ACF
+	// histogram; missing parts are implied as Corrector::setup does (a total
+	// alone is all hardware, hardware + in-vivo images sum to the total, in-vivo
+	// x hardware ACF histograms multiply to the total).  ACF histograms apply
+	// to histogram inputs (and to every sensitivity image).
+	void setHardwareAttenuationImage(const ImageDevice* mu) { m_hwAtt = mu; }
+	void setInVivoAttenuationImage(const ImageDevice* mu) { m_ivAtt = mu; }
+	void setACFHistogram(const Histogram3DDevice* h) { m_acf = h; }
+	void setHardwareACFHistogram(const Histogram3DDevice* h) { m_hwAcf = h; }
+	void setInVivoACFHistogram(const Histogram3DDevice* h) { m_ivAcf = h; }
 	void setRandomsHistogram(const Histogram3DDevice* h) { m_randomsHisto = h; }
 	void setScatterHistogram(const Histogram3DDevice* h) { m_scatterHisto = h; }
 	void setSensitivityHistogram(const Histogram3DDevice* h) { m_sensHisto = h; }
@@ -387,6 +400,16 @@ private:
 	const Histogram3DDevice* m_histo = nullptr;
 	float m_globalScale = 1.0f;
 	const ImageDevice* m_att = nullptr;
+	const ImageDevice *m_hwAtt = nullptr, *m_ivAtt = nullptr;
+	const Histogram3DDevice *m_acf = nullptr, *m_hwAcf = nullptr, *m_ivAcf = nullptr;
+	std::unique_ptr<ImageDevice> m_impliedAtt;  // hardware + in-vivo images
+	struct Attenuation
+	{
+		const ImageDevice *total, *hardware, *inVivo;
+		const Histogram3DDevice *acf, *hwAcf, *ivAcf;
+		bool totalFromHisto;
+	};
+	Attenuation resolveAttenuation();
 	const Histogram3DDevice *m_randomsHisto = nullptr, *m_scatterHisto = nullptr,
 	                        *m_sensHisto = nullptr;
 	bool m_invertSens = false;

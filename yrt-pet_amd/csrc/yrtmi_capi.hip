@@ -860,6 +860,7 @@ int yrtmi_em_subset(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
 		a.c.scatter = corr->scatter;
 		a.c.g = corr->global_scale;
 		a.c.denomThr = corr->denom_threshold;
+		a.c.precorr = corr->precorr;
 	}
 	const int64_t ny = a.im.nvox * a.im.nt;
 	r = setupAccum(c, p, a, ny);
@@ -1517,6 +1518,7 @@ static void setCorr(KArgs& a, const yrtmi_corrections* corr)
 		a.c.scatter = corr->scatter;
 		a.c.g = corr->global_scale;
 		a.c.denomThr = corr->denom_threshold;
+		a.c.precorr = corr->precorr;
 	}
 }
 

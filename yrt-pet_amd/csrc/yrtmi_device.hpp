@@ -127,6 +127,7 @@ struct DCorr
 	const float* scatter;
 	float g;
 	float denomThr;
+	const float* precorr;  // in-vivo attenuation precorrection (nullable)
 };
 
 struct KArgs

@@ -145,6 +145,8 @@ __global__ void __launch_bounds__(256) kEvents(KArgs a)
 				upd += a.c.randoms[e];
 			if (a.c.scatter)
 				upd += a.c.scatter[e];
+			if (a.c.precorr)
+				upd *= a.c.precorr[e];
 			if (fabsf(upd) > a.c.denomThr)
 			{
 				float v = m / upd;

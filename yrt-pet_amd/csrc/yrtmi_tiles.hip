@@ -595,6 +595,9 @@ __device__ __forceinline__ float emRatioOut(const KArgs& a, int64_t e, float s, 
 		upd += a.c.randoms[e];
 	if (a.c.scatter)
 		upd += a.c.scatter[e];
+	// in-vivo attenuation precorrection (OSEM_GPU.cu:1303-1308)
+	if (a.c.precorr)
+		upd *= a.c.precorr[e];
 	float out = 0.f;
 	if (fabsf(upd) > a.c.denomThr)
 	{

@@ -60,7 +60,7 @@ class Corrections(C.Structure):
     _fields_ = [("meas", C.c_void_p), ("sens", C.c_void_p),
                 ("att", C.c_void_p), ("randoms", C.c_void_p),
                 ("scatter", C.c_void_p), ("global_scale", C.c_float),
-                ("denom_threshold", C.c_float)]
+                ("denom_threshold", C.c_float), ("precorr", C.c_void_p)]
 
 
 class ScannerDesc(C.Structure):

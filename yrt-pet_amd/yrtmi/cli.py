@@ -20,8 +20,7 @@ import sys
 
 import numpy as np
 
-UNSUPPORTED = ["varpsf", "detmask", "att_invivo", "acf_invivo", "att_hardware",
-               "acf_hardware", "acf", "projector_updater", "sparse",
+UNSUPPORTED = ["varpsf", "detmask", "projector_updater", "sparse",
                "save_iter_step", "save_iter_ranges"]
 
 
@@ -135,14 +134,21 @@ def reconstruct(argv):
     p.add_argument("--randoms", default="", help="Randoms estimate histogram filename")
     p.add_argument("--scatter", default="", help="Scatter estimate histogram filename")
     p.add_argument("--att", default="", help="Total attenuation image filename")
+    p.add_argument("--att_invivo", default="", help="In-vivo attenuation image filename")
+    p.add_argument("--att_hardware", default="", help="Hardware attenuation image filename")
+    p.add_argument("--acf", default="", help="Total ACF histogram filename")
+    p.add_argument("--acf_invivo", default="", help="In-vivo ACF histogram filename")
+    p.add_argument("--acf_hardware", default="", help="Hardware ACF histogram filename")
     p.add_argument("--hard_threshold", type=float, default=None, help="Hard threshold")
     p.add_argument("--denom_threshold", type=float, default=None,
                    help="Denominator threshold for the EM ratio")
-    for fmt in ("sensitivity_format", "randoms_format", "scatter_format"):
+    for fmt in ("sensitivity_format", "randoms_format", "scatter_format", "acf_format",
+                "acf_invivo_format", "acf_hardware_format"):
         p.add_argument("--" + fmt, default="H", help=argparse.SUPPRESS)
     a = p.parse_args(argv)
     _refuse_unsupported(a)
-    for fmt in ("sensitivity_format", "randoms_format", "scatter_format"):
+    for fmt in ("sensitivity_format", "randoms_format", "scatter_format", "acf_format",
+                "acf_invivo_format", "acf_hardware_format"):
         if getattr(a, fmt).upper() != "H":
             raise SystemExit(f"--{fmt}: only H is supported")
     if not a.sens_only and not a.out:
@@ -191,8 +197,17 @@ def reconstruct(argv):
         att = Y.ImageDeviceFromFile(ctx, a.att)
         keep.append(att)
         osem.setAttenuationImage(att)
+    for name, setter in (("att_invivo", osem.setInVivoAttenuationImage),
+                         ("att_hardware", osem.setHardwareAttenuationImage)):
+        if getattr(a, name):
+            img = Y.ImageDeviceFromFile(ctx, getattr(a, name))
+            keep.append(img)
+            setter(img)
     for name, setter in (("randoms", osem.setRandomsHistogram),
-                         ("scatter", osem.setScatterHistogram)):
+                         ("scatter", osem.setScatterHistogram),
+                         ("acf", osem.setACFHistogram),
+                         ("acf_invivo", osem.setInVivoACFHistogram),
+                         ("acf_hardware", osem.setHardwareACFHistogram)):
         if getattr(a, name):
             h = Y.Histogram3DDeviceOwned(ctx)
             h.readFromFile(getattr(a, name))
