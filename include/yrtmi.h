@@ -339,6 +339,17 @@ int yrtmi_subset_interleaved(yrtmi_ctx* ctx, const void* src, int elem_bytes,
  * indices >= src_n give 0.  invert != 0: nonzero values become their
  * reciprocal (Corrector::getSensitivity with invertSensitivity,
  * Corrector.cpp:295-308). */
+/* out[i] = histo[bin of the detector pair (d1[i], d2[i])] for list-mode
+ * events: Histogram3D::getProjectionValueFromHistogramBin with a detector
+ * pair (src/datastruct/projection/Histogram3D.cpp:588-606; the pair -> bin
+ * map getCoordsFromDetPair / getBinIdFromCoords, 267-271, 366-405), as
+ * Corrector::getTotalACFFromHistogram / getInVivoAttenuationFactor read ACF
+ * histograms for list-mode data.  histo: the context's Histogram3D (device);
+ * the pair (0, 0) reads 0; a pair outside the histogram fails the call (the
+ * reference throws).  Synchronises the context stream; not under graph
+ * capture. */
+int yrtmi_histo_gather_pairs(yrtmi_ctx* ctx, const float* histo, const uint32_t* d1,
+                             const uint32_t* d2, int64_t n, float* out);
 int yrtmi_gather(yrtmi_ctx* ctx, const float* src, int64_t src_n,
                  const uint32_t* idx, int64_t n, int invert, float* out);
 /* out[i] = histo[bin of element i of histogram subset s of S], for every

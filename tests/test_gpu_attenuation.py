@@ -100,19 +100,81 @@ def test_osem_histogram_attenuation_split(gpu, mode):
         assert rel_l2(x, xn) > 1e-3
 
 
-def test_listmode_rejects_acf_histograms(gpu):
+@pytest.mark.parametrize("mode", ["acf_split", "acf_total"])
+def test_osem_listmode_acf_histograms(gpu, mode):
+    """ACF histograms read at each event's detector pair (list-mode)."""
+    sc = yrtmi.Scanner("h", 48.0, 4.0, 4.0, 10.0, 110.0, 64, 12, maxRingDiff=7)
+    ysc = Y.Scanner("h", 48.0, 4.0, 4.0, 10.0, 110.0, 64, 12, 1, 7, 1, 1)
+    im = yrtmi.ImageParams(40, 40, 20, 3.0, 3.0, 2.5)
+    yim = Y.ImageParams.fromParams(40, 40, 20, 3.0, 3.0, 2.5)
+    osc, oi = oracle_scanner(sc), oracle_image(im)
+    pos, ori = O.lut(osc)
+    zb, ph, nr = O.histo_dims(osc)
+    hb = np.arange(zb * ph * nr)
+    L = O.lors_from_pairs(pos, ori, *O.histo_pairs(osc, hb))
+    po = O.proj_for_scanner(osc)
+    iv = synth.cylinder_spheres(im, 45.0, 40.0, 0.096, [], supersample=1)[None].astype(np.float32)
+    hw = np.zeros_like(iv)
+    hw[..., 8:11, :] = 0.3
+    acf_iv = O.attenuation_factors(oi, iv, L)
+    acf_hw = O.attenuation_factors(oi, hw, L)
+    act = synth.cylinder_spheres(im, 40.0, 30.0, 1.0, [(5.0, 3.0, 0.0, 8.0, 4.0)],
+                                 supersample=1)
+    ev = synth.sample_events(sc, im, act, 60000, seed=23)
+    d1, d2 = ev["d1"].astype(np.uint32), ev["d2"].astype(np.uint32)
+    ring = lambda d: (d // 64) % 12
+    keep = np.abs(ring(d1).astype(int) - ring(d2).astype(int)) <= 7
+    d1, d2 = d1[keep], d2[keep]
+    bins = O.histo_bins_from_pairs(osc, d1, d2)
+    assert np.all(bins >= 0)
+    S, iters = 3, 2
+    osem = Y.createOSEM(ysc)
+    ctx = osem.getContext()
+    osem.setImageParams(yim)
+    osem.num_OSEM_subsets = S
+    osem.num_MLEM_iterations = iters
+    lm = Y.ListModeDeviceOwned(ctx, d1, d2)
+    osem.setDataInput(lm)
+    hs = []
+    for a in ((acf_iv, acf_hw) if mode == "acf_split" else (acf_iv * acf_hw,)):
+        h = Y.Histogram3DDeviceOwned(ctx)
+        h.copyFromHost(np.ascontiguousarray(a, np.float32))
+        hs.append(h)
+    if mode == "acf_split":
+        osem.setInVivoACFHistogram(hs[0])
+        osem.setHardwareACFHistogram(hs[1])
+        hw_h, att, pre = acf_hw, (acf_iv[bins] * acf_hw[bins]).astype(np.float32), acf_iv[bins]
+    else:
+        tot = (acf_iv * acf_hw).astype(np.float32)
+        osem.setACFHistogram(hs[0])
+        hw_h, att, pre = tot, tot[bins], None
+    sens = osem.generateSensitivityImages()[0].transferToHostMemory()
+    osens = O.sensitivity_images(osc, oi, po, S, True, pos_ori=(pos, ori), acf_histo=hw_h)
+    r = rel_l2(sens, osens[0])
+    print(f"lm {mode} sens: {r:.2e}", flush=True)
+    assert r <= 1e-5
+    x = osem.reconstruct().transferToHostMemory()
+    Le = O.lors_from_pairs(pos, ori, d1, d2)
+    xo = O.reconstruct(osc, oi, po, Le, osens, iters, S, True, att=att, precorr=pre)
+    r = rel_l2(x, xo)
+    print(f"lm {mode} osem: {r:.2e}", flush=True)
+    assert r <= 1e-4
+
+
+def test_listmode_acf_rejects_pairs_outside_the_histogram(gpu):
     ysc = Y.Scanner("h", 48.0, 4.0, 4.0, 10.0, 110.0, 64, 12, 1, 7, 1, 1)
     osem = Y.createOSEM(ysc)
     ctx = osem.getContext()
     osem.setImageParams(Y.ImageParams.fromParams(20, 20, 10, 4.0, 4.0, 4.0))
-    lm = Y.ListModeDeviceOwned(ctx, np.arange(100, dtype=np.uint32),
-                               (np.arange(100, dtype=np.uint32) + 32) % 768)
+    # ring 0 to ring 10: a ring difference above the limit of 7
+    lm = Y.ListModeDeviceOwned(ctx, np.array([5, 3], np.uint32),
+                               np.array([64 * 10 + 37, 64 + 35], np.uint32))
     osem.setDataInput(lm)
-    h = Y.Histogram3DDeviceOwned(ctx)
     zb, ph, nr = O.histo_dims(O.ScannerDef(48.0, 4.0, 4.0, 10.0, 110.0, 64, 12,
                                            max_ring_diff=7))
+    h = Y.Histogram3DDeviceOwned(ctx)
     h.copyFromHost(np.ones(zb * ph * nr, np.float32))
     osem.setInVivoACFHistogram(h)
     osem.generateSensitivityImages()
-    with pytest.raises(Exception, match="histogram inputs only"):
+    with pytest.raises(Exception, match="outside the histogram"):
         osem.reconstruct()

@@ -973,9 +973,16 @@ void OSEM_MI::prepareSubsets(OperatorProjectorMI& proj, const ImageParams& imt)
 	};
 	if (m_lm != nullptr)
 	{
-		if (at.totalFromHisto || at.ivAcf != nullptr)
-			throw Error("ACF histograms apply to histogram inputs only; give attenuation "
-			            "images for list-mode data");
+		// ACF histograms read at each event's detector pair
+		auto gatherPairs = [&](std::unique_ptr<DeviceArray<float>>& dst,
+		                       const Histogram3DDevice* h, const Subset& q, int64_t n) {
+			if (h == nullptr || n == 0)
+				return;
+			dst.reset(new DeviceArray<float>(*m_ctx, (size_t)n));
+			check(yrtmi_histo_gather_pairs(c, h->getDevicePointer(), q.d1->data(),
+			                               q.d2->data(), n, dst->data()),
+			      "ACF histogram");
+		};
 		if (!m_lm->motionTransforms.empty())
 			m_ctx->uploadMotion(m_lm->motionTransforms);
 		const int64_t N = (int64_t)m_lm->count();
@@ -1034,8 +1041,24 @@ void OSEM_MI::prepareSubsets(OperatorProjectorMI& proj, const ImageParams& imt)
 				q->lors.motion_frame = q->motion ? q->motion->data() : nullptr;
 			}
 			// per-event attenuation factors, once (precomputeAttenuationFactors)
-			imageFactors(q->att, totImg, ca, q->lors, n);
-			imageFactors(q->pre, ivImg, cv, q->lors, n);
+			if (at.totalFromHisto && n > 0)
+			{
+				if (at.acf != nullptr)
+					gatherPairs(q->att, at.acf, *q, n);
+				else
+				{
+					std::unique_ptr<DeviceArray<float>> hwv;
+					gatherPairs(q->att, at.ivAcf, *q, n);
+					gatherPairs(hwv, at.hwAcf, *q, n);
+					multiplyInto(*m_ctx, hwv->data(), q->att->data(), n);
+				}
+			}
+			else
+				imageFactors(q->att, totImg, ca, q->lors, n);
+			if (at.ivAcf != nullptr)
+				gatherPairs(q->pre, at.ivAcf, *q, n);
+			else
+				imageFactors(q->pre, ivImg, cv, q->lors, n);
 			m_subsets.push_back(std::move(q));
 		}
 	}

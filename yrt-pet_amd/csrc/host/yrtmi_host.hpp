@@ -344,8 +344,8 @@ public:
 	// comes from an image (exp(-0.1 * Siddon(mu)) per LOR) or from an ACF
 	// histogram; missing parts are implied as Corrector::setup does (a total
 	// alone is all hardware, hardware + in-vivo images sum to the total, in-vivo
-	// x hardware ACF histograms multiply to the total).  ACF histograms apply
-	// to histogram inputs (and to every sensitivity image).
+	// x hardware ACF histograms multiply to the total).  For list-mode data
+	// the ACF histograms are read at each event's detector pair.
 	void setHardwareAttenuationImage(const ImageDevice* mu) { m_hwAtt = mu; }
 	void setInVivoAttenuationImage(const ImageDevice* mu) { m_ivAtt = mu; }
 	void setACFHistogram(const Histogram3DDevice* h) { m_acf = h; }

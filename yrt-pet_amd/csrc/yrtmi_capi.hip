@@ -52,6 +52,8 @@ struct yrtmi_ctx
 	yrtmi_histo_desc histo{};
 	uint32_t numR = 0, numPhi = 0, numZBin = 0, numZBinDiff = 0, rCut = 0;
 	uint32_t* zbin = nullptr;
+	int32_t* ringMap = nullptr;  // in-ring pair -> rr | phi << 16 (dpr x dpr)
+	int* gatherErr = nullptr;
 	float* motion = nullptr;
 	int nmotion = 0;
 	// frames of yrtmi_time_average_move (4 float4 each), grown on demand
@@ -304,6 +306,8 @@ int yrtmi_ctx_destroy(yrtmi_ctx* c)
 	(void)hipFree(c->zbin);
 	(void)hipFree(c->motion);
 	(void)hipFree(c->moveFrames);
+	(void)hipFree(c->ringMap);
+	(void)hipFree(c->gatherErr);
 	(void)hipFree(c->counter);
 	(void)hipFree(c->part);
 	if (c->owned != nullptr)
@@ -567,6 +571,8 @@ int yrtmi_histo_setup(yrtmi_ctx* c, const yrtmi_histo_desc* h,
 		tab[zb] = (uint32_t)z1 | ((uint32_t)z2 << 16);
 	}
 	HIP_TRY(hipStreamSynchronize(c->stream));
+	HIP_TRY(hipFree(c->ringMap));
+	c->ringMap = nullptr;  // rebuilt for the new geometry on first use
 	HIP_TRY(hipFree(c->zbin));
 	c->zbin = nullptr;
 	HIP_TRY(hipMalloc(&c->zbin, tab.size() * sizeof(uint32_t)));
@@ -1073,6 +1079,77 @@ int yrtmi_time_average_move(yrtmi_ctx* c, const yrtmi_image* im, const float* in
 	g.ny = (int)im->ny;
 	g.nz = (int)im->nz;
 	HIP_TRY(launchTimeAverageMove(in, out, c->moveFrames, n, g, c->stream));
+	return YRTMI_OK;
+}
+
+int yrtmi_histo_gather_pairs(yrtmi_ctx* c, const float* histo, const uint32_t* d1,
+                             const uint32_t* d2, int64_t n, float* out)
+{
+	int r = setDev(c);
+	if (r)
+		return r;
+	r = notCapturing(c, "histo_gather_pairs");
+	if (r)
+		return r;
+	if (!c->histoReady)
+		return fail(YRTMI_EINVAL, "histo_gather_pairs: no histogram geometry");
+	if (histo == nullptr || d1 == nullptr || d2 == nullptr || out == nullptr || n < 0)
+		return fail(YRTMI_EINVAL, "histo_gather_pairs: bad arguments");
+	if (n == 0)
+		return YRTMI_OK;
+	const int64_t dpr = c->histo.dets_per_ring;
+	if (dpr >= 65536)
+		return fail(YRTMI_EINVAL, "histo_gather_pairs: too many detectors per ring");
+	if (c->ringMap == nullptr)
+	{
+		// the inverse of the in-ring pair of every (rr, phi)
+		// (Histogram3D::setupHistogram's ring map, Histogram3D.cpp:286-297),
+		// with the pair formula of getDetPairInSameRing (428-450)
+		std::vector<int32_t> map((size_t)(dpr * dpr), -1);
+		const int64_t nrr = c->numR / (c->histo.num_doi * c->histo.num_doi);
+		for (int64_t phi = 0; phi < c->numPhi; phi++)
+			for (int64_t rr = 0; rr < nrr; rr++)
+			{
+				const int64_t d02 = dpr / 2 + (phi & 1);
+				const int64_t sh = rr - dpr / 4 + (int64_t)c->rCut;
+				int64_t a = sh, b = d02 - sh;
+				if (a < 0)
+					a += dpr;
+				if (b < 0)
+					b += dpr;
+				a = (a + phi / 2) % dpr;
+				b = (b + phi / 2) % dpr;
+				if (a > b)
+					std::swap(a, b);
+				if (a != b)
+					map[(size_t)(a * dpr + b)] = (int32_t)(rr | (phi << 16));
+			}
+		HIP_TRY(hipStreamSynchronize(c->stream));
+		HIP_TRY(hipMalloc(&c->ringMap, map.size() * sizeof(int32_t)));
+		HIP_TRY(hipMemcpy(c->ringMap, map.data(), map.size() * sizeof(int32_t),
+		                  hipMemcpyHostToDevice));
+	}
+	if (c->gatherErr == nullptr)
+		HIP_TRY(hipMalloc(&c->gatherErr, sizeof(int)));
+	HIP_TRY(hipMemsetAsync(c->gatherErr, 0, sizeof(int), c->stream));
+	PairGeom g{};
+	g.dpr = (uint32_t)dpr;
+	g.nrings = (uint32_t)c->histo.num_rings;
+	g.ndet = (uint32_t)(dpr * c->histo.num_rings * c->histo.num_doi);
+	g.numDOI = (uint32_t)c->histo.num_doi;
+	g.numDOIPoss = (uint32_t)(c->histo.num_doi * c->histo.num_doi);
+	g.numR = c->numR;
+	g.numPhi = c->numPhi;
+	g.numZBinDiff = c->numZBinDiff;
+	g.maxRingDiff = (int32_t)c->histo.max_ring_diff;
+	HIP_TRY(launchGatherPairs(histo, d1, d2, n, c->ringMap, g, out, c->gatherErr, c->stream));
+	int err = 0;
+	HIP_TRY(hipMemcpyAsync(&err, c->gatherErr, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+	HIP_TRY(hipStreamSynchronize(c->stream));
+	if (err)
+		return fail(YRTMI_EINVAL, "histo_gather_pairs: a detector pair outside the histogram "
+		                          "(unknown detector, same in-ring position or ring difference "
+		                          "above the limit)");
 	return YRTMI_OK;
 }
 

@@ -616,4 +616,76 @@ hipError_t launchFill(float* x, int64_t n, float v, hipStream_t st)
 	return hipGetLastError();
 }
 
+// Histogram3D::getProjectionValueFromHistogramBin with a detector pair
+// (Histogram3D.cpp:588-606, getCoordsFromDetPair 366-405,
+// getBinIdFromCoords 267-271): the pair ordered by id then by position in
+// its ring, (r, phi) from the in-ring table (ringMap[a * dpr + b] =
+// rr | phi << 16, -1 where no bin), the z bin from the two rings.  The pair
+// (0, 0) reads 0; a pair outside the histogram (a detector beyond the
+// scanner, no in-ring bin, a ring difference above the limit) sets *err and
+// reads 0 (the reference throws).
+__global__ void __launch_bounds__(256) kGatherPairs(const float* __restrict__ histo,
+                                                    const uint32_t* __restrict__ d1s,
+                                                    const uint32_t* __restrict__ d2s, int64_t n,
+                                                    const int32_t* __restrict__ ringMap,
+                                                    PairGeom g, float* __restrict__ out,
+                                                    int* __restrict__ err)
+{
+	const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+	for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+	{
+		uint32_t d1 = d1s[i], d2 = d2s[i];
+		float v = 0.f;
+		if (d1 != 0 || d2 != 0)
+		{
+			bool ok = d1 < g.ndet && d2 < g.ndet;
+			if (d1 > d2)
+			{
+				const uint32_t t = d1;
+				d1 = d2;
+				d2 = t;
+			}
+			uint32_t r1 = d1 % g.dpr, r2 = d2 % g.dpr;
+			if (r1 > r2)
+			{
+				uint32_t t = d1;
+				d1 = d2;
+				d2 = t;
+				t = r1;
+				r1 = r2;
+				r2 = t;
+			}
+			const int32_t m = ok ? ringMap[r1 * g.dpr + r2] : -1;
+			const int32_t z1 = (int32_t)((d1 / g.dpr) % g.nrings);
+			const int32_t z2 = (int32_t)((d2 / g.dpr) % g.nrings);
+			const int32_t dz = z2 > z1 ? z2 - z1 : z1 - z2;
+			ok = ok && m >= 0 && dz <= g.maxRingDiff;
+			if (ok)
+			{
+				const uint32_t doi1 = d1 / (g.nrings * g.dpr), doi2 = d2 / (g.nrings * g.dpr);
+				const uint32_t r = (uint32_t)(m & 0xffff) * g.numDOIPoss + doi1 + doi2 * g.numDOI;
+				const uint32_t phi = (uint32_t)m >> 16;
+				int64_t zb = (int64_t)dz * g.nrings + (z1 < z2 ? z1 : z2) -
+				             (int64_t)dz * (dz - 1) / 2;
+				if (dz > 0 && z1 > z2)
+					zb += g.numZBinDiff;
+				v = histo[(zb * g.numPhi + phi) * g.numR + r];
+			}
+			else
+				atomicOr(err, 1);
+		}
+		out[i] = v;
+	}
+}
+
+hipError_t launchGatherPairs(const float* histo, const uint32_t* d1, const uint32_t* d2,
+                             int64_t n, const int32_t* ringMap, const PairGeom& g, float* out,
+                             int* err, hipStream_t st)
+{
+	if (n <= 0)
+		return hipSuccess;
+	kGatherPairs<<<gridFor(n), 256, 0, st>>>(histo, d1, d2, n, ringMap, g, out, err);
+	return hipGetLastError();
+}
+
 }  // namespace yrtmi

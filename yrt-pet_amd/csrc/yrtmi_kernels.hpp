@@ -21,6 +21,15 @@ hipError_t launchThreshold(float* x, const float* m, int64_t nvox, int64_t nt,
                            int64_t mFrames, float thr, float les, float leo,
                            float gts, float gto, hipStream_t st);
 hipError_t launchFill(float* x, int64_t n, float v, hipStream_t st);
+// Histogram values of detector pairs (list-mode events).
+struct PairGeom
+{
+	uint32_t dpr, nrings, ndet, numDOI, numDOIPoss, numR, numPhi, numZBinDiff;
+	int32_t maxRingDiff;
+};
+hipError_t launchGatherPairs(const float* histo, const uint32_t* d1, const uint32_t* d2,
+                             int64_t n, const int32_t* ringMap, const PairGeom& g, float* out,
+                             int* err, hipStream_t st);
 
 // Image-space PSF: one axis (0 x, 1 y, 2 z) of the separable circular
 // convolution of every frame (nx * ny * nz * frames = total < 2^31 voxels),

@@ -118,6 +118,7 @@ _SIGS = {
     "yrtmi_fill": (C.c_int, [_P, _P, C.c_int64, C.c_float]),
     "yrtmi_psf_apply": (C.c_int, [_P, C.POINTER(Image), _P, _P, _P, _P, C.c_int, _P, C.c_int,
                                   _P, C.c_int, C.c_int]),
+    "yrtmi_histo_gather_pairs": (C.c_int, [_P, _P, _P, _P, C.c_int64, _P]),
     "yrtmi_time_average_move": (C.c_int, [_P, C.POINTER(Image), _P, _P, _P, _P, C.c_int]),
     "yrtmi_count_segments": (C.c_int, [_P, _P, C.POINTER(Image), C.POINTER(Lors),
                                        C.POINTER(C.c_int64)]),
