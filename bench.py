@@ -38,6 +38,11 @@ after frame with the fused EM kernel, (b) frame after frame through tile
 plans, and (c) batched: all frames in one multi-frame image and one event
 set (frame per event), tile plans, autotuned, graph-replayed.
 
+--workload motion: list-mode TOF OSEM with LOR motion: 250M events in 8
+rigid motion frames (contiguous stretches of the acquisition), 256x256x128
+@1.6 mm, 10 subsets, 4 rays; the sensitivity image time-averaged over the
+motion.  No CPU baseline (the default workload carries it).
+
 Prints one JSON line (see the dict each workload returns).
 """
 from __future__ import annotations
@@ -81,7 +86,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=("listmode", "frames", "histogram", "dd"),
+    ap.add_argument("--workload", choices=("listmode", "frames", "histogram", "dd", "motion"),
                     default="listmode")
     ap.add_argument("--dd-min", type=float, default=1e6, help="dd workload: min counts/frame")
     ap.add_argument("--dd-max", type=float, default=40e6, help="dd workload: max counts/frame")
@@ -118,9 +123,12 @@ def log(args, *a):
         print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def scanner_and_image():
+def scanner_and_image(motion=False):
     import yrtmi
     sc = yrtmi.Scanner("cfg2", 64 * 3.2, 3.2, 3.2, 20.0, 400.0, 512, 64)
+    if motion:
+        # the motion-corrected configuration: 256x256x128 @1.6 mm
+        return sc, yrtmi.ImageParams(256, 256, 128, 1.6, 1.6, 1.6)
     im = yrtmi.ImageParams(200, 200, 128, 2.0, 2.0, 2.0)
     return sc, im
 
@@ -137,7 +145,7 @@ def subset_major(torch, ev, n, S, dev):
     """Event i belongs to subset i mod S (BinIteratorChronologicalInterleaved);
     permute once so each subset is contiguous."""
     perm = torch.cat([torch.arange(s, n, S, device=dev) for s in range(S)])
-    out = {k: ev[k][perm].contiguous() for k in ("d1", "d2", "tof") if k in ev}
+    out = {k: ev[k][perm].contiguous() for k in ("d1", "d2", "tof", "motion") if k in ev}
     bounds = [0]
     for s in range(S):
         bounds.append(bounds[-1] + len(range(s, n, S)))
@@ -179,19 +187,29 @@ def bench_listmode(args, torch, dev, stream):
     from yrtmi import synth
 
     t_setup = time.time()
-    sc, im = scanner_and_image()
+    motion = args.workload == "motion"
+    sc, im = scanner_and_image(motion)
     act = phantom(args, im)
     log(args, f"phantom ready ({time.time() - t_setup:.1f}s)")
-    ev = synth.sample_events(sc, im, act, args.events, seed=args.seed,
+    # the motion workload: 250M events in 8 rigid motion frames (contiguous
+    # stretches of the acquisition) unless --events says otherwise
+    n_events = 250_000_000 if motion and args.events == 100_000_000 else args.events
+    NMF = 8
+    ev = synth.sample_events(sc, im, act, n_events, seed=args.seed,
                              tof_fwhm_ps=250.0, device=dev)
-    S, n = args.subsets, args.events
+    S, n = args.subsets, n_events
+    if motion:
+        ev["motion"] = (torch.arange(n, device=dev, dtype=torch.int64) * NMF // n).to(torch.uint8)
     ev, bounds = subset_major(torch, ev, n, S, dev)
     d1, d2, tof = ev["d1"], ev["d2"], ev["tof"]
+    mot = ev.get("motion")
     del ev
     torch.cuda.synchronize()
     log(args, f"{n} events drawn ({time.time() - t_setup:.1f}s)")
 
     ctx = yrtmi.Context(sc, stream=stream)
+    if motion:
+        ctx.upload_motion(synth.rigid_transforms(np.random.default_rng(args.seed), NMF))
     pp = yrtmi.ProjectorParams(sc, numRays=4)
     pp.addTOF(250.0, 5)
     proj = ctx.projector(pp)
@@ -214,6 +232,14 @@ def bench_listmode(args, torch, dev, stream):
             del pl
         else:
             ctx.sensitivity(psens, im, sens, Ls)
+    if motion:
+        # the sensitivity image time-averaged over the motion frames (equal
+        # durations: weight 1/8 each), as the reconstruct command does
+        moved = torch.zeros_like(sens)
+        ctx.time_average_move(im, sens, moved,
+                              synth.rigid_transforms(np.random.default_rng(args.seed), NMF),
+                              np.full(NMF, 1.0 / NMF, np.float32))
+        sens = moved
     ctx.synchronize()
     sens_s = time.time() - t0
     n_lors = sum(yrtmi.LorSet.histo_subset(ctx, SS, s).n for s in range(SS))
@@ -225,7 +251,9 @@ def bench_listmode(args, torch, dev, stream):
     ctx.threshold(x, sens, im.nvox, 0.0, 0.0, 0.0, 1.0, 0.0)
     u = torch.zeros(im.shape, device=dev)
     sub = [yrtmi.LorSet.pairs(d1[bounds[s]:bounds[s + 1]], d2[bounds[s]:bounds[s + 1]],
-                              tof=tof[bounds[s]:bounds[s + 1]]) for s in range(S)]
+                              tof=tof[bounds[s]:bounds[s + 1]],
+                              motion_frame=None if mot is None else mot[bounds[s]:bounds[s + 1]])
+           for s in range(S)]
     order_s = 0.0
     if args.order == "lor":
         # each subset's events in locality order (the EM update does not
@@ -332,7 +360,7 @@ def bench_listmode(args, torch, dev, stream):
     # measured traffic applies to the profiled configuration only
     traffic, traffic_src = None, None
     if (tiled and n == 100_000_000 and S == 10 and args.order == "lor"
-            and not args.no_tune):
+            and not args.no_tune and not motion):
         traffic = profiled_traffic(dom["kernel"])
         if traffic is not None:
             traffic_src = os.path.relpath(TRAFFIC_PROFILE, REPO)
@@ -340,7 +368,7 @@ def bench_listmode(args, torch, dev, stream):
         f"{k['kernel']}: {k['avg_launch_ms']:.2f} ms, {k['achieved']:.0f} GB/s alg" for k in kernels))
 
     cpu = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and not motion:
         cpu = cpu_baseline(args, sc, im, pp, d1, d2, tof, bounds, x)
     return {
         "metric": "fwd+back event-projections/s, s/OSEM iter",
@@ -356,10 +384,14 @@ def bench_listmode(args, torch, dev, stream):
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded phantom, physically sampled TOF list-mode events)",
-        "config": {"workload": "list-mode TOF OSEM, 4-ray Siddon, 64x512 scanner R400, "
-                               "100M events, 200x200x128 @2mm, 10 subsets",
+        "config": {"workload": (f"list-mode TOF OSEM with LOR motion ({NMF} rigid frames), "
+                                f"4-ray Siddon, 64x512 scanner R400, {n / 1e6:g}M events, "
+                                f"256x256x128 @1.6mm, {S} subsets") if motion else
+                               ("list-mode TOF OSEM, 4-ray Siddon, 64x512 scanner R400, "
+                                "100M events, 200x200x128 @2mm, 10 subsets"),
+                   "motion_frames": NMF if motion else 0,
                    "events": n, "subsets": S, "rays": 4, "tof_fwhm_ps": 250.0,
-                   "tof_n_std": 5, "image": [200, 200, 128], "voxel_mm": 2.0,
+                   "tof_n_std": 5, "image": [im.nx, im.ny, im.nz], "voxel_mm": float(im.vx),
                    "lanes_per_event": lpe, "accumulation": args.accum,
                    "event_order": args.order,
                    "graph": graph is not None, "parallelism": "single GPU"},
