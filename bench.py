@@ -38,6 +38,9 @@ after frame with the fused EM kernel, (b) frame after frame through tile
 plans, and (c) batched: all frames in one multi-frame image and one event
 set (frame per event), tile plans, autotuned, graph-replayed.
 
+--workload batch: 8 independent images of 128x128x64 @2 mm, 20M TOF
+(350 ps) events each, one launch per subset for all eight.
+
 --workload motion: list-mode TOF OSEM with LOR motion: 250M events in 8
 rigid motion frames (contiguous stretches of the acquisition), 256x256x128
 @1.6 mm, 10 subsets, 4 rays; the sensitivity image time-averaged over the
@@ -86,7 +89,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=("listmode", "frames", "histogram", "dd", "motion"),
+    ap.add_argument("--workload", choices=("listmode", "frames", "histogram", "dd", "motion",
+                                           "batch"),
                     default="listmode")
     ap.add_argument("--dd-min", type=float, default=1e6, help="dd workload: min counts/frame")
     ap.add_argument("--dd-max", type=float, default=40e6, help="dd workload: max counts/frame")
@@ -170,7 +174,7 @@ def main():
         del a, b
     stream = torch.cuda.Stream()  # default priority; graphs need a non-null stream
     with torch.cuda.stream(stream):
-        if args.workload == "frames":
+        if args.workload in ("frames", "batch"):
             out = bench_frames(args, torch, dev, stream)
         elif args.workload == "histogram":
             out = bench_histogram(args, torch, dev, stream)
@@ -420,19 +424,24 @@ def bench_frames(args, torch, dev, stream):
     from yrtmi import synth
 
     sc, im1 = scanner_and_image()
-    F, S = args.frames, args.subsets
+    # batch: 8 independent images of 128x128x64 @2 mm, 20M TOF (350 ps) events each
+    batch = args.workload == "batch"
+    tof_ps = 350.0 if batch else 250.0
+    if batch:
+        im1 = yrtmi.ImageParams(128, 128, 64, 2.0, 2.0, 2.0)
+    F, S = (8 if batch else args.frames), args.subsets
     rng = np.random.default_rng(args.seed + 1)
     counts = np.exp(rng.uniform(math.log(args.frame_min), math.log(args.frame_max), F))
-    counts = [int(c) for c in counts]
+    counts = [20_000_000] * F if batch else [int(c) for c in counts]
     act = phantom(args, im1)
     t0 = time.time()
-    evs = [synth.sample_events(sc, im1, act, c, seed=args.seed + 7 * f, tof_fwhm_ps=250.0,
+    evs = [synth.sample_events(sc, im1, act, c, seed=args.seed + 7 * f, tof_fwhm_ps=tof_ps,
                                device=dev) for f, c in enumerate(counts)]
     n = sum(counts)
     log(args, f"{F} frames, {n} events ({min(counts)}..{max(counts)}) in {time.time() - t0:.1f}s")
     ctx = yrtmi.Context(sc, stream=stream)
     pp = yrtmi.ProjectorParams(sc, numRays=4)
-    pp.addTOF(250.0, 5)
+    pp.addTOF(tof_ps, 5)
     proj = ctx.projector(pp)
     proj.set_launch(args.lpe)
     sens = torch.full(im1.shape, 1.0, device=dev)  # a sensitivity of ones: EM-loop timing
@@ -511,9 +520,11 @@ def bench_frames(args, torch, dev, stream):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded phantom, physically sampled TOF list-mode events)",
-        "config": {"workload": f"{F} dynamic frames, log-uniform [{args.frame_min:g}, "
-                               f"{args.frame_max:g}] events, 4-ray Siddon TOF 250 ps, "
-                               f"200x200x128 @2mm, {S} subsets",
+        "config": {"workload": (f"batch of {F} independent images, 20M events each, "
+                                f"4-ray Siddon TOF 350 ps, 128x128x64 @2mm, {S} subsets")
+                   if batch else (f"{F} dynamic frames, log-uniform [{args.frame_min:g}, "
+                                  f"{args.frame_max:g}] events, 4-ray Siddon TOF 250 ps, "
+                                  f"200x200x128 @2mm, {S} subsets"),
                    "frame_events": counts, "events": n},
         "plain_fused_value": 2.0 * n / res["plain_fused_s"],
         "plain_tiled_value": 2.0 * n / res["plain_tiled_s"],
