@@ -21,8 +21,13 @@
  *                                      intersection, regression coordinates,
  *                                      adjoint identity
  *   DD (+TOF, +PSF)                  - getOverlap known answers, uniform-image
- *                                      footprint identity, delta-PSF == plain
- *                                      DD, adjoint
+ *                                      footprint identity, adjoint, per-voxel
+ *                                      closed forms for LORs whose crystals
+ *                                      face each other along an axis (unequal
+ *                                      crystal sizes, anisotropic voxels,
+ *                                      footprint-end rounding, flat and
+ *                                      asymmetric PSF rows with the flip on
+ *                                      backprojection, TOF slab weights)
  *   LUT / Histogram3D / subsets      - closed-form geometry, bin<->pair
  *                                      round trip, subset partition
  *   EM subset / update / threshold   - elementwise closed forms, MLEM
