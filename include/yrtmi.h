@@ -262,6 +262,10 @@ int yrtmi_threshold(yrtmi_ctx* ctx, float* x, const float* mask, int64_t nvox,
                     int64_t nt, int64_t mask_frames, float thr, float le_scale,
                     float le_off, float gt_scale, float gt_off);
 int yrtmi_fill(yrtmi_ctx* ctx, float* x, int64_t n, float value);
+/* x[i] = v for n int32 elements (device): frame indices of a batch of
+ * frames.  No reference counterpart (its frames are host properties,
+ * ProjectionData::getDynamicFrame).  Recordable. */
+int yrtmi_fill_i32(yrtmi_ctx* ctx, int32_t* x, int64_t n, int32_t v);
 /* Image-space PSF (OperatorPsf::applyA / applyAH,
  * src/operators/OperatorPsf.cpp:254-375; the device operator
  * OperatorPsfDevice): out = the separable circular convolution of `in`
@@ -316,6 +320,11 @@ int yrtmi_count_segments(yrtmi_ctx* ctx, const yrtmi_proj* p,
  * synchronous on the context's stream.  kind: 0 host->device, 1 device->host,
  * 2 device->device. */
 int yrtmi_device_alloc(yrtmi_ctx* ctx, int64_t bytes, void** out);
+/* Free and total device memory of the context's device, bytes (HOST
+ * pointers, nullable): globals::getAvailableVRAM's cudaMemGetInfo
+ * (src/utils/GPUUtils.cu:77), from which OSEM_GPU sizes its batches
+ * (getMemAvailable, src/recon/OSEM_GPU.cu:583-591). */
+int yrtmi_mem_info(yrtmi_ctx* ctx, int64_t* free_bytes, int64_t* total_bytes);
 int yrtmi_device_free(yrtmi_ctx* ctx, void* p);
 int yrtmi_copy(yrtmi_ctx* ctx, void* dst, const void* src, int64_t bytes,
                int kind);
@@ -350,6 +359,16 @@ int yrtmi_subset_interleaved(yrtmi_ctx* ctx, const void* src, int elem_bytes,
  * capture. */
 int yrtmi_histo_gather_pairs(yrtmi_ctx* ctx, const float* histo, const uint32_t* d1,
                              const uint32_t* d2, int64_t n, float* out);
+/* bins[i] = the Histogram3D bin of the detector pair (d1[i], d2[i])
+ * (Histogram3D::getBinIdFromDetPair, src/datastruct/projection/
+ * Histogram3D.cpp:416-421, with getCoordsFromDetPair 366-405 and the ring
+ * map of setupHistogram 286-297), on the device: e.g. to histogram
+ * list-mode events.  A pair outside the histogram, or (0, 0), gets
+ * 0xffffffff and fails the call (the reference throws).  The histogram must
+ * have fewer than 2^32 - 1 bins.  Synchronises the context stream; not under
+ * graph capture. */
+int yrtmi_histo_pair_bins(yrtmi_ctx* ctx, const uint32_t* d1, const uint32_t* d2, int64_t n,
+                          uint32_t* bins);
 int yrtmi_gather(yrtmi_ctx* ctx, const float* src, int64_t src_n,
                  const uint32_t* idx, int64_t n, int invert, float* out);
 /* out[i] = histo[bin of element i of histogram subset s of S], for every
@@ -393,9 +412,11 @@ int yrtmi_permute(yrtmi_ctx* ctx, const void* src, int elem_bytes,
  * Siddon: a plan bins the walk of every (LOR, ray) of `lors` by the image
  * tiles it crosses, recording where it enters each (12 bytes per record,
  * records of a tile ordered by the length of their piece), and keeps each
- * ray's end points, TOF value and frame (32 bytes per LOR and ray), so that
- * the backprojection reads one table entry per record instead of resolving
- * the LOR.
+ * ray's end points, TOF value and frame (32 bytes per LOR and ray), plus,
+ * with the FP32 set-up, the ray's walk set-up (inverse direction, exit
+ * parameter, TOF terms: 64 bytes per LOR and ray in all), so that the
+ * backprojection reads one table entry per record instead of resolving the
+ * LOR and redoing its set-up.
  * DD: a plan keeps each LOR's footprint set-up (96 bytes per LOR) and one
  * record per tile the footprint reaches (8 bytes: the LOR and the slices of
  * the tile's slab that reach it, records of a tile ordered by their voxel
@@ -421,7 +442,14 @@ int yrtmi_permute(yrtmi_ctx* ctx, const void* src, int elem_bytes,
  * workgroup; the tile is accumulated in LDS in 64-bit fixed point, FP32 for
  * the 32x32x32 shape); persistent,
  * 0 for one workgroup per chunk, else that many workgroups taking chunks from
- * a device queue (changeable with yrtmi_plan_set_persistent). */
+ * a device queue (changeable with yrtmi_plan_set_persistent); max_bytes, 0
+ * for no limit, else the device memory the plan may hold: a plan that would
+ * hold more (known from the ray count before the ray table is allocated and
+ * from the count pass before the records are) is not built and the call
+ * fails with YRTMI_ENOMEM, having allocated nothing beyond its build
+ * scratch (the reference sizes its batches to the memory available,
+ * OperatorProjectorDevice::create's memAvailable, OperatorProjectorDevice.
+ * cu:36-46, OSEM_GPU.cu:413-435). */
 #define YRTMI_TILE_16x16x16 0
 #define YRTMI_TILE_32x32x16 1
 #define YRTMI_TILE_32x32x32 2
@@ -431,6 +459,7 @@ typedef struct
 	int chunk_records;
 	int tile_shape;
 	int persistent;
+	int64_t max_bytes;
 } yrtmi_plan_options;
 typedef struct yrtmi_plan yrtmi_plan;
 int yrtmi_plan_create(yrtmi_ctx* ctx, const yrtmi_proj* p,
@@ -443,12 +472,14 @@ int yrtmi_plan_set_persistent(yrtmi_plan* plan, int workgroups);
 int yrtmi_plan_info(const yrtmi_plan* plan, int64_t* records, int64_t* chunks,
                     int64_t* bytes);
 /* Launch-shape autotuner (no reference counterpart).  On `lors` and image x
- * it times the EM ratio kernel for lanes per event 1, 2, 4 and 8 and the
- * planned backprojection for every tile shape, each with one workgroup per
- * chunk and as a persistent launch (one plan per shape is built and freed),
- * then sets the projector's lanes per event to the fastest and returns the
- * fastest settings; plans made afterwards with best.tile_shape /
- * best.persistent use them.  Synchronises and allocates temporaries. */
+ * it times the EM ratio kernel for lanes per event 1, 2, 4 and 8 (Siddon;
+ * a distance-driven LOR is one ray: lanes per event 1, the other entries of
+ * ratio_ms_lpe -1) and the planned backprojection for every tile shape,
+ * each with one workgroup per chunk and as a persistent launch (one plan per
+ * shape is built and freed), then sets the projector's lanes per event to
+ * the fastest and returns the fastest settings; plans made afterwards with
+ * best.tile_shape / best.persistent use them.  Synchronises and allocates
+ * temporaries. */
 typedef struct
 {
 	int lanes_per_event;

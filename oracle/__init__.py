@@ -645,7 +645,12 @@ def time_average_frames(timestamps, xf, t0, t1):
     if not frames:
         raise ValueError("no motion frame starts inside the time range")
     scan = np.float32(t1 - t0)
-    w = np.array([motion_duration(t, f) / scan for f in frames], np.float32)
+    if len(t) == 1:
+        # one motion frame: LORMotion::getDuration would read before the
+        # first record; the one frame covers the whole range
+        w = np.ones(len(frames), np.float32)
+    else:
+        w = np.array([motion_duration(t, f) / scan for f in frames], np.float32)
     inv = invert_transform(np.asarray(xf, np.float32).reshape(-1, 12)[frames])
     return np.array(frames), inv, w
 

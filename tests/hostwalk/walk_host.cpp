@@ -1,6 +1,8 @@
-// Host build of the Siddon walk (yrtmi_rays.hpp) for CPU tests: the full
-// walk of one ray, and its tile decomposition (the bin pass's tile entries,
-// each re-entered as the tile kernel does) summed back per voxel.
+// Host build of the ray kernels (yrtmi_rays.hpp) for CPU tests: the full
+// Siddon walk of one ray, its tile decomposition (the bin pass's tile
+// entries, each re-entered as the tile kernel does) summed back per voxel,
+// and the distance-driven sweep of one LOR (forward and backprojection, with
+// TOF and the projection-space PSF), whole and cut into tiles.
 #include "yrtmi_rays.hpp"
 
 #include <map>
@@ -35,7 +37,7 @@ double tiles(const DImage& P, float3 a, float3 b, const DTof& T, float tofv, int
 {
 	std::map<int, double> full, tiled;
 	SidRay r;
-	out[0] = out[1] = out[2] = 0;
+	out[0] = out[1] = out[2] = out[3] = 0;
 	if (!sidSetup<TOF, S64>(P, a, b, T, tofv, r))
 		return 0.0;
 	int dx = 0, dy = 0, dz = 0;
@@ -61,22 +63,42 @@ double tiles(const DImage& P, float3 a, float3 b, const DTof& T, float tofv, int
 		                  }
 	                  });
 	out[0] = (int)ents.size();
+	// the plan's ray-table entry of an FP32 set-up (kRayPrep)
+	float4 st[2];
+	sidStore(st, r);
+	// the tiled walks, the FP32 set-up's non-first records re-entered from
+	// the stored set-up (kTileBack) and, beside them, from a fresh set-up:
+	// out[3] counts voxels where the two differ at all
+	std::map<int, double> fresh;
 	for (const Ent& e : ents)
 	{
-		SidRay q;
-		sidSetup<TOF, S64>(P, a, b, T, tofv, q);
 		const int x0 = e.vx / TX * TX, y0 = e.vy / TY * TY, z0 = e.vz / TZ * TZ;
 		const WalkBox B{x0, std::min(x0 + TX, P.nx), y0, std::min(y0 + TY, P.ny),
 		                z0, std::min(z0 + TZ, P.nz), P.nx, P.nxy, 0};
-		if (!e.first)
-			sidRestart<S64>(P, q, e.a, e.vx, e.vy, e.vz, dx, dy, dz);
-		sidWalk<TOF, S64>(P, q, T, B, e.first, e.vx, e.vy, e.vz,
-		                  [&](int, int, int, int idx, float, float wt) {
-			                  tiled[idx] += wt;
-			                  if (wt > 0.f)
-				                  out[2]++;
-		                  });
+		for (int pass = 0; pass < 2; pass++)
+		{
+			SidRay q;
+			if (pass == 0 && !S64 && !e.first)
+				sidLoad(a, b, st[0], st[1], q);
+			else
+				sidSetup<TOF, S64>(P, a, b, T, tofv, q);
+			if (!e.first)
+				sidRestart<S64>(P, q, e.a, e.vx, e.vy, e.vz, dx, dy, dz);
+			std::map<int, double>& m = pass == 0 ? tiled : fresh;
+			sidWalk<TOF, S64>(P, q, T, B, e.first, e.vx, e.vy, e.vz,
+			                  [&](int, int, int, int idx, float, float wt) {
+				                  m[idx] += wt;
+				                  if (wt > 0.f && pass == 0)
+					                  out[2]++;
+			                  });
+		}
 	}
+	for (auto& kv : fresh)
+		if (!tiled.count(kv.first) || tiled[kv.first] != kv.second)
+			out[3]++;
+	for (auto& kv : tiled)
+		if (!fresh.count(kv.first))
+			out[3]++;
 	double mx = 0, md = 0;
 	for (auto& kv : full)
 		mx = std::max(mx, std::fabs(kv.second));
@@ -128,7 +150,8 @@ float hw_forward(const float* imp, const float* img, const float* p1, const floa
 
 // Tile decomposition of one ray: max over voxels of |full - tiled| / max
 // |full|.  out[0] tiles entered, out[1] segments of the full walk, out[2]
-// nonempty segments of the tiled walks.
+// nonempty segments of the tiled walks, out[3] voxels where re-entering
+// from the stored FP32 set-up differs from re-entering from a fresh one.
 double hw_tiles(const float* imp, const float* p1, const float* p2, int tof, int s64,
                 const float* tofc, float tofv, int TX, int TY, int TZ, int* out)
 {
@@ -142,6 +165,108 @@ double hw_tiles(const float* imp, const float* p1, const float* p2, int tof, int
 	if (s64)
 		return tiles<false, true>(P, a, b, T, tofv, TX, TY, TZ, out);
 	return tiles<false, false>(P, a, b, T, tofv, TX, TY, TZ, out);
+}
+
+// Distance-driven LOR (ddSetup + ddVisit): forward projection of `img`
+// (returned) and backprojection of `val` added into `back` (may be null),
+// the image swept whole; tile = {TX, TY, TZ} > 0 also sweeps it tile by tile
+// (the tile kernel's ranges) into `tiled` (may be null).  psf: numS x K taps
+// or null.  *nvox = footprint voxels visited.
+float hw_dd(const float* imp, const float* img, const float* p1, const float* p2,
+            const float* n1, const float* n2, int tof, const float* tofc, float tofv,
+            float tz, float tt, const float* psf, int numS, int K, float sStep,
+            float kSpacing, float val, float* back, const int* tile, float* tiled,
+            int* nvox)
+{
+	const DImage P = image(imp);
+	DProj pr{};
+	pr.type = 1;
+	pr.numRays = 1;
+	pr.useTof = tof;
+	pr.tof = DTof{tofc[0], tofc[1], tofc[2], rcpNearest(tofc[0])};
+	pr.tz = tz;
+	pr.tt = tt;
+	std::vector<float> taps;
+	std::vector<double> cum;
+	if (psf != nullptr)
+	{
+		const int64_t nk = (int64_t)numS * K;
+		taps.resize(2 * nk);
+		for (int64_t i = 0; i < numS; i++)
+			for (int64_t j = 0; j < K; j++)
+			{
+				taps[i * K + j] = psf[i * K + j];
+				taps[nk + i * K + (K - 1 - j)] = psf[i * K + j];
+			}
+		cum.resize(2 * (size_t)numS * (K + 2));
+		psfPrefixSums(taps.data(), numS, K, kSpacing, cum.data());
+		psfPrefixSums(taps.data() + nk, numS, K, kSpacing, cum.data() + (size_t)numS * (K + 2));
+		pr.psf.k = taps.data();
+		pr.psf.kflip = taps.data() + nk;
+		pr.psf.cum = cum.data();
+		pr.psf.cumFlip = cum.data() + (size_t)numS * (K + 2);
+		pr.psf.numS = numS;
+		pr.psf.K = K;
+		pr.psf.sStep = sStep;
+		pr.psf.kSpacing = kSpacing;
+		pr.psf.halfWidth = (float)((K + 1) / 2) * kSpacing;
+		pr.psf.rks = rcpNearest(kSpacing);
+	}
+	const float3 a = make_float3(p1[0], p1[1], p1[2]), b = make_float3(p2[0], p2[1], p2[2]);
+	const float3 na = make_float3(n1[0], n1[1], n1[2]), nb = make_float3(n2[0], n2[1], n2[2]);
+	auto sweep = [&](bool flip, int xlo, int xhi, int ylo, int yhi, int zlo, int zhi,
+	                 auto&& fn) {
+		DDRay R;
+		if (tof)
+			ddSetup<true, false>(P, a, b, na, nb, pr, tofv, flip, R);
+		else
+			ddSetup<false, false>(P, a, b, na, nb, pr, tofv, flip, R);
+		if (tof)
+			ddVisit<true>(P, R, pr, xlo, xhi, ylo, yhi, zlo, zhi, fn);
+		else
+			ddVisit<false>(P, R, pr, xlo, xhi, ylo, yhi, zlo, zhi, fn);
+		return R;
+	};
+	const int big = 1 << 30;
+	float acc = 0.f;
+	int cnt = 0;
+	sweep(false, 0, big, 0, big, 0, big, [&](int io, int, int, int, float w) {
+		acc = fmaf(w, img[io], acc);
+		cnt++;
+	});
+	*nvox = cnt;
+	if (back != nullptr)
+		sweep(true, 0, big, 0, big, 0, big,
+		      [&](int io, int, int, int, float w) { back[io] += val * w; });
+	if (tiled != nullptr && tile != nullptr)
+	{
+		// the tiles of the image: slices [x0, x0 + TX) of the major axis,
+		// minor axis [y0, y0 + TY), z [z0, z0 + TZ), as the tile kernel's
+		// records restrict the sweep
+		const int TX = tile[0], TY = tile[1], TZ = tile[2];
+		for (int z0 = 0; z0 < P.nz; z0 += TZ)
+			for (int y0 = 0; y0 < P.ny; y0 += TY)
+				for (int x0 = 0; x0 < P.nx; x0 += TX)
+				{
+					DDRay R;
+					if (tof)
+						ddSetup<true, false>(P, a, b, na, nb, pr, tofv, true, R);
+					else
+						ddSetup<false, false>(P, a, b, na, nb, pr, tofv, true, R);
+					const int mlo = R.flag_y ? y0 : x0, mhi = (R.flag_y ? std::min(y0 + TY, P.ny)
+					                                                    : std::min(x0 + TX, P.nx)) - 1;
+					const int nlo = R.flag_y ? x0 : y0, nhi = (R.flag_y ? std::min(x0 + TX, P.nx)
+					                                                    : std::min(y0 + TY, P.ny)) - 1;
+					auto fn = [&](int io, int, int, int, float w) { tiled[io] += val * w; };
+					if (tof)
+						ddVisit<true>(P, R, pr, mlo, mhi, nlo, nhi, z0,
+						              std::min(z0 + TZ, P.nz) - 1, fn);
+					else
+						ddVisit<false>(P, R, pr, mlo, mhi, nlo, nhi, z0,
+						               std::min(z0 + TZ, P.nz) - 1, fn);
+				}
+	}
+	return acc;
 }
 
 }  // extern "C"

@@ -101,6 +101,17 @@ def test_host_time_average_full_range_and_dynamic(gpu):
           O.time_average_move_dynamic(x, oi, ts, xf, bounds), "dynamic")
     with pytest.raises(Exception, match="greater"):
         Y.timeAverageMoveImage(ts, xf, img, 50, 50)
+    # a motion of one frame: its frame covers the whole range (weight 1),
+    # over the full range and over a given one
+    one = Y.timeAverageMoveImage(ts[:1], xf[:1], img).transferToHostMemory()
+    want = O.move_accumulate(x[0], oi, O.invert_transform(xf[:1]), [1.0],
+                             np.zeros(x.shape[1:], np.float32))
+    check(one.reshape(x.shape[1:]), want, "one motion frame")
+    one = Y.timeAverageMoveImage(ts[:1], xf[:1], img, int(ts[0]), int(ts[0]) + 1000)
+    check(one.transferToHostMemory().reshape(x.shape[1:]), want, "one motion frame, range")
+    check(one.transferToHostMemory().reshape(x.shape[1:]),
+          O.time_average_move(x, oi, ts[:1], xf[:1], int(ts[0]), int(ts[0]) + 1000),
+          "one motion frame vs oracle driver")
 
 
 def test_cli_reconstruct_moves_the_sensitivity(gpu, tmp_path):

@@ -82,3 +82,29 @@ def test_scanner_checks_and_cpu_osem_refused():
     assert s.getNumDets() == 96 * 16
     with pytest.raises(RuntimeError, match="GPU OSEM only"):
         Y.createOSEM(s, use_gpu=False)
+
+
+def test_motion_frame_weights_match_oracle_and_one_frame():
+    """The frames and weights of a time-averaged moved image
+    (ReconstructionUtilsDevice.cu:146-190, LORMotion::getDuration): hand
+    values, the oracle's selection, and a one-frame motion (full range,
+    weight 1) that the reference's duration rule cannot serve."""
+    import oracle as O
+    ident = np.eye(3, 4, dtype=np.float32).reshape(1, -1)
+    f0, w = Y.motionFrameWeights([0, 10, 30], 0, 50)
+    assert f0 == 0 and np.allclose(w, [0.2, 0.4, 0.4])
+    for t, t0, t1 in [([0, 10, 30], 0, 50), ([0, 10, 30], 10, 30), ([0, 10, 30], 5, 40),
+                      ([3, 7, 8, 20], 3, 9), ([5], 5, 6), ([5], 0, 100)]:
+        f0, w = Y.motionFrameWeights(t, t0, t1)
+        frames, _, wo = O.time_average_frames(t, np.tile(ident[0], (len(t), 1)), t0, t1)
+        assert f0 == frames[0] and len(w) == len(frames), (t, t0, t1)
+        np.testing.assert_array_equal(np.float32(w), wo)
+    # one frame: the whole range, weight 1, and the full time range of it
+    assert Y.getFullTimeRange([5]) == (5, 6)
+    assert Y.motionFrameWeights([5], 5, 6) == (0, [1.0])
+    with pytest.raises(RuntimeError, match="greater"):
+        Y.motionFrameWeights([0, 10], 20, 10)
+    with pytest.raises(RuntimeError, match="no motion frame"):
+        Y.motionFrameWeights([0, 10], 11, 12)
+    with pytest.raises(RuntimeError, match="no motion frames"):
+        Y.motionFrameWeights([], 0, 1)

@@ -309,6 +309,11 @@ PYBIND11_MODULE(_pyyrtmi, m)
 	    .def("setDataInput",
 	         static_cast<void (OSEM_MI::*)(const Histogram3DDevice*)>(&OSEM_MI::setDataInput),
 	         "proj_data"_a, py::keep_alive<1, 2>())
+	    // batched histogram frames (the list is kept alive with the OSEM)
+	    .def("setDataInput",
+	         static_cast<void (OSEM_MI::*)(const std::vector<const Histogram3DDevice*>&)>(
+	             &OSEM_MI::setDataInput),
+	         "histogram_frames"_a, py::keep_alive<1, 2>())
 	    .def("isListModeEnabled", &OSEM_MI::isListModeEnabled)
 	    .def("setProjector", &OSEM_MI::setProjector, "projector_name"_a)
 	    .def("setNumRays", &OSEM_MI::setNumRays, "num_rays"_a)
@@ -381,7 +386,12 @@ PYBIND11_MODULE(_pyyrtmi, m)
 	    .def_readwrite("denomThreshold", &OSEM_MI::denomThreshold)
 	    .def_readwrite("useTilePlans", &OSEM_MI::useTilePlans)
 	    .def_readwrite("useLorOrder", &OSEM_MI::useLorOrder)
-	    .def_readwrite("lanesPerEvent", &OSEM_MI::lanesPerEvent);
+	    .def_readwrite("lanesPerEvent", &OSEM_MI::lanesPerEvent)
+	    .def_readwrite("useGraphs", &OSEM_MI::useGraphs)
+	    .def_readwrite("useAutotune", &OSEM_MI::useAutotune)
+	    .def_readwrite("memoryBudget", &OSEM_MI::memoryBudget)
+	    .def("numPlannedSubsets", &OSEM_MI::numPlannedSubsets)
+	    .def("numHistogramFrames", &OSEM_MI::numHistogramFrames);
 
 	m.def(
 	    "createOSEM",

@@ -228,6 +228,14 @@ void bindIO(py::module_& m, py::class_<Scanner>& scanner, py::class_<ImageParams
 	    "getFullTimeRange", [](CArray<int64_t> t) { return getFullTimeRange(toVector(t)); },
 	    "timestamps"_a);
 	m.def(
+	    "motionFrameWeights",
+	    [](CArray<int64_t> t, int64_t t0, int64_t t1) {
+		    std::vector<float> w;
+		    const int64_t f0 = motionFrameWeights(toVector(t), t0, t1, w);
+		    return std::make_pair(f0, w);
+	    },
+	    "timestamps"_a, "time_start"_a, "time_stop"_a);
+	m.def(
 	    "timeAverageMoveImage",
 	    [xfVec](CArray<int64_t> t, CArray<float> xf, const ImageDevice& img, py::object t0,
 	            py::object t1) {

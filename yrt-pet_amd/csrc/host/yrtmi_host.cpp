@@ -240,7 +240,7 @@ yrtmi_projector_desc ProjectorParams::desc(bool withTOF) const
 // Context
 // ---------------------------------------------------------------------------
 Context::Context(const Scanner& s, int device, void* stream)
-    : m_scanner(s), m_device(device)
+    : m_scanner(s), m_device(device), m_nullStream(stream == nullptr)
 {
 	check(yrtmi_ctx_create(device, stream, &m_ctx), "context");
 	const auto& pos = s.lutPositions();
@@ -264,6 +264,16 @@ Context::~Context()
 void Context::setStream(void* stream)
 {
 	check(yrtmi_ctx_set_stream(m_ctx, stream), "set stream");
+	m_nullStream = stream == nullptr;
+}
+
+void Context::ensureOwnStream()
+{
+	if (!m_nullStream)
+		return;
+	synchronize();
+	check(yrtmi_ctx_own_stream(m_ctx, nullptr), "own stream");
+	m_nullStream = false;
 }
 
 void Context::synchronize() const
@@ -563,27 +573,43 @@ struct PlanHandle
 	PlanHandle& operator=(const PlanHandle&) = delete;
 };
 
-std::unique_ptr<PlanHandle> makePlan(const Context& ctx, const OperatorProjectorMI& proj,
-                                     const yrtmi_image& im, const yrtmi_lors& l)
+// The default tile shape of a projector and image: Siddon 32 x 32 x 8 (flat
+// tiles suit the mostly transaxial rays; the fastest shape measured on
+// MI355X for list-mode TOF subsets and for sensitivity images); DD 16^3
+// (five workgroups per CU) while the tile count (frames included) stays
+// within the plan's bin limit (5459 tiles with 64 KB of LDS per
+// workgroup), 32 x 32 x 8 beyond it.
+int defaultTileShape(const OperatorProjectorMI& proj, const yrtmi_image& im)
 {
-	// Siddon: 32 x 32 x 8 tiles: flat tiles suit the mostly transaxial rays
-	// (fewer tile pieces per ray); the fastest shape measured on MI355X for
-	// list-mode TOF subsets and for sensitivity images.  DD: 16^3 tiles (five
-	// workgroups per CU) measured fastest, while the tile count (frames
-	// included) stays within the plan's bin limit (5459 tiles with 64 KB of
-	// LDS per workgroup); 32 x 32 x 8 beyond it.
-	int shape = YRTMI_TILE_32x32x8;
-	if (!proj.isSiddon())
-	{
-		const int64_t t16 = ((im.nx + 15) / 16) * (int64_t)((im.ny + 15) / 16) *
-		                    ((im.nz + 15) / 16) * im.nt;
-		if (t16 <= 5459)
-			shape = YRTMI_TILE_16x16x16;
-	}
-	const yrtmi_plan_options o{0, shape, 0};
+	if (proj.isSiddon())
+		return YRTMI_TILE_32x32x8;
+	const int64_t t16 =
+	    ((im.nx + 15) / 16) * (int64_t)((im.ny + 15) / 16) * ((im.nz + 15) / 16) * im.nt;
+	return t16 <= 5459 ? YRTMI_TILE_16x16x16 : YRTMI_TILE_32x32x8;
+}
+
+// A tile plan of `l`; with maxBytes > 0, null when the plan would hold more
+// device memory than that (or the allocation fails): the caller then uses
+// the unplanned kernels.
+std::unique_ptr<PlanHandle> makePlan(const Context& ctx, const OperatorProjectorMI& proj,
+                                     const yrtmi_image& im, const yrtmi_lors& l,
+                                     int shape = -1, int persistent = 0, int64_t maxBytes = 0)
+{
+	const yrtmi_plan_options o{0, shape >= 0 ? shape : defaultTileShape(proj, im), persistent,
+	                           maxBytes};
 	yrtmi_plan* q = nullptr;
-	check(yrtmi_plan_create(ctx.get(), proj.get(), &im, &l, &o, &q), "tile plan");
+	const int rc = yrtmi_plan_create(ctx.get(), proj.get(), &im, &l, &o, &q);
+	if (rc == YRTMI_ENOMEM && maxBytes > 0)
+		return nullptr;
+	check(rc, "tile plan");
 	return std::unique_ptr<PlanHandle>(new PlanHandle(q));
+}
+
+int64_t planBytes(const PlanHandle& p)
+{
+	int64_t b = 0;
+	check(yrtmi_plan_info(p.p, nullptr, nullptr, &b), "plan info");
+	return b;
 }
 }  // namespace
 
@@ -608,12 +634,25 @@ OSEM_MI::~OSEM_MI() = default;
 void OSEM_MI::setDataInput(const ListModeDevice* lm)
 {
 	m_lm = lm;
-	m_histo = nullptr;
+	m_histos.clear();
 }
 
 void OSEM_MI::setDataInput(const Histogram3DDevice* h)
 {
-	m_histo = h;
+	m_histos.clear();
+	if (h != nullptr)
+		m_histos.push_back(h);
+	m_lm = nullptr;
+}
+
+void OSEM_MI::setDataInput(const std::vector<const Histogram3DDevice*>& frames)
+{
+	for (const Histogram3DDevice* h : frames)
+		if (h == nullptr)
+			throw Error("setDataInput: a null histogram frame");
+	if (frames.empty())
+		throw Error("setDataInput: no histogram frames");
+	m_histos = frames;
 	m_lm = nullptr;
 }
 
@@ -641,23 +680,14 @@ std::pair<int64_t, int64_t> getFullTimeRange(const std::vector<int64_t>& t)
 	return {t[0], t[last] + (t[last] - t[last - 1])};
 }
 
-void timeAverageMoveImageDevice(const std::vector<int64_t>& t, const std::vector<float>& xf,
-                                const ImageDevice& unmoved, ImageDevice& out,
-                                int64_t timeStart, int64_t timeStop, int64_t outFrame)
+int64_t motionFrameWeights(const std::vector<int64_t>& t, int64_t timeStart, int64_t timeStop,
+                           std::vector<float>& w)
 {
 	const size_t k = t.size();
-	if (xf.size() != 12 * k)
-		throw Error("timeAverageMoveImage: need 12 transform values per motion frame");
 	if (k == 0)
 		throw Error("timeAverageMoveImage: no motion frames");
 	if (!(timeStop > timeStart))
 		throw Error("Time stop must be greater than time start");
-	const ImageParams& p = unmoved.getParams();
-	const ImageParams& q = out.getParams();
-	if (p.nx != q.nx || p.ny != q.ny || p.nz != q.nz)
-		throw Error("timeAverageMoveImage: input and output image sizes differ");
-	if (outFrame < 0 || outFrame >= q.nt)
-		throw Error("timeAverageMoveImage: output frame out of range");
 	// ReconstructionUtilsDevice.cu:146-166: the frames from the first start
 	// >= timeStart up to (excluding) the first start > timeStop
 	int64_t f0 = -1, f1 = (int64_t)k;
@@ -678,14 +708,38 @@ void timeAverageMoveImageDevice(const std::vector<int64_t>& t, const std::vector
 		throw Error("timeAverageMoveImage: no motion frame starts inside the time range");
 	const float scan = (float)(timeStop - timeStart);
 	const int n = (int)(f1 - f0);
-	std::vector<float> w(n);
+	w.assign(n, 0.f);
 	for (int i = 0; i < n; i++)
 	{
-		// LORMotion::getDuration (LORMotion.cpp:73-88)
+		// LORMotion::getDuration (LORMotion.cpp:73-88); one frame: the range
 		const size_t f = (size_t)(f0 + i);
+		if (k == 1)
+		{
+			w[i] = 1.f;
+			continue;
+		}
 		const int64_t d = f + 1 < k ? t[f + 1] - t[f] : t[f] - t[f - 1];
 		w[i] = (float)d / scan;
 	}
+	return f0;
+}
+
+void timeAverageMoveImageDevice(const std::vector<int64_t>& t, const std::vector<float>& xf,
+                                const ImageDevice& unmoved, ImageDevice& out,
+                                int64_t timeStart, int64_t timeStop, int64_t outFrame)
+{
+	const size_t k = t.size();
+	if (xf.size() != 12 * k)
+		throw Error("timeAverageMoveImage: need 12 transform values per motion frame");
+	const ImageParams& p = unmoved.getParams();
+	const ImageParams& q = out.getParams();
+	if (p.nx != q.nx || p.ny != q.ny || p.nz != q.nz)
+		throw Error("timeAverageMoveImage: input and output image sizes differ");
+	if (outFrame < 0 || outFrame >= q.nt)
+		throw Error("timeAverageMoveImage: output frame out of range");
+	std::vector<float> w;
+	const int64_t f0 = motionFrameWeights(t, timeStart, timeStop, w);
+	const int n = (int)w.size();
 	yrtmi_image d = p.desc(1);
 	check(yrtmi_time_average_move(unmoved.context().get(), &d, unmoved.getDevicePointer(),
 	                              out.getDevicePointer() + outFrame * (int64_t)q.numVoxels(),
@@ -943,7 +997,7 @@ const std::vector<std::unique_ptr<ImageDevice>>& OSEM_MI::generateSensitivityIma
 	return m_sens;
 }
 
-void OSEM_MI::prepareSubsets(OperatorProjectorMI& proj, const ImageParams& imt)
+void OSEM_MI::prepareSubsets(OperatorProjectorMI& proj, const ImageParams& imt, const float* x)
 {
 	m_subsets.clear();
 	const int S = num_OSEM_subsets;
@@ -1064,23 +1118,83 @@ void OSEM_MI::prepareSubsets(OperatorProjectorMI& proj, const ImageParams& imt)
 	}
 	else
 	{
+		const int F = (int)m_histos.size();
 		for (int s = 0; s < S; s++)
 		{
 			std::unique_ptr<Subset> q(new Subset());
 			const int64_t cap = yrtmi_histo_subset_size(c, S, s);
 			if (cap < 0)
 				throw Error("histogram subset out of range");
-			q->bins.reset(new DeviceArray<uint32_t>(*m_ctx, (size_t)std::max<int64_t>(cap, 1)));
-			q->meas.reset(new DeviceArray<float>(*m_ctx, (size_t)std::max<int64_t>(cap, 1)));
+			const size_t capA = (size_t)std::max<int64_t>(cap, 1);
 			int64_t n = 0;
 			// zero-count bins contribute nothing (OSEM_GPU.cu:1222-1225)
-			check(yrtmi_histo_subset_nonzero(c, m_histo->getDevicePointer(), S, s,
-			                                 q->bins->data(), q->meas->data(), &n),
-			      "nonzero bins");
+			if (F == 1)
+			{
+				q->bins.reset(new DeviceArray<uint32_t>(*m_ctx, capA));
+				q->meas.reset(new DeviceArray<float>(*m_ctx, capA));
+				check(yrtmi_histo_subset_nonzero(c, m_histos[0]->getDevicePointer(), S, s,
+				                                 q->bins->data(), q->meas->data(), &n),
+				      "nonzero bins");
+			}
+			else
+			{
+				// the subset's nonzero bins of every frame, frame after frame,
+				// each with its frame index (one pass counts, one copies)
+				DeviceArray<uint32_t> tb(*m_ctx, capA);
+				DeviceArray<float> tm(*m_ctx, capA);
+				std::vector<int64_t> nf(F, 0);
+				for (int f = 0; f < F; f++)
+				{
+					check(yrtmi_histo_subset_nonzero(c, m_histos[f]->getDevicePointer(), S, s,
+					                                 tb.data(), tm.data(), &nf[f]),
+					      "nonzero bins");
+					n += nf[f];
+				}
+				const size_t na = (size_t)std::max<int64_t>(n, 1);
+				q->bins.reset(new DeviceArray<uint32_t>(*m_ctx, na));
+				q->meas.reset(new DeviceArray<float>(*m_ctx, na));
+				q->frame.reset(new DeviceArray<int32_t>(*m_ctx, na));
+				int64_t off = 0;
+				for (int f = 0; f < F; f++)
+				{
+					if (nf[f] == 0)
+						continue;
+					int64_t k = 0;
+					check(yrtmi_histo_subset_nonzero(c, m_histos[f]->getDevicePointer(), S, s,
+					                                 tb.data(), tm.data(), &k),
+					      "nonzero bins");
+					check(yrtmi_copy(c, q->bins->data() + off, tb.data(), 4 * k, 2), "copy");
+					check(yrtmi_copy(c, q->meas->data() + off, tm.data(), 4 * k, 2), "copy");
+					check(yrtmi_fill_i32(c, q->frame->data() + off, k, f), "frame indices");
+					off += k;
+				}
+			}
 			q->lors.kind = YRTMI_SRC_BINS;
 			q->lors.n = n;
 			q->lors.bins = q->bins->data();
 			q->lors.num_subsets = 1;
+			q->lors.frame = q->frame ? q->frame->data() : nullptr;
+			if (useLorOrder && n > 1)
+			{
+				// the bins in locality order (frame first); the per-bin
+				// corrections gathered below follow
+				DeviceArray<uint32_t> perm(*m_ctx, (size_t)n);
+				check(yrtmi_lor_order(c, &ci, &q->lors, perm.data()), "locality order");
+				auto reorder = [&](auto& arr) {
+					using A = typename std::remove_reference<decltype(*arr)>::type;
+					if (!arr)
+						return;
+					std::unique_ptr<A> out(new A(*m_ctx, (size_t)n));
+					check(yrtmi_permute(c, arr->data(), 4, perm.data(), n, out->data()),
+					      "permute");
+					arr = std::move(out);
+				};
+				reorder(q->bins);
+				reorder(q->meas);
+				reorder(q->frame);
+				q->lors.bins = q->bins->data();
+				q->lors.frame = q->frame ? q->frame->data() : nullptr;
+			}
 			auto gatherHisto = [&](std::unique_ptr<DeviceArray<float>>& dst,
 			                       const Histogram3DDevice* h, bool invert) {
 				if (h == nullptr || n == 0)
@@ -1116,17 +1230,49 @@ void OSEM_MI::prepareSubsets(OperatorProjectorMI& proj, const ImageParams& imt)
 			m_subsets.push_back(std::move(q));
 		}
 	}
-	if (planned())
+	m_numPlanned = 0;
+	if (!planned())
+		return;
+	// launch shapes timed on the first nonempty subset
+	m_tileShape = -1;
+	m_persistent = 0;
+	if (useAutotune && x != nullptr)
 		for (auto& q : m_subsets)
 			if (q->lors.n > 0)
-				q->plan = makePlan(*m_ctx, proj, ci, q->lors);
+			{
+				yrtmi_tuning t{};
+				check(yrtmi_autotune(c, proj.get(), &ci, &q->lors, x, &t), "autotune");
+				m_tileShape = t.tile_shape;
+				m_persistent = t.persistent;
+				break;
+			}
+	// the plans within the memory budget, in subset order
+	int64_t budget = (int64_t)memoryBudget;
+	if (budget <= 0)
+	{
+		int64_t freeB = 0;
+		check(yrtmi_mem_info(c, &freeB, nullptr), "memory info");
+		budget = (int64_t)(0.9 * (double)freeB);
+	}
+	int64_t used = 0;
+	for (auto& q : m_subsets)
+	{
+		if (q->lors.n == 0 || used >= budget)
+			continue;
+		q->plan = makePlan(*m_ctx, proj, ci, q->lors, m_tileShape, m_persistent, budget - used);
+		if (q->plan)
+		{
+			used += planBytes(*q->plan);
+			m_numPlanned++;
+		}
+	}
 }
 
 std::unique_ptr<ImageDevice> OSEM_MI::reconstruct()
 {
 	if (!m_hasImageParams)
 		throw Error("Image parameters not set");
-	if (m_lm == nullptr && m_histo == nullptr)
+	if (m_lm == nullptr && m_histos.empty())
 		throw Error("Data input not set");
 	const int S = num_OSEM_subsets;
 	if (S <= 0 || num_MLEM_iterations < 0)
@@ -1139,7 +1285,7 @@ std::unique_ptr<ImageDevice> OSEM_MI::reconstruct()
 	yrtmi_ctx* c = m_ctx->get();
 	// setupForDynamicRecon (OSEM.cpp:823-830)
 	ImageParams imt = m_imageParams;
-	imt.nt = m_lm != nullptr ? m_lm->numFrames : 1;
+	imt.nt = m_lm != nullptr ? m_lm->numFrames : (int64_t)m_histos.size();
 	const int64_t nvox = (int64_t)imt.numVoxels();
 	const ImageParams im1 = singleFrame();
 	// list-mode sensitivity x 1/S on a copy (OSEM.cpp:1007-1035)
@@ -1177,7 +1323,7 @@ std::unique_ptr<ImageDevice> OSEM_MI::reconstruct()
 	                      0.f, 0.f, 0.f, 1.f, 0.f),
 	      "mask");
 	auto proj = makeProjector(true);
-	prepareSubsets(*proj, imt);
+	prepareSubsets(*proj, imt, x->getDevicePointer());
 	ImageDevice u(*m_ctx, imt);
 	const yrtmi_image ci = imt.desc();
 	// image-space PSF buffers: PSF(x), the blurred update image, scratch
@@ -1195,8 +1341,8 @@ std::unique_ptr<ImageDevice> OSEM_MI::reconstruct()
 		                      (int)m_psf[2].size(), transpose),
 		      "image PSF");
 	};
-	for (int it = 0; it < num_MLEM_iterations; it++)
-	{
+	// one OSEM iteration: every subset's EM update image and image update
+	auto iteration = [&]() {
 		for (int s = 0; s < S; s++)
 		{
 			Subset& q = *m_subsets[s];
@@ -1241,7 +1387,40 @@ std::unique_ptr<ImageDevice> OSEM_MI::reconstruct()
 			                      sImg.getDevicePointer(), nvox, imt.nt, 1, EPS_FLT),
 			      "EM update");
 		}
+	};
+	// the first iteration call by call (it also makes any per-XCD partial
+	// images of unplanned subsets), the next recorded once as a graph and
+	// replayed for every later one
+	int it = 0;
+	if (num_MLEM_iterations > 0)
+	{
+		iteration();
+		it = 1;
 	}
+	if (useGraphs && num_MLEM_iterations > 1)
+	{
+		m_ctx->ensureOwnStream();
+		yrtmi_graph* g = nullptr;
+		check(yrtmi_graph_begin(c), "graph begin");
+		try
+		{
+			iteration();
+		}
+		catch (...)
+		{
+			yrtmi_graph* partial = nullptr;
+			if (yrtmi_graph_end(c, &partial) == YRTMI_OK)
+				yrtmi_graph_destroy(partial);
+			throw;
+		}
+		check(yrtmi_graph_end(c, &g), "graph end");
+		std::unique_ptr<yrtmi_graph, int (*)(yrtmi_graph*)> gh(g, yrtmi_graph_destroy);
+		for (; it < num_MLEM_iterations; it++)
+			check(yrtmi_graph_launch(c, g), "graph launch");
+		m_ctx->synchronize();
+	}
+	for (; it < num_MLEM_iterations; it++)
+		iteration();
 	m_ctx->synchronize();
 	return x;
 }

@@ -145,6 +145,9 @@ public:
 	int64_t numR() const { return m_numR; }
 	int64_t histogramSize() const { return m_numZBin * m_numPhi * m_numR; }
 	void setStream(void* stream);
+	// Give the context a stream of its own when it runs on the null stream
+	// (graph recording needs one); waits for the null stream's work first.
+	void ensureOwnStream();
 	void synchronize() const;
 	// the motion table of the context's scanner tables (device state, not
 	// this object's)
@@ -154,6 +157,7 @@ private:
 	const Scanner& m_scanner;
 	int m_device;
 	yrtmi_ctx* m_ctx = nullptr;
+	bool m_nullStream = true;
 	int64_t m_numZBin = 0, m_numPhi = 0, m_numR = 0;
 };
 
@@ -331,7 +335,16 @@ public:
 	const std::vector<float>& imagePSFKernel(int axis) const { return m_psf[axis]; }
 	void setDataInput(const ListModeDevice* lm);
 	void setDataInput(const Histogram3DDevice* h);
+	// Batched histogram frames: frame f of the reconstructed image is the
+	// OSEM reconstruction of frames[f] (the reference reconstructs one
+	// Histogram3D at a time); the frames share the scanner's sensitivity
+	// images and every other correction, and each subset of all frames is
+	// projected in one launch (the nonzero bins of every frame, each with
+	// its frame index).
+	void setDataInput(const std::vector<const Histogram3DDevice*>& frames);
 	bool isListModeEnabled() const { return m_lm != nullptr; }
+	// Histogram frames of the data input (0 for list-mode).
+	int numHistogramFrames() const { return (int)m_histos.size(); }
 	void setProjector(const std::string& name) { projectorParams.setProjector(name); }
 	void setNumRays(int n) { projectorParams.numRays = n; }
 	void addTOF(float widthPs, int numStd) { projectorParams.addTOF(widthPs, numStd); }
@@ -382,6 +395,22 @@ public:
 	// (yrtmi_lor_order); the EM update does not depend on the order.
 	bool useLorOrder = true;
 	int lanesPerEvent = 0;  // 0: chosen from the number of rays
+	// Launch one OSEM iteration as a recorded graph: the first iteration
+	// runs call by call, the next is recorded once and replayed for every
+	// later iteration (the subset loop of OSEM.cpp:788-811).
+	bool useGraphs = true;
+	// Before the tile plans are built: lanes per event, the tile shape and a
+	// persistent grid chosen by timing them on the first subset
+	// (yrtmi_autotune).
+	bool useAutotune = true;
+	// Device memory the subsets' tile plans may hold, bytes: 0 = 90 % of
+	// the device's free memory when they are built (OSEM_GPU sizes its
+	// batches from that share, OSEM_GPU.cu:413-435, or from
+	// OperatorProjectorDevice::create's memAvailable).  Subsets whose plan
+	// does not fit (in subset order) run the unplanned EM kernel.
+	size_t memoryBudget = 0;
+	// Subsets that got a tile plan in the last reconstruct().
+	int numPlannedSubsets() const { return m_numPlanned; }
 
 	const Context& getContext() const { return *m_ctx; }
 
@@ -389,7 +418,9 @@ private:
 	struct Subset;
 	std::unique_ptr<OperatorProjectorMI> makeProjector(bool withTOF) const;
 	bool planned() const;
-	void prepareSubsets(OperatorProjectorMI& proj, const ImageParams& imt);
+	// The subsets' device data, corrections and tile plans; x: the initial
+	// image (for the autotuner)
+	void prepareSubsets(OperatorProjectorMI& proj, const ImageParams& imt, const float* x);
 	ImageParams singleFrame() const;
 
 	std::unique_ptr<Context> m_ctx;
@@ -397,7 +428,9 @@ private:
 	ImageParams m_imageParams;
 	bool m_hasImageParams = false;
 	const ListModeDevice* m_lm = nullptr;
-	const Histogram3DDevice* m_histo = nullptr;
+	std::vector<const Histogram3DDevice*> m_histos;
+	int m_numPlanned = 0;
+	int m_tileShape = -1, m_persistent = 0;  // from the autotuner
 	float m_globalScale = 1.0f;
 	const ImageDevice* m_att = nullptr;
 	const ImageDevice *m_hwAtt = nullptr, *m_ivAtt = nullptr;
@@ -432,6 +465,16 @@ std::vector<float> gaussianKernel1D(float sigma, float voxelSize, size_t size = 
 // first start + 1] for one frame, else [first start, last start + the
 // second-to-last frame's duration].
 std::pair<int64_t, int64_t> getFullTimeRange(const std::vector<int64_t>& timestamps);
+// The motion frames a time average over [timeStart, timeStop] takes and their
+// weights (ReconstructionUtilsDevice.cu:146-190): the frames from the first
+// start >= timeStart up to (excluding) the first start > timeStop, each
+// weighted LORMotion::getDuration (LORMotion.cpp:73-88) / (timeStop -
+// timeStart).  Returns the first frame; `weights` receives one weight per
+// frame taken.  A motion of one frame has no second-to-last frame to take a
+// duration from (the reference reads before its records there): its frame
+// lasts the whole range and gets weight 1.
+int64_t motionFrameWeights(const std::vector<int64_t>& timestamps, int64_t timeStart,
+                           int64_t timeStop, std::vector<float>& weights);
 // util::timeAverageMoveImageDevice (ReconstructionUtilsDevice.cu:120-225):
 // adds to frame `outFrame` of `out` the copies of frame 0 of `unmoved`
 // moved by every motion frame starting in [timeStart, timeStop] (both

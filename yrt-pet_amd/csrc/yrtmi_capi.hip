@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -84,6 +85,7 @@ struct yrtmi_proj
 	int device = 0;
 	DProj d{};
 	float* psf = nullptr;  // numS*K taps followed by the flipped copy
+	double* psfCum = nullptr;  // numS*(K+2) trapezoid prefix sums, then the flipped copy's
 	int lpe = 1;
 	int block = 256;
 	int64_t grid = 0;
@@ -685,18 +687,35 @@ int yrtmi_proj_create(yrtmi_ctx* c, const yrtmi_projector_desc* d,
 				tab[i * d->psf_ksize + j] = v;
 				tab[nk + i * d->psf_ksize + (d->psf_ksize - 1 - j)] = v;
 			}
+		// prefix sums of each row's trapezoids kSpacing * (tap(xi) + tap(xi+1))
+		// / 2 over xi = -1 .. K - 1 (taps outside [0, K) are 0), in FP64: the
+		// integral over whole kernel segments is a difference of two entries
+		const int64_t K = d->psf_ksize, KC = K + 2;
+		std::vector<double> cum(2 * (size_t)d->psf_num_s * KC);
+		psfPrefixSums(tab.data(), d->psf_num_s, K, d->psf_k_spacing, cum.data());
+		psfPrefixSums(tab.data() + nk, d->psf_num_s, K, d->psf_k_spacing,
+		              cum.data() + (size_t)d->psf_num_s * KC);
 		hipError_t e = hipMalloc(&p->psf, tab.size() * sizeof(float));
 		if (e == hipSuccess)
 			e = hipMemcpy(p->psf, tab.data(), tab.size() * sizeof(float),
 			              hipMemcpyHostToDevice);
+		if (e == hipSuccess)
+			e = hipMalloc(&p->psfCum, cum.size() * sizeof(double));
+		if (e == hipSuccess)
+			e = hipMemcpy(p->psfCum, cum.data(), cum.size() * sizeof(double),
+			              hipMemcpyHostToDevice);
 		if (e != hipSuccess)
 		{
 			(void)hipFree(p->psf);
+			(void)hipFree(p->psfCum);
 			delete p;
 			return hipFail(e, "yrtmi_proj_create (PSF upload)");
 		}
 		q.psf.k = p->psf;
 		q.psf.kflip = p->psf + nk;
+		q.psf.cum = p->psfCum;
+		q.psf.cumFlip = p->psfCum + (size_t)d->psf_num_s * KC;
+		q.psf.rks = rcpNearest(d->psf_k_spacing);
 		q.psf.numS = d->psf_num_s;
 		q.psf.K = d->psf_ksize;
 		q.psf.sStep = d->psf_s_step;
@@ -718,6 +737,7 @@ int yrtmi_proj_destroy(yrtmi_proj* p)
 		// hipFree waits for the device's outstanding work
 		(void)hipSetDevice(p->device);
 		(void)hipFree(p->psf);
+		(void)hipFree(p->psfCum);
 	}
 	delete p;
 	return YRTMI_OK;
@@ -1082,29 +1102,25 @@ int yrtmi_time_average_move(yrtmi_ctx* c, const yrtmi_image* im, const float* in
 	return YRTMI_OK;
 }
 
-int yrtmi_histo_gather_pairs(yrtmi_ctx* c, const float* histo, const uint32_t* d1,
-                             const uint32_t* d2, int64_t n, float* out)
+}  // extern "C"
+
+namespace
 {
-	int r = setDev(c);
-	if (r)
-		return r;
-	r = notCapturing(c, "histo_gather_pairs");
-	if (r)
-		return r;
+// The device ring map of the context's histogram (the inverse of the
+// in-ring pair of every (rr, phi), Histogram3D::setupHistogram's m_ringMap,
+// Histogram3D.cpp:286-297, with the pair formula of getDetPairInSameRing,
+// 428-450: phi outer, rr inner, a later (rr, phi) replacing an earlier one
+// with the same pair, pairs of one in-ring position included) and the pair
+// geometry; shared by the pair lookups.
+int pairSetup(yrtmi_ctx* c, const char* what, PairGeom& g)
+{
 	if (!c->histoReady)
-		return fail(YRTMI_EINVAL, "histo_gather_pairs: no histogram geometry");
-	if (histo == nullptr || d1 == nullptr || d2 == nullptr || out == nullptr || n < 0)
-		return fail(YRTMI_EINVAL, "histo_gather_pairs: bad arguments");
-	if (n == 0)
-		return YRTMI_OK;
+		return fail(YRTMI_EINVAL, std::string(what) + ": no histogram geometry");
 	const int64_t dpr = c->histo.dets_per_ring;
 	if (dpr >= 65536)
-		return fail(YRTMI_EINVAL, "histo_gather_pairs: too many detectors per ring");
+		return fail(YRTMI_EINVAL, std::string(what) + ": too many detectors per ring");
 	if (c->ringMap == nullptr)
 	{
-		// the inverse of the in-ring pair of every (rr, phi)
-		// (Histogram3D::setupHistogram's ring map, Histogram3D.cpp:286-297),
-		// with the pair formula of getDetPairInSameRing (428-450)
 		std::vector<int32_t> map((size_t)(dpr * dpr), -1);
 		const int64_t nrr = c->numR / (c->histo.num_doi * c->histo.num_doi);
 		for (int64_t phi = 0; phi < c->numPhi; phi++)
@@ -1121,8 +1137,7 @@ int yrtmi_histo_gather_pairs(yrtmi_ctx* c, const float* histo, const uint32_t* d
 				b = (b + phi / 2) % dpr;
 				if (a > b)
 					std::swap(a, b);
-				if (a != b)
-					map[(size_t)(a * dpr + b)] = (int32_t)(rr | (phi << 16));
+				map[(size_t)(a * dpr + b)] = (int32_t)(rr | (phi << 16));
 			}
 		HIP_TRY(hipStreamSynchronize(c->stream));
 		HIP_TRY(hipMalloc(&c->ringMap, map.size() * sizeof(int32_t)));
@@ -1131,8 +1146,7 @@ int yrtmi_histo_gather_pairs(yrtmi_ctx* c, const float* histo, const uint32_t* d
 	}
 	if (c->gatherErr == nullptr)
 		HIP_TRY(hipMalloc(&c->gatherErr, sizeof(int)));
-	HIP_TRY(hipMemsetAsync(c->gatherErr, 0, sizeof(int), c->stream));
-	PairGeom g{};
+	g = PairGeom{};
 	g.dpr = (uint32_t)dpr;
 	g.nrings = (uint32_t)c->histo.num_rings;
 	g.ndet = (uint32_t)(dpr * c->histo.num_rings * c->histo.num_doi);
@@ -1142,14 +1156,79 @@ int yrtmi_histo_gather_pairs(yrtmi_ctx* c, const float* histo, const uint32_t* d
 	g.numPhi = c->numPhi;
 	g.numZBinDiff = c->numZBinDiff;
 	g.maxRingDiff = (int32_t)c->histo.max_ring_diff;
-	HIP_TRY(launchGatherPairs(histo, d1, d2, n, c->ringMap, g, out, c->gatherErr, c->stream));
+	return YRTMI_OK;
+}
+
+int pairErr(yrtmi_ctx* c, const char* what)
+{
 	int err = 0;
 	HIP_TRY(hipMemcpyAsync(&err, c->gatherErr, sizeof(int), hipMemcpyDeviceToHost, c->stream));
 	HIP_TRY(hipStreamSynchronize(c->stream));
 	if (err)
-		return fail(YRTMI_EINVAL, "histo_gather_pairs: a detector pair outside the histogram "
-		                          "(unknown detector, same in-ring position or ring difference "
-		                          "above the limit)");
+		return fail(YRTMI_EINVAL, std::string(what) +
+		                              ": a detector pair outside the histogram (unknown "
+		                              "detector, no in-ring bin, the pair (0, 0) or a ring "
+		                              "difference above the limit)");
+	return YRTMI_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int yrtmi_histo_gather_pairs(yrtmi_ctx* c, const float* histo, const uint32_t* d1,
+                             const uint32_t* d2, int64_t n, float* out)
+{
+	int r = setDev(c);
+	if (r)
+		return r;
+	r = notCapturing(c, "histo_gather_pairs");
+	if (r)
+		return r;
+	if (histo == nullptr || d1 == nullptr || d2 == nullptr || out == nullptr || n < 0)
+		return fail(YRTMI_EINVAL, "histo_gather_pairs: bad arguments");
+	if (n == 0)
+		return YRTMI_OK;
+	PairGeom g;
+	r = pairSetup(c, "histo_gather_pairs", g);
+	if (r)
+		return r;
+	HIP_TRY(hipMemsetAsync(c->gatherErr, 0, sizeof(int), c->stream));
+	HIP_TRY(launchGatherPairs(histo, d1, d2, n, c->ringMap, g, out, c->gatherErr, c->stream));
+	return pairErr(c, "histo_gather_pairs");
+}
+
+int yrtmi_histo_pair_bins(yrtmi_ctx* c, const uint32_t* d1, const uint32_t* d2, int64_t n,
+                          uint32_t* bins)
+{
+	int r = setDev(c);
+	if (r)
+		return r;
+	r = notCapturing(c, "histo_pair_bins");
+	if (r)
+		return r;
+	if (d1 == nullptr || d2 == nullptr || bins == nullptr || n < 0)
+		return fail(YRTMI_EINVAL, "histo_pair_bins: bad arguments");
+	if (n == 0)
+		return YRTMI_OK;
+	PairGeom g;
+	r = pairSetup(c, "histo_pair_bins", g);
+	if (r)
+		return r;
+	if ((int64_t)c->numZBin * c->numPhi * c->numR >= 0xffffffffLL)
+		return fail(YRTMI_EINVAL, "histo_pair_bins: the histogram has 2^32 - 1 bins or more");
+	HIP_TRY(hipMemsetAsync(c->gatherErr, 0, sizeof(int), c->stream));
+	HIP_TRY(launchPairBins(d1, d2, n, c->ringMap, g, bins, c->gatherErr, c->stream));
+	return pairErr(c, "histo_pair_bins");
+}
+
+int yrtmi_fill_i32(yrtmi_ctx* c, int32_t* x, int64_t n, int32_t v)
+{
+	int r = setDev(c);
+	if (r)
+		return r;
+	if (n < 0 || (n > 0 && x == nullptr))
+		return fail(YRTMI_EINVAL, "fill_i32: bad arguments");
+	HIP_TRY(launchFillI32(x, n, v, c->stream));
 	return YRTMI_OK;
 }
 
@@ -1310,13 +1389,23 @@ int yrtmi_plan_create(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
 	const int R = p->d.numRays;
 	if ((l->n * (int64_t)R) >= (int64_t(1) << 32))
 		return fail(YRTMI_EINVAL, "plan: n * num_rays must be below 2^32");
-	const yrtmi_plan_options o = opt ? *opt : yrtmi_plan_options{0, 0, 0};
+	const yrtmi_plan_options o = opt ? *opt : yrtmi_plan_options{0, 0, 0, 0};
 	if (o.chunk_records < 0 || o.chunk_records > (1 << 24))
 		return fail(YRTMI_EINVAL, "plan: chunk_records out of range");
 	if (o.tile_shape < 0 || o.tile_shape >= kNumTileShapes)
 		return fail(YRTMI_EINVAL, "plan: unknown tile shape");
 	if (o.persistent < 0)
 		return fail(YRTMI_EINVAL, "plan: persistent workgroups must be >= 0");
+	if (o.max_bytes < 0)
+		return fail(YRTMI_EINVAL, "plan: max_bytes must be >= 0");
+	// the plan's memory within the budget: the value buffer and ray table
+	// now, the records after the count pass
+	// ray table: six float4 per DD LOR (its set-up), two per Siddon ray (its
+	// end points) and two more with the FP32 set-up (rayTabVec)
+	const int64_t tabVec = dd ? 6 : rayTabVec(p->d.setup64 != 0);
+	const int64_t tabBytes = tabVec * (int64_t)sizeof(float4) * l->n * (int64_t)R;
+	if (o.max_bytes > 0 && (int64_t)sizeof(float) * l->n + tabBytes > o.max_bytes)
+		return fail(YRTMI_ENOMEM, "plan: the ray table exceeds the memory budget");
 	const int CH = o.chunk_records > 0 ? o.chunk_records : 4096;
 	DTiles T{};
 	T.shape = o.tile_shape;
@@ -1427,8 +1516,6 @@ int yrtmi_plan_create(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
 	    hipMalloc(&dOff, sizeof(int64_t) * nB) != hipSuccess ||
 	    hipMalloc(&dErr, sizeof(int)) != hipSuccess)
 		return cleanup(fail(YRTMI_ENOMEM, "plan: build scratch"));
-	// ray table: two float4 per Siddon ray, six per DD LOR (its set-up)
-	const int64_t tabVec = dd ? 6 : 2;
 	if (hipMalloc(&q->ray, tabVec * sizeof(float4) * rays) != hipSuccess)
 		return cleanup(fail(YRTMI_ENOMEM, "plan: ray table"));
 	q->bytes += rays * tabVec * (int64_t)sizeof(float4);
@@ -1485,6 +1572,10 @@ int yrtmi_plan_create(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
 	}
 	if ((int64_t)chunks.size() >= 0x7fffffffLL)
 		return cleanup(fail(YRTMI_EINVAL, "plan: too many chunks"));
+	if (o.max_bytes > 0 &&
+	    q->bytes + total * (dd ? 8 : 12) + (int64_t)(sizeof(TileChunk) * chunks.size()) >
+	        o.max_bytes)
+		return cleanup(fail(YRTMI_ENOMEM, "plan: the records exceed the memory budget"));
 	// full chunks first, the tiles' partial tail chunks last (they end the
 	// launch while the full ones are still running)
 	std::stable_sort(chunks.begin(), chunks.end(),
@@ -1702,6 +1793,20 @@ int yrtmi_sensitivity_planned(yrtmi_ctx* c, const yrtmi_proj* p,
 }  // extern "C"
 
 extern "C" {
+
+int yrtmi_mem_info(yrtmi_ctx* c, int64_t* free_bytes, int64_t* total_bytes)
+{
+	int r = setDev(c);
+	if (r)
+		return r;
+	size_t fr = 0, tot = 0;
+	HIP_TRY(hipMemGetInfo(&fr, &tot));
+	if (free_bytes != nullptr)
+		*free_bytes = (int64_t)fr;
+	if (total_bytes != nullptr)
+		*total_bytes = (int64_t)tot;
+	return YRTMI_OK;
+}
 
 int yrtmi_device_alloc(yrtmi_ctx* c, int64_t bytes, void** out)
 {
@@ -1969,8 +2074,6 @@ int yrtmi_autotune(yrtmi_ctx* c, yrtmi_proj* p, const yrtmi_image* im,
 		return r;
 	if (best == nullptr || x == nullptr)
 		return fail(YRTMI_EINVAL, "autotune: null argument");
-	if (p->d.type != YRTMI_PROJ_SIDDON)
-		return fail(YRTMI_EINVAL, "autotune: Siddon projector only");
 	*best = yrtmi_tuning{};
 	if (l->n == 0)
 		return YRTMI_OK;
@@ -2014,9 +2117,13 @@ int yrtmi_autotune(yrtmi_ctx* c, yrtmi_proj* p, const yrtmi_image* im,
 		ms *= 0.5f;
 		return e;
 	};
+	// lanes per event split a LOR's rays: distance-driven LORs have one
 	const int lpes[4] = {1, 2, 4, 8};
+	const int nLpe = (p->d.type == YRTMI_PROJ_DD) ? 1 : 4;
 	best->ratio_ms = 1e30f;
 	for (int i = 0; i < 4; i++)
+		best->ratio_ms_lpe[i] = -1.f;
+	for (int i = 0; i < nLpe; i++)
 	{
 		float ms = 0.f;
 		hipError_t e = timeIt([&]() { return launchEmRatio(a, lpes[i], v, c->stream); }, ms);
@@ -2034,7 +2141,7 @@ int yrtmi_autotune(yrtmi_ctx* c, yrtmi_proj* p, const yrtmi_image* im,
 	best->back_ms = 1e30f;
 	for (int shape = 0; shape < kNumTileShapes; shape++)
 	{
-		yrtmi_plan_options o{0, shape, 0};
+		yrtmi_plan_options o{0, shape, 0, 0};
 		yrtmi_plan* q = nullptr;
 		best->back_ms_shape[shape][0] = best->back_ms_shape[shape][1] = -1.f;
 		r = yrtmi_plan_create(c, p, im, l, &o, &q);

@@ -16,6 +16,39 @@
 namespace yrtmi
 {
 
+// FP32 operations each rounded once to nearest and never fused with a
+// neighbouring one (contraction is off in these bodies, so the compiler
+// cannot turn a product and a sum into an FMA even after inlining): the
+// reference's separately rounded float arithmetic where a result must match
+// it bit for bit.  The HIP headers' __fadd_rn / __fmul_rn are plain,
+// contractible operators and __fsqrt_rn a native (not correctly rounded)
+// square root, so they cannot serve here.  sqrtf and '/' are correctly
+// rounded in this build (hipcc's default for FP32 division and sqrt).
+__device__ __forceinline__ float fmulRn(float a, float b)
+{
+#pragma clang fp contract(off)
+	return a * b;
+}
+__device__ __forceinline__ float faddRn(float a, float b)
+{
+#pragma clang fp contract(off)
+	return a + b;
+}
+__device__ __forceinline__ float fsubRn(float a, float b)
+{
+#pragma clang fp contract(off)
+	return a - b;
+}
+__device__ __forceinline__ float fdivRn(float a, float b)
+{
+#pragma clang fp contract(off)
+	return a / b;
+}
+__device__ __forceinline__ float fsqrtRn(float a)
+{
+	return sqrtf(a);
+}
+
 constexpr int kMaxRays = 16;
 
 enum Mode : int
@@ -38,7 +71,8 @@ struct DImage
 
 // TimeOfFlightHelper constants (TimeOfFlight.hpp:19-34): sigma, truncation
 // width (<= 0: none) and Gaussian norm, all mm; rsigma = RN(1/sigma), with
-// which the walk divides by sigma exactly (divBySigma).
+// which the distance-driven slab weight divides by sigma exactly
+// (divBySigma).
 struct DTof
 {
 	float sigma, trunc, norm, rsigma;
@@ -65,6 +99,29 @@ inline float rcpNearest(float s)
 	return best;
 }
 
+// Prefix sums of the trapezoids of each row of `taps` (numS rows of K):
+// out[r * (K + 2) + m + 1] = sum over xi in [-1, m) of kSpacing * (tap(xi) +
+// tap(xi + 1)) / 2 in FP64, taps outside [0, K) being 0, out[r * (K + 2)] =
+// 0.  The integral of a row's linear interpolant over whole kernel segments
+// [a, b) is then out[b + 1] - out[a + 1] (DPsf::cum).
+inline void psfPrefixSums(const float* taps, int64_t numS, int64_t K, double kSpacing,
+                          double* out)
+{
+	const int64_t KC = K + 2;
+	for (int64_t i = 0; i < numS; i++)
+	{
+		const float* row = taps + i * K;
+		double* c = out + i * KC;
+		c[0] = 0.0;
+		for (int64_t m = 0; m <= K; m++)
+		{
+			const double a = (m - 1 >= 0) ? (double)row[m - 1] : 0.0;
+			const double b = (m < K) ? (double)row[m] : 0.0;
+			c[m + 1] = c[m] + kSpacing * 0.5 * (a + b);
+		}
+	}
+}
+
 // The same constants in FP64 (the FP64 projector).
 struct DTof64
 {
@@ -76,8 +133,14 @@ struct DPsf
 {
 	const float* k;      // numS x K
 	const float* kflip;  // numS x K, reversed taps (backprojection)
+	// numS x (K + 2) prefix sums of the kernel's trapezoids (FP64): row r,
+	// entry m + 1 = sum over xi in [-1, m) of kSpacing * (tap(xi) +
+	// tap(xi + 1)) / 2, taps outside [0, K) being 0; cumFlip for kflip
+	const double* cum;
+	const double* cumFlip;
 	int32_t numS, K;
 	float sStep, kSpacing, halfWidth;
+	float rks;  // RN(1 / kSpacing)
 };
 
 struct DProj
@@ -268,14 +331,14 @@ __device__ __forceinline__ float3 lutOri(const DScanner& S, uint32_t d)
 __device__ __forceinline__ float3 moveP(const float* m, float3 p)
 {
 	float3 o;
-	o.x = __fadd_rn(__fadd_rn(__fadd_rn(__fmul_rn(m[0], p.x), __fmul_rn(m[1], p.y)),
-	                          __fmul_rn(m[2], p.z)),
+	o.x = faddRn(faddRn(faddRn(fmulRn(m[0], p.x), fmulRn(m[1], p.y)),
+	                          fmulRn(m[2], p.z)),
 	                m[3]);
-	o.y = __fadd_rn(__fadd_rn(__fadd_rn(__fmul_rn(m[4], p.x), __fmul_rn(m[5], p.y)),
-	                          __fmul_rn(m[6], p.z)),
+	o.y = faddRn(faddRn(faddRn(fmulRn(m[4], p.x), fmulRn(m[5], p.y)),
+	                          fmulRn(m[6], p.z)),
 	                m[7]);
-	o.z = __fadd_rn(__fadd_rn(__fadd_rn(__fmul_rn(m[8], p.x), __fmul_rn(m[9], p.y)),
-	                          __fmul_rn(m[10], p.z)),
+	o.z = faddRn(faddRn(faddRn(fmulRn(m[8], p.x), fmulRn(m[9], p.y)),
+	                          fmulRn(m[10], p.z)),
 	                m[11]);
 	return o;
 }
@@ -322,13 +385,13 @@ __device__ __forceinline__ Lor fetchLor(const DScanner& S, const DLors& L,
 __device__ __forceinline__ float3 transDir(float3 n)
 {
 	// normalize(n x z) with Vector3D::normalize's 1e-24 guard
-	const float cx = __fsub_rn(__fmul_rn(n.y, 1.f), __fmul_rn(n.z, 0.f));
-	const float cy = __fsub_rn(__fmul_rn(n.z, 0.f), __fmul_rn(n.x, 1.f));
-	const float cz = __fsub_rn(__fmul_rn(n.x, 0.f), __fmul_rn(n.y, 0.f));
-	const float nn = __fsqrt_rn(
-	    __fadd_rn(__fadd_rn(__fmul_rn(cx, cx), __fmul_rn(cy, cy)), __fmul_rn(cz, cz)));
+	const float cx = fsubRn(fmulRn(n.y, 1.f), fmulRn(n.z, 0.f));
+	const float cy = fsubRn(fmulRn(n.z, 0.f), fmulRn(n.x, 1.f));
+	const float cz = fsubRn(fmulRn(n.x, 0.f), fmulRn(n.y, 0.f));
+	const float nn = fsqrtRn(
+	    faddRn(faddRn(fmulRn(cx, cx), fmulRn(cy, cy)), fmulRn(cz, cz)));
 	if ((double)nn > 10e-25)
-		return make_float3(__fdiv_rn(cx, nn), __fdiv_rn(cy, nn), __fdiv_rn(cz, nn));
+		return make_float3(fdivRn(cx, nn), fdivRn(cy, nn), fdivRn(cz, nn));
 	return make_float3(0.f, 0.f, 0.f);
 }
 
@@ -344,14 +407,14 @@ __device__ __forceinline__ void rayLine(const DProj& pr, const Lor& l, int r,
 	const float* o = pr.rayOff + 4 * (r - 1);
 	const float3 t1 = transDir(l.n1);
 	const float3 t2 = transDir(l.n2);
-	const float s1 = __fmul_rn(o[0], pr.tz), s2 = __fmul_rn(o[1], pr.tt);
-	const float s3 = __fmul_rn(o[2], pr.tz), s4 = __fmul_rn(o[3], pr.tt);
-	q1.x = __fadd_rn(__fadd_rn(l.p1.x, __fmul_rn(0.f, s1)), __fmul_rn(t1.x, s2));
-	q1.y = __fadd_rn(__fadd_rn(l.p1.y, __fmul_rn(0.f, s1)), __fmul_rn(t1.y, s2));
-	q1.z = __fadd_rn(__fadd_rn(l.p1.z, __fmul_rn(1.f, s1)), __fmul_rn(t1.z, s2));
-	q2.x = __fsub_rn(__fadd_rn(l.p2.x, __fmul_rn(0.f, s3)), __fmul_rn(t2.x, s4));
-	q2.y = __fsub_rn(__fadd_rn(l.p2.y, __fmul_rn(0.f, s3)), __fmul_rn(t2.y, s4));
-	q2.z = __fsub_rn(__fadd_rn(l.p2.z, __fmul_rn(1.f, s3)), __fmul_rn(t2.z, s4));
+	const float s1 = fmulRn(o[0], pr.tz), s2 = fmulRn(o[1], pr.tt);
+	const float s3 = fmulRn(o[2], pr.tz), s4 = fmulRn(o[3], pr.tt);
+	q1.x = faddRn(faddRn(l.p1.x, fmulRn(0.f, s1)), fmulRn(t1.x, s2));
+	q1.y = faddRn(faddRn(l.p1.y, fmulRn(0.f, s1)), fmulRn(t1.y, s2));
+	q1.z = faddRn(faddRn(l.p1.z, fmulRn(1.f, s1)), fmulRn(t1.z, s2));
+	q2.x = fsubRn(faddRn(l.p2.x, fmulRn(0.f, s3)), fmulRn(t2.x, s4));
+	q2.y = fsubRn(faddRn(l.p2.y, fmulRn(0.f, s3)), fmulRn(t2.y, s4));
+	q2.z = fsubRn(faddRn(l.p2.z, fmulRn(1.f, s3)), fmulRn(t2.z, s4));
 }
 
 // ---------------------------------------------------------------------------
@@ -377,8 +440,23 @@ __device__ __forceinline__ float psfTap(const float* k, int K, int i)
 	return (i < 0 || i >= K) ? 0.f : k[i];
 }
 
-__device__ float psfWeight(const DPsf& P, const float* k, float x0, float x1)
+// x / y from ry = RN(1/y): q = RN(x*ry) corrected by one FMA residual step
+// (P. Markstein, IBM J. Res. Dev. 34(1), 1990): RN(x/y) for finite x, y away
+// from the overflow and underflow thresholds.
+__device__ __forceinline__ float divByRecip(float x, float y, float ry)
 {
+#pragma clang fp contract(off)
+	const float q = x * ry;
+	return fmaf(fmaf(-q, y, x), ry, q);
+}
+
+// The integral of the taps' linear interpolant over [x0, x1]: the partial
+// trapezoids at both ends as the reference computes them, the whole
+// trapezoids between them from the row's FP64 prefix table `cum` (the
+// reference adds them one by one in FP32; the two sums differ by rounding).
+__device__ float psfWeight(const DPsf& P, const float* k, const double* cum, float x0, float x1)
+{
+#pragma clang fp contract(off)
 	const int hw = (P.K + 1) / 2;
 	const float ks = P.kSpacing;
 	if (x0 > hw * ks || x1 < -hw * ks || x0 >= x1)
@@ -387,13 +465,13 @@ __device__ float psfWeight(const DPsf& P, const float* k, float x0, float x1)
 	const int half = (K - 1) / 2;
 	if (x0 > (half + 1) * ks || x1 < -(half + 1) * ks)
 		return 0.f;
-	const float x0s = x0 / ks;
+	const float x0s = divByRecip(x0, ks, P.rks);
 	const int x0i = (int)floorf(x0s + (float)half);
 	const float tau0 = x0s + (float)(-(-half + x0i));
 	const float v0lo = psfTap(k, K, x0i);
 	const float v0hi = (x0i < -1 || x0i >= K - 1) ? 0.f : k[x0i + 1];
 	const float v0 = v0lo * (1.f - tau0) + v0hi * tau0;
-	const float x1s = x1 / ks;
+	const float x1s = divByRecip(x1, ks, P.rks);
 	const int x1i = (int)floorf(x1s + (float)half);
 	const float tau1 = x1s - (float)(-half + x1i);
 	const float v1lo = psfTap(k, K, x1i);
@@ -408,8 +486,11 @@ __device__ float psfWeight(const DPsf& P, const float* k, float x0, float x1)
 	{
 		out += 0.5f * (v0 + v0hi) * ks * (1.f - tau0);
 		out += 0.5f * (v1lo + v1) * ks * tau1;
-		for (int xi = x0i + 1; xi < x1i; xi++)
-			out += ks * 0.5f * (psfTap(k, K, xi) + psfTap(k, K, xi + 1));
+		// whole trapezoids xi in [x0i + 1, x1i); those below xi = -1 and
+		// from xi = K on are zero (x0 may lie far below the kernel)
+		const int ia = max(x0i + 1, -1), ib = min(x1i, K);
+		if (ib > ia)
+			out += (float)(cum[ib + 1] - cum[ia + 1]);
 	}
 	return out;
 }

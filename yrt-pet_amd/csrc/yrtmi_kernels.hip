@@ -616,14 +616,53 @@ hipError_t launchFill(float* x, int64_t n, float v, hipStream_t st)
 	return hipGetLastError();
 }
 
+// Histogram3D::getBinIdFromDetPair (Histogram3D.cpp:416-421, getCoordsFromDetPair
+// 366-405, getBinIdFromCoords 267-271): the pair ordered by id then by
+// position in its ring, (r, phi) from the in-ring table (ringMap[a * dpr + b]
+// = rr | phi << 16, -1 where no bin), the z bin from the two rings.  False
+// for a pair outside the histogram (a detector beyond the scanner, no
+// in-ring bin, a ring difference above the limit), where the reference
+// throws.
+__device__ __forceinline__ bool pairBin(const PairGeom& g, const int32_t* __restrict__ ringMap,
+                                        uint32_t d1, uint32_t d2, int64_t& bin)
+{
+	bool ok = d1 < g.ndet && d2 < g.ndet;
+	if (d1 > d2)
+	{
+		const uint32_t t = d1;
+		d1 = d2;
+		d2 = t;
+	}
+	uint32_t r1 = d1 % g.dpr, r2 = d2 % g.dpr;
+	if (r1 > r2)
+	{
+		uint32_t t = d1;
+		d1 = d2;
+		d2 = t;
+		t = r1;
+		r1 = r2;
+		r2 = t;
+	}
+	const int32_t m = ok ? ringMap[r1 * g.dpr + r2] : -1;
+	const int32_t z1 = (int32_t)((d1 / g.dpr) % g.nrings);
+	const int32_t z2 = (int32_t)((d2 / g.dpr) % g.nrings);
+	const int32_t dz = z2 > z1 ? z2 - z1 : z1 - z2;
+	ok = ok && m >= 0 && dz <= g.maxRingDiff;
+	if (!ok)
+		return false;
+	const uint32_t doi1 = d1 / (g.nrings * g.dpr), doi2 = d2 / (g.nrings * g.dpr);
+	const uint32_t r = (uint32_t)(m & 0xffff) * g.numDOIPoss + doi1 + doi2 * g.numDOI;
+	const uint32_t phi = (uint32_t)m >> 16;
+	int64_t zb = (int64_t)dz * g.nrings + (z1 < z2 ? z1 : z2) - (int64_t)dz * (dz - 1) / 2;
+	if (dz > 0 && z1 > z2)
+		zb += g.numZBinDiff;
+	bin = (zb * g.numPhi + phi) * g.numR + r;
+	return true;
+}
+
 // Histogram3D::getProjectionValueFromHistogramBin with a detector pair
-// (Histogram3D.cpp:588-606, getCoordsFromDetPair 366-405,
-// getBinIdFromCoords 267-271): the pair ordered by id then by position in
-// its ring, (r, phi) from the in-ring table (ringMap[a * dpr + b] =
-// rr | phi << 16, -1 where no bin), the z bin from the two rings.  The pair
-// (0, 0) reads 0; a pair outside the histogram (a detector beyond the
-// scanner, no in-ring bin, a ring difference above the limit) sets *err and
-// reads 0 (the reference throws).
+// (Histogram3D.cpp:588-606): the value of the pair's bin.  The pair (0, 0)
+// reads 0; a pair outside the histogram sets *err and reads 0.
 __global__ void __launch_bounds__(256) kGatherPairs(const float* __restrict__ histo,
                                                     const uint32_t* __restrict__ d1s,
                                                     const uint32_t* __restrict__ d2s, int64_t n,
@@ -634,47 +673,37 @@ __global__ void __launch_bounds__(256) kGatherPairs(const float* __restrict__ hi
 	const int64_t stride = (int64_t)gridDim.x * blockDim.x;
 	for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
 	{
-		uint32_t d1 = d1s[i], d2 = d2s[i];
+		const uint32_t d1 = d1s[i], d2 = d2s[i];
 		float v = 0.f;
 		if (d1 != 0 || d2 != 0)
 		{
-			bool ok = d1 < g.ndet && d2 < g.ndet;
-			if (d1 > d2)
-			{
-				const uint32_t t = d1;
-				d1 = d2;
-				d2 = t;
-			}
-			uint32_t r1 = d1 % g.dpr, r2 = d2 % g.dpr;
-			if (r1 > r2)
-			{
-				uint32_t t = d1;
-				d1 = d2;
-				d2 = t;
-				t = r1;
-				r1 = r2;
-				r2 = t;
-			}
-			const int32_t m = ok ? ringMap[r1 * g.dpr + r2] : -1;
-			const int32_t z1 = (int32_t)((d1 / g.dpr) % g.nrings);
-			const int32_t z2 = (int32_t)((d2 / g.dpr) % g.nrings);
-			const int32_t dz = z2 > z1 ? z2 - z1 : z1 - z2;
-			ok = ok && m >= 0 && dz <= g.maxRingDiff;
-			if (ok)
-			{
-				const uint32_t doi1 = d1 / (g.nrings * g.dpr), doi2 = d2 / (g.nrings * g.dpr);
-				const uint32_t r = (uint32_t)(m & 0xffff) * g.numDOIPoss + doi1 + doi2 * g.numDOI;
-				const uint32_t phi = (uint32_t)m >> 16;
-				int64_t zb = (int64_t)dz * g.nrings + (z1 < z2 ? z1 : z2) -
-				             (int64_t)dz * (dz - 1) / 2;
-				if (dz > 0 && z1 > z2)
-					zb += g.numZBinDiff;
-				v = histo[(zb * g.numPhi + phi) * g.numR + r];
-			}
+			int64_t bin = 0;
+			if (pairBin(g, ringMap, d1, d2, bin))
+				v = histo[bin];
 			else
 				atomicOr(err, 1);
 		}
 		out[i] = v;
+	}
+}
+
+// The bin of each pair; a pair outside the histogram (or the disabled pair
+// (0, 0), ProjectionData's null LOR) gets 0xffffffff and sets *err.
+__global__ void __launch_bounds__(256) kPairBins(const uint32_t* __restrict__ d1s,
+                                                 const uint32_t* __restrict__ d2s, int64_t n,
+                                                 const int32_t* __restrict__ ringMap,
+                                                 PairGeom g, uint32_t* __restrict__ out,
+                                                 int* __restrict__ err)
+{
+	const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+	for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+	{
+		const uint32_t d1 = d1s[i], d2 = d2s[i];
+		int64_t bin = 0;
+		const bool ok = (d1 != 0 || d2 != 0) && pairBin(g, ringMap, d1, d2, bin);
+		if (!ok)
+			atomicOr(err, 1);
+		out[i] = ok ? (uint32_t)bin : 0xffffffffu;
 	}
 }
 
@@ -685,6 +714,31 @@ hipError_t launchGatherPairs(const float* histo, const uint32_t* d1, const uint3
 	if (n <= 0)
 		return hipSuccess;
 	kGatherPairs<<<gridFor(n), 256, 0, st>>>(histo, d1, d2, n, ringMap, g, out, err);
+	return hipGetLastError();
+}
+
+hipError_t launchPairBins(const uint32_t* d1, const uint32_t* d2, int64_t n,
+                          const int32_t* ringMap, const PairGeom& g, uint32_t* out, int* err,
+                          hipStream_t st)
+{
+	if (n <= 0)
+		return hipSuccess;
+	kPairBins<<<gridFor(n), 256, 0, st>>>(d1, d2, n, ringMap, g, out, err);
+	return hipGetLastError();
+}
+
+__global__ void __launch_bounds__(256) kFillI32(int32_t* __restrict__ x, int64_t n, int32_t v)
+{
+	const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+	for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+		x[i] = v;
+}
+
+hipError_t launchFillI32(int32_t* x, int64_t n, int32_t v, hipStream_t st)
+{
+	if (n <= 0)
+		return hipSuccess;
+	kFillI32<<<gridFor(n), 256, 0, st>>>(x, n, v);
 	return hipGetLastError();
 }
 

@@ -30,6 +30,10 @@ struct PairGeom
 hipError_t launchGatherPairs(const float* histo, const uint32_t* d1, const uint32_t* d2,
                              int64_t n, const int32_t* ringMap, const PairGeom& g, float* out,
                              int* err, hipStream_t st);
+hipError_t launchPairBins(const uint32_t* d1, const uint32_t* d2, int64_t n,
+                          const int32_t* ringMap, const PairGeom& g, uint32_t* out, int* err,
+                          hipStream_t st);
+hipError_t launchFillI32(int32_t* x, int64_t n, int32_t v, hipStream_t st);
 
 // Image-space PSF: one axis (0 x, 1 y, 2 z) of the separable circular
 // convolution of every frame (nx * ny * nz * frames = total < 2^31 voxels),

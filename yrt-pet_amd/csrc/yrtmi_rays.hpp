@@ -71,11 +71,11 @@ struct SidRay
 	float axn, ayn, azn;  // alpha of the next plane per axis (FMAX: flat)
 	float xc, yc, zc;     // coordinate of that plane
 	float dnorm;
-	// TOF.  FP32 setup: tB = pc = d/2 + tof_mm, and the weight of a segment
-	// [a0, a1] follows the reference's operation order.  FP64 setup: the
-	// Gaussian's argument is x_c = (a0 + a1) * tA - tB (tA = d / (2 sigma),
-	// tB = pc / sigma, both from FP64) and the weight (a1 - a0) * wn *
-	// exp(-x_c^2 / 2), wn = d * norm.
+	// TOF: the Gaussian's argument at a segment [a0, a1] is x_c = (a0 + a1)
+	// * tA - tB (tA = d / (2 sigma), tB = pc / sigma, pc = d/2 + tof_mm, both
+	// rounded once from FP64) and its weight (a1 - a0) * wn * exp(-x_c^2 /
+	// 2), wn = d * norm: the reference's value to a few FP32 ulps (it forms
+	// (a_mid d - pc) / sigma in FP32).
 	float tA, tB, wn;
 	int dirx, diry, dirz;
 	// index of the first plane per axis, counted from the image face the
@@ -280,7 +280,11 @@ __device__ __forceinline__ bool sidSetup(const DImage& P, float3 p1, float3 p2,
 				amin = fmaxf(amin, lo);
 				amax = fminf(amax, hi);
 			}
-			r.tB = (float)(0.5 * (double)r.dnorm + (double)tmm);  // pc
+			// x_c = (a0 + a1) tA - tB, weight wn * exp(-x_c^2 / 2) per unit
+			// alpha, from FP64 (the S64 form)
+			r.tB = (float)((0.5 * (double)r.dnorm + (double)tmm) / (double)tof.sigma);
+			r.tA = (float)(0.5 * (double)r.dnorm / (double)tof.sigma);
+			r.wn = (float)((double)r.dnorm * (double)tof.norm);
 		}
 		r.xc = (r.invx > 0.0f) ? x0 : x1;
 		r.yc = (r.invy > 0.0f) ? y0 : y1;
@@ -316,6 +320,46 @@ __device__ __forceinline__ bool sidSetup(const DImage& P, float3 p1, float3 p2,
 		}
 	}
 	return true;
+}
+
+// The part of an FP32 set-up (sidSetup<TOF, false>) that a walk re-entered
+// at a tile record needs beyond the ray's end points, in two float4 (a
+// plan's ray table): the per-axis reciprocals, the exit alpha, the length
+// and the TOF constants.  sidLoad rebuilds the SidRay from the end points
+// and them: every field sidRestart and sidWalk read equals the set-up's
+// (the planes are recomputed by sidRestart, with the set-up's own
+// expression in the FP32 set-up, so kx/ky/kz are set to never match and
+// every non-flat axis is recomputed).
+__device__ __forceinline__ void sidStore(float4* e, const SidRay& r)
+{
+	e[0] = make_float4(r.invx, r.invy, r.invz, r.amax);
+	e[1] = make_float4(r.dnorm, r.tA, r.tB, r.wn);
+}
+
+__device__ __forceinline__ void sidLoad(float3 p1, float3 p2, const float4& s0,
+                                        const float4& s1, SidRay& r)
+{
+	const float FMAX = 3.402823466e+38f;
+	r.p1 = p1;
+	r.p2 = p2;
+	r.invx = s0.x;
+	r.invy = s0.y;
+	r.invz = s0.z;
+	r.amax = s0.w;
+	r.dnorm = s1.x;
+	r.tA = s1.y;
+	r.tB = s1.z;
+	r.wn = s1.w;
+	r.dirx = (r.invx >= 0.0f) ? 1 : -1;
+	r.diry = (r.invy >= 0.0f) ? 1 : -1;
+	r.dirz = (r.invz >= 0.0f) ? 1 : -1;
+	r.axn = FMAX;
+	r.ayn = FMAX;
+	r.azn = FMAX;
+	r.xc = r.yc = r.zc = 0.f;
+	r.kx = r.ky = r.kz = -0x40000000;
+	r.a_cur = 0.f;
+	r.x1d = r.y1d = r.z1d = r.ddx = r.ddy = r.ddz = 0.0;
 }
 
 // The voxel holding the midpoint of segment [a0, a1] (the walk's rule for
@@ -536,25 +580,16 @@ __device__ __forceinline__ void sidWalk(const DImage& P, const SidRay& r,
 	// its index base + vz * sz + vy * sy + vx is off + lx, off = base +
 	// vz * sz + vy * sy + x0
 	int lx = 0, ly = 0, lz = 0, off = 0;
-	float acd = 0.f;  // a_cur * dnorm, carried from the previous segment's end
 	auto emit = [&](int off) {
 		float w;
 		if (TOF)
 		{
-			if constexpr (S64)
-			{
-				// exp(-x^2/2) = 2^(-x^2 log2(e) / 2)
-				const float xc = fmaf(a_cur + a_next, r.tA, -r.tB);
-				w = (a_next - a_cur) * r.wn * __builtin_amdgcn_exp2f(-0.72134752f * xc * xc);
-			}
-			else
-			{
-				// TimeOfFlightHelper::getWeight (TimeOfFlight.hpp:175-185)
-				const float and_ = a_next * dnorm;
-				const float xc = divBySigma(0.5f * (acd + and_) - r.tB, tof.sigma, tof.rsigma);
-				acd = and_;
-				w = (a_next - a_cur) * dnorm * gaussW(xc, tof.norm);
-			}
+			// TimeOfFlightHelper::getWeight (TimeOfFlight.hpp:175-185): the
+			// Gaussian at the segment's midpoint, its argument
+			// (a_mid d - pc) / sigma formed as (a0 + a1) tA - tB with one FMA
+			// and exp(-x^2/2) = 2^(-x^2 log2(e) / 2)
+			const float xc = fmaf(a_cur + a_next, r.tA, -r.tB);
+			w = (a_next - a_cur) * r.wn * __builtin_amdgcn_exp2f(-0.72134752f * xc * xc);
 		}
 		else
 			w = (a_next - a_cur) * dnorm;
@@ -579,8 +614,6 @@ __device__ __forceinline__ void sidWalk(const DImage& P, const SidRay& r,
 		ly = vy - B.y0;
 		lz = vz - B.z0;
 		off = B.base + vz * B.sz + vy * B.sy + B.x0;
-		if (TOF && !S64)
-			acd = a_cur * dnorm;
 		emit(off);
 	}
 	else
@@ -589,8 +622,6 @@ __device__ __forceinline__ void sidWalk(const DImage& P, const SidRay& r,
 		ly = vy - B.y0;
 		lz = vz - B.z0;
 		off = B.base + vz * B.sz + vy * B.sy + B.x0;
-		if (TOF && !S64)
-			acd = a_cur * dnorm;
 	}
 	const int ys = r.diry * B.sy, zs = r.dirz * B.sz;
 	const unsigned bx = (unsigned)(B.x1 - B.x0), by = (unsigned)(B.y1 - B.y0),
@@ -658,9 +689,16 @@ __device__ __forceinline__ float siddonRay(const DImage& P,
 struct DDRay
 {
 	bool usePsf;
-	int psfRow;       // row of the PSF table for this LOR
-	const float* pk;  // PSF taps (flipped for backprojection)
+	int psfRow;         // row of the PSF table for this LOR
+	const float* pk;    // PSF taps (flipped for backprojection)
+	const double* pcum; // the row's trapezoid prefix sums (flipped likewise)
 	float ext, d_norm, d1_i, d2_i;
+	// reciprocals of the slice-invariant denominators of ddSlice: the
+	// footprint edges' (a_xy_lo, a_xy_hi, a_z_lo, a_z_hi) and the TOF slab's
+	// (d2_i - d1_i); rOk: all five usable (finite, normal), else ddSlice
+	// divides
+	float r_xy_lo, r_xy_hi, r_z_lo, r_z_hi, r_i;
+	bool rOk;
 	int xy_i_0, xy_i_1, nyx, strideXY, strideYX;
 	bool flag_y;  // major axis y (else x)
 	float lxy, lyx, dxy, dyx;
@@ -669,6 +707,28 @@ struct DDRay
 	float d1zloI, d1zloZ, d1zhiI, d1zhiZ, d2zloI, d2zloZ, d2zhiI, d2zhiZ;
 	float dxy_cos_theta, yx_off, inv_dyx, z_off, inv_dz, half_dyx, hvz, pc;
 };
+
+// The reciprocals of ddSlice's slice-invariant denominators.
+__device__ __forceinline__ bool recipOk(float d, float r)
+{
+	return fabsf(d) >= 1.17549435e-38f && fabsf(d) < 3.0e38f && fabsf(r) >= 1.17549435e-38f &&
+	       fabsf(r) < 3.0e38f;
+}
+
+__device__ __forceinline__ void ddRecips(DDRay& R)
+{
+#pragma clang fp contract(off)
+	const float dxl = R.d2_xy_hi - R.d1_xy_lo, dxh = R.d2_xy_lo - R.d1_xy_hi;
+	const float dzl = R.d2zloI - R.d1zloI, dzh = R.d2zhiI - R.d1zhiI;
+	const float di = R.d2_i - R.d1_i;
+	R.r_xy_lo = 1.0f / dxl;
+	R.r_xy_hi = 1.0f / dxh;
+	R.r_z_lo = 1.0f / dzl;
+	R.r_z_hi = 1.0f / dzh;
+	R.r_i = 1.0f / di;
+	R.rOk = recipOk(dxl, R.r_xy_lo) && recipOk(dxh, R.r_xy_hi) && recipOk(dzl, R.r_z_lo) &&
+	        recipOk(dzh, R.r_z_hi) && recipOk(di, R.r_i);
+}
 
 template <bool TOF, bool S64>
 __device__ __forceinline__ void ddSetup(const DImage& P, float3 d1, float3 d2, float3 n1,
@@ -681,6 +741,7 @@ __device__ __forceinline__ void ddSetup(const DImage& P, float3 d1, float3 d2, f
 	const float d_norm = sqrtf(d12x * d12x + d12y * d12y + d12z * d12z);
 	const bool usePsf = pr.psf.k != nullptr;
 	const float* pk = nullptr;
+	const double* pcum = nullptr;
 	float ext = 0.f;
 	int psfRow = 0;
 	if (usePsf)
@@ -701,6 +762,7 @@ __device__ __forceinline__ void ddSetup(const DImage& P, float3 d1, float3 d2, f
 		}
 		psfRow = s;
 		pk = (flipPsf ? pr.psf.kflip : pr.psf.k) + s * pr.psf.K;
+		pcum = (flipPsf ? pr.psf.cumFlip : pr.psf.cum) + s * (pr.psf.K + 2);
 		ext = pr.psf.halfWidth;
 	}
 	const float dx = P.vx, dy = P.vy, dz = P.vz;
@@ -859,6 +921,7 @@ __device__ __forceinline__ void ddSetup(const DImage& P, float3 d1, float3 d2, f
 	R.usePsf = usePsf;
 	R.psfRow = psfRow;
 	R.pk = pk;
+	R.pcum = pcum;
 	R.ext = ext;
 	R.d_norm = d_norm;
 	R.d1_i = d1_i;
@@ -897,6 +960,7 @@ __device__ __forceinline__ void ddSetup(const DImage& P, float3 d1, float3 d2, f
 	R.half_dyx = half_dyx;
 	R.hvz = hvz;
 	R.pc = pc;
+	ddRecips(R);
 }
 
 // The per-LOR part of a DDRay in six float4 (a plan's DD ray table), and
@@ -968,6 +1032,8 @@ __device__ __forceinline__ void ddLoad(const DImage& P, const DProj& pr, const f
 	R.psfRow = __float_as_int(e[5].z);
 	R.usePsf = pr.psf.k != nullptr;
 	R.pk = R.usePsf ? (flipPsf ? pr.psf.kflip : pr.psf.k) + R.psfRow * pr.psf.K : nullptr;
+	R.pcum = R.usePsf ? (flipPsf ? pr.psf.cumFlip : pr.psf.cum) + R.psfRow * (pr.psf.K + 2)
+	                  : nullptr;
 	R.ext = R.usePsf ? pr.psf.halfWidth : 0.f;
 	const bool fy = R.flag_y;
 	R.lxy = fy ? P.ly : P.lx;
@@ -983,11 +1049,13 @@ __device__ __forceinline__ void ddLoad(const DImage& P, const DProj& pr, const f
 	R.inv_dz = 1.0f / P.vz;
 	R.half_dyx = R.dyx * 0.5f;
 	R.hvz = P.vz * 0.5f;
+	ddRecips(R);
 }
 
 // Footprint range of slice xyi: minor-axis voxels [yx0, yx1], z voxels
 // [zi0, zi1] (clamped to the image), and the quantities the voxel weights
-// need.
+// need.  The quotients by slice-invariant denominators use the LOR's
+// reciprocals (divByRecip: the same correctly rounded quotients).
 struct DDSlice
 {
 	float r0, r1, r0ov, r1ov, widthFrac_yx, z0r, z1r, widthFrac_z, tw;
@@ -1000,10 +1068,21 @@ __device__ __forceinline__ void ddSlice(const DImage& P, const DDRay& R, const D
 {
 #pragma clang fp contract(off)
 	const float pix_xy = -0.5f * R.lxy + ((float)xyi + 0.5f) * R.dxy;
-	const float a_xy_lo = (pix_xy - R.d1_xy_lo) / (R.d2_xy_hi - R.d1_xy_lo);
-	const float a_xy_hi = (pix_xy - R.d1_xy_hi) / (R.d2_xy_lo - R.d1_xy_hi);
-	const float a_z_lo = (pix_xy - R.d1zloI) / (R.d2zloI - R.d1zloI);
-	const float a_z_hi = (pix_xy - R.d1zhiI) / (R.d2zhiI - R.d1zhiI);
+	float a_xy_lo, a_xy_hi, a_z_lo, a_z_hi;
+	if (R.rOk)
+	{
+		a_xy_lo = divByRecip(pix_xy - R.d1_xy_lo, R.d2_xy_hi - R.d1_xy_lo, R.r_xy_lo);
+		a_xy_hi = divByRecip(pix_xy - R.d1_xy_hi, R.d2_xy_lo - R.d1_xy_hi, R.r_xy_hi);
+		a_z_lo = divByRecip(pix_xy - R.d1zloI, R.d2zloI - R.d1zloI, R.r_z_lo);
+		a_z_hi = divByRecip(pix_xy - R.d1zhiI, R.d2zhiI - R.d1zhiI, R.r_z_hi);
+	}
+	else
+	{
+		a_xy_lo = (pix_xy - R.d1_xy_lo) / (R.d2_xy_hi - R.d1_xy_lo);
+		a_xy_hi = (pix_xy - R.d1_xy_hi) / (R.d2_xy_lo - R.d1_xy_hi);
+		a_z_lo = (pix_xy - R.d1zloI) / (R.d2zloI - R.d1zloI);
+		a_z_hi = (pix_xy - R.d1zhiI) / (R.d2zhiI - R.d1zhiI);
+	}
 	float r0 = R.d1_yx_lo + a_xy_lo * (R.d2_yx_hi - R.d1_yx_lo);
 	float r1 = R.d1_yx_hi + a_xy_hi * (R.d2_yx_lo - R.d1_yx_hi);
 	if (r0 > r1)
@@ -1038,8 +1117,11 @@ __device__ __forceinline__ void ddSlice(const DImage& P, const DDRay& R, const D
 	if (TOF)
 	{
 		// getWeight at the slab's [a_lo, a_hi] (ProjectorDD.cpp:507-515)
-		const float alo = (pix_xy - R.d1_i - 0.5f * R.dxy) / (R.d2_i - R.d1_i);
-		const float ahi = (pix_xy - R.d1_i + 0.5f * R.dxy) / (R.d2_i - R.d1_i);
+		const float di = R.d2_i - R.d1_i;
+		const float nlo = pix_xy - R.d1_i - 0.5f * R.dxy;
+		const float nhi = pix_xy - R.d1_i + 0.5f * R.dxy;
+		const float alo = R.rOk ? divByRecip(nlo, di, R.r_i) : nlo / di;
+		const float ahi = R.rOk ? divByRecip(nhi, di, R.r_i) : nhi / di;
 		const float xcn =
 		    divBySigma(0.5f * (alo * R.d_norm + ahi * R.d_norm) - R.pc, pr.tof.sigma,
 		               pr.tof.rsigma);
@@ -1047,10 +1129,24 @@ __device__ __forceinline__ void ddSlice(const DImage& P, const DDRay& R, const D
 	}
 }
 
+// A slice's footprint weight x / width from the slice's reciprocal of the
+// width (ok: usable), else by division.
+__device__ __forceinline__ float ddDivW(float x, float width, float rw, bool ok)
+{
+	return ok ? divByRecip(x, width, rw) : x / width;
+}
+
+// z weights of a slice computed once for all its minor-axis voxels when the
+// slice spans at most this many z planes (else per voxel)
+constexpr int DD_ZW = 4;
+
 // The footprint voxels of slices [xlo, xhi] of the major axis, minor-axis
 // index in [ylo, yhi] and z in [zlo, zhi] (each also clamped to the ray's own
 // range): fn(io, xyi, yxi, zi, w), io the voxel's index in the frame.  The
-// whole image (ranges covering it) is ProjectorDD's sweep.
+// whole image (ranges covering it) is ProjectorDD's sweep.  The weight of
+// voxel (xyi, yxi, zi) is ProjectorDD's: (w_yx / width_yx) * (w_z / width_z)
+// * dxy / |cos|, times the TOF weight of the slice; the z factor depends on
+// the slice and zi only, so it is computed once per slice and z plane.
 template <bool TOF, class Fn>
 __device__ __forceinline__ void ddVisit(const DImage& P, const DDRay& R, const DProj& pr,
                                         int xlo, int xhi, int ylo, int yhi, int zlo, int zhi,
@@ -1064,6 +1160,29 @@ __device__ __forceinline__ void ddVisit(const DImage& P, const DDRay& R, const D
 		ddSlice<TOF>(P, R, pr, xyi, q);
 		const int yx0 = max(q.yx0, ylo), yx1 = min(q.yx1, yhi);
 		const int zi0 = max(q.zi0, zlo), zi1 = min(q.zi1, zhi);
+		if (yx0 > yx1 || zi0 > zi1)
+			continue;
+		const float ry = 1.0f / q.widthFrac_yx, rz = 1.0f / q.widthFrac_z;
+		const bool oky = recipOk(q.widthFrac_yx, ry), okz = recipOk(q.widthFrac_z, rz);
+		const int nzv = zi1 - zi0 + 1;
+		// z weights of the slice (at most DD_ZW planes) and which planes the
+		// footprint overlaps
+		float wz[DD_ZW];
+		uint32_t zmask = 0u;
+		if (nzv <= DD_ZW)
+		{
+#pragma unroll
+			for (int j = 0; j < DD_ZW; j++)
+			{
+				const int zi = zi0 + j;
+				const float pix_z = -0.5f * P.lz + ((float)zi + 0.5f) * P.vz;
+				const float zp0 = pix_z - R.hvz;
+				const float zp1 = pix_z + R.hvz;
+				wz[j] = ddDivW(fminf(zp1, q.z1r) - fmaxf(zp0, q.z0r), q.widthFrac_z, rz, okz);
+				if (j < nzv && q.z1r >= zp0 && q.z0r < zp1)
+					zmask |= 1u << j;
+			}
+		}
 		for (int yxi = yx0; yxi <= yx1; yxi++)
 		{
 			const float pix_yx = -0.5f * R.lyx + ((float)yxi + 0.5f) * R.dyx;
@@ -1072,23 +1191,40 @@ __device__ __forceinline__ void ddVisit(const DImage& P, const DDRay& R, const D
 			if (!(q.r1 >= p0 && q.r0 < p1))
 				continue;
 			const float wxy =
-			    R.usePsf ? psfWeight(pr.psf, R.pk, p0 - q.r1ov, p1 - q.r0ov)
+			    R.usePsf ? psfWeight(pr.psf, R.pk, R.pcum, p0 - q.r1ov, p1 - q.r0ov)
 			             : (fminf(p1, q.r1ov) - fmaxf(p0, q.r0ov));
-			const float wxy_s = wxy / q.widthFrac_yx;
-			for (int zi = zi0; zi <= zi1; zi++)
+			const float wxy_s = ddDivW(wxy, q.widthFrac_yx, ry, oky);
+			const int ib = xyi * R.strideXY + yxi * R.strideYX;
+			if (nzv <= DD_ZW)
 			{
-				const float pix_z = -0.5f * P.lz + ((float)zi + 0.5f) * P.vz;
-				const float zp0 = pix_z - R.hvz;
-				const float zp1 = pix_z + R.hvz;
-				if (!(q.z1r >= zp0 && q.z0r < zp1))
-					continue;
-				const float wz = fminf(zp1, q.z1r) - fmaxf(zp0, q.z0r);
-				const float wz_s = wz / q.widthFrac_z;
-				const int io = zi * P.nxy + xyi * R.strideXY + yxi * R.strideYX;
-				float w = wxy_s * wz_s * R.dxy_cos_theta;
-				if (TOF)
-					w *= q.tw;
-				fn(io, xyi, yxi, zi, w);
+#pragma unroll
+				for (int j = 0; j < DD_ZW; j++)
+				{
+					if (!((zmask >> j) & 1u))
+						continue;
+					const int zi = zi0 + j;
+					float w = wxy_s * wz[j] * R.dxy_cos_theta;
+					if (TOF)
+						w *= q.tw;
+					fn(zi * P.nxy + ib, xyi, yxi, zi, w);
+				}
+			}
+			else
+			{
+				for (int zi = zi0; zi <= zi1; zi++)
+				{
+					const float pix_z = -0.5f * P.lz + ((float)zi + 0.5f) * P.vz;
+					const float zp0 = pix_z - R.hvz;
+					const float zp1 = pix_z + R.hvz;
+					if (!(q.z1r >= zp0 && q.z0r < zp1))
+						continue;
+					const float wz_s = ddDivW(fminf(zp1, q.z1r) - fmaxf(zp0, q.z0r),
+					                          q.widthFrac_z, rz, okz);
+					float w = wxy_s * wz_s * R.dxy_cos_theta;
+					if (TOF)
+						w *= q.tw;
+					fn(zi * P.nxy + ib, xyi, yxi, zi, w);
+				}
 			}
 		}
 	}

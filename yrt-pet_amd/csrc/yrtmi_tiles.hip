@@ -73,17 +73,29 @@ __global__ void __launch_bounds__(256) kRayPrep(KArgs a, float4* __restrict__ ta
 			ddStore(tab, (uint64_t)t, D, l.frame);
 			continue;
 		}
+		constexpr int V = rayTabVec(S64);
+		const float3 z = make_float3(0.f, 0.f, 0.f);
 		if (l.frame < 0)
 		{
-			const float3 z = make_float3(0.f, 0.f, 0.f);
-			rayEntry(tab, (uint32_t)t, z, z, 0.f, -1);
+			rayEntry(tab, V, (uint32_t)t, z, z, 0.f, -1);
+			if (!S64)
+				tab[V * (uint64_t)t + 2] = tab[V * (uint64_t)t + 3] = make_float4(0.f, 0.f, 0.f, 0.f);
 			continue;
 		}
 		float3 q1, q2;
 		rayLine(a.pr, l, r, q1, q2);
 		q1 = make_float3(q1.x - a.im.ox, q1.y - a.im.oy, q1.z - a.im.oz);
 		q2 = make_float3(q2.x - a.im.ox, q2.y - a.im.oy, q2.z - a.im.oz);
-		rayEntry(tab, (uint32_t)t, q1, q2, l.tof, l.frame);
+		rayEntry(tab, V, (uint32_t)t, q1, q2, l.tof, l.frame);
+		if (!S64)
+		{
+			// the ray's FP32 set-up, for walks re-entered at a tile record
+			SidRay sr;
+			if (sidSetup<TOF, false>(a.im, q1, q2, a.pr.tof, l.tof, sr))
+				sidStore(tab + V * (uint64_t)t + 2, sr);
+			else
+				tab[V * (uint64_t)t + 2] = tab[V * (uint64_t)t + 3] = make_float4(0.f, 0.f, 0.f, 0.f);
+		}
 	}
 }
 
@@ -96,7 +108,8 @@ __device__ __forceinline__ void binRay(const DImage& im, const DTof& tof, const 
                                        const float4* __restrict__ tab, uint32_t t, int* err,
                                        Fn&& onRecord)
 {
-	const float4 A = tab[2 * (uint64_t)t], Bq = tab[2 * (uint64_t)t + 1];
+	constexpr int V = rayTabVec(S64);
+	const float4 A = tab[V * (uint64_t)t], Bq = tab[V * (uint64_t)t + 1];
 	const int frame = __float_as_int(Bq.w);
 	if (frame < 0)
 		return;
@@ -393,8 +406,9 @@ __device__ __forceinline__ void tileChunk(const KArgs& a, const DTiles& T,
 		uint32_t key, pv;
 		float al;
 	};
-	// Siddon: two float4 of the ray table; DD: six (ddStore)
-	constexpr int NE = DD ? 6 : 2;
+	// Siddon: the ray's end points, TOF value and frame (two float4), with
+	// the FP32 set-up two more (sidStore); DD: six (ddStore)
+	constexpr int NE = DD ? 6 : rayTabVec(S64);
 	struct Ray
 	{
 		float4 e[NE];
@@ -448,7 +462,7 @@ __device__ __forceinline__ void tileChunk(const KArgs& a, const DTiles& T,
 		}
 		else
 		{
-			const float4 qa = q.e[0], qb = q.e[NE - 1];
+			const float4 qa = q.e[0], qb = q.e[1];
 			if (__float_as_int(qb.w) != f)
 				return;
 			const float3 q1 = make_float3(qa.x, qa.y, qa.z), q2 = make_float3(qa.w, qb.x, qb.y);
@@ -463,7 +477,12 @@ __device__ __forceinline__ void tileChunk(const KArgs& a, const DTiles& T,
 			if (!first && (vx >= B.x1 || vy >= B.y1 || vz >= B.z1))
 				return;
 			SidRay s;
-			if (!sidSetup<TOF, S64>(a.im, q1, q2, a.pr.tof, qb.z, s))
+			if (!S64 && !first)
+			{
+				// re-entered at the record: the ray's stored set-up
+				sidLoad(q1, q2, q.e[2 % NE], q.e[3 % NE], s);
+			}
+			else if (!sidSetup<TOF, S64>(a.im, q1, q2, a.pr.tof, qb.z, s))
 				return;
 			if (!first)
 				sidRestart<S64>(a.im, s, rc.al, vx, vy, vz, dx, dy, dz);
@@ -631,8 +650,8 @@ __global__ void __launch_bounds__(256) kEmRatio(KArgs a, float* __restrict__ v)
 		if (TAB)
 		{
 			const int64_t t0 = e * R + min(lane, R - 1);
-			A = a.rays[2 * t0];
-			B = a.rays[2 * t0 + 1];
+			A = a.rays[rayTabVec(S64) * t0];
+			B = a.rays[rayTabVec(S64) * t0 + 1];
 			l.frame = __float_as_int(B.w);
 			l.tof = B.z;
 		}
@@ -653,8 +672,8 @@ __global__ void __launch_bounds__(256) kEmRatio(KArgs a, float* __restrict__ v)
 				{
 					if (r != lane)
 					{
-						A = a.rays[2 * (e * R + r)];
-						B = a.rays[2 * (e * R + r) + 1];
+						A = a.rays[rayTabVec(S64) * (e * R + r)];
+						B = a.rays[rayTabVec(S64) * (e * R + r) + 1];
 					}
 					q1 = make_float3(A.x, A.y, A.z);
 					q2 = make_float3(A.w, B.x, B.y);

@@ -73,19 +73,27 @@ struct DPlan
 	float* alpha;               // alpha where the walk enters the tile (Siddon)
 	uint32_t* vox;              // packed entry voxel (packEntry); DD: the
 	                            // record's slices in the slab (binRayDD)
-	const float4* ray;          // ray table (rayEntry / ddStore, kRayPrep)
+	const float4* ray;          // ray table (rayEntry + sidStore / ddStore, kRayPrep)
 	int* err;                   // set when a tile range would overflow
 };
 
-// Ray table of a plan, two float4 per (LOR, ray) t = lor * R + ray: the
-// ray's end points after multi-ray offsets, motion and the image offset
-// (q1, q2, exactly as the walk sets it up), its TOF value and its frame
-// (-1: skipped).  The backprojection reads it instead of resolving the LOR.
-__device__ __forceinline__ void rayEntry(float4* tab, uint32_t t, float3 q1, float3 q2,
+// Ray table of a Siddon plan, rayTabVec(S64) float4 per (LOR, ray) t = lor *
+// R + ray.  The first two: the ray's end points after multi-ray offsets,
+// motion and the image offset (q1, q2, exactly as the walk sets it up), its
+// TOF value and its frame (-1: skipped); the backprojection reads them
+// instead of resolving the LOR.  With the FP32 setup two more hold the
+// ray's set-up (sidStore), from which a walk re-entered at a tile record
+// starts without redoing it.
+constexpr int rayTabVec(bool S64)
+{
+	return S64 ? 2 : 4;
+}
+
+__device__ __forceinline__ void rayEntry(float4* tab, int vec, uint32_t t, float3 q1, float3 q2,
                                          float tof, int32_t frame)
 {
-	tab[2 * (uint64_t)t] = make_float4(q1.x, q1.y, q1.z, q2.x);
-	tab[2 * (uint64_t)t + 1] = make_float4(q2.y, q2.z, tof, __int_as_float(frame));
+	tab[vec * (uint64_t)t] = make_float4(q1.x, q1.y, q1.z, q2.x);
+	tab[vec * (uint64_t)t + 1] = make_float4(q2.y, q2.z, tof, __int_as_float(frame));
 }
 
 // Plan in use.  queue != null: persistent launch, workgroups take chunks
@@ -99,7 +107,7 @@ struct DPlanRead
 	const uint32_t* key;
 	const float* alpha;
 	const uint32_t* vox;
-	const float4* ray;  // rayEntry (Siddon) or ddStore (DD) table
+	const float4* ray;  // rayEntry + sidStore (Siddon) or ddStore (DD) table
 	unsigned int* queue;
 	// binade E of the median nonzero |v| of the launch (kVExpHist + kPickExp
 	// before each launch): sets the fixed-point scale of the tile

@@ -75,7 +75,8 @@ _lib = None
 
 _P = C.c_void_p
 class PlanOptions(C.Structure):
-    _fields_ = [("chunk_records", C.c_int), ("tile_shape", C.c_int), ("persistent", C.c_int)]
+    _fields_ = [("chunk_records", C.c_int), ("tile_shape", C.c_int), ("persistent", C.c_int),
+                ("max_bytes", C.c_int64)]
 
 
 TILE_16x16x16, TILE_32x32x16, TILE_32x32x32, TILE_32x32x8 = 0, 1, 2, 3
@@ -119,6 +120,9 @@ _SIGS = {
     "yrtmi_psf_apply": (C.c_int, [_P, C.POINTER(Image), _P, _P, _P, _P, C.c_int, _P, C.c_int,
                                   _P, C.c_int, C.c_int]),
     "yrtmi_histo_gather_pairs": (C.c_int, [_P, _P, _P, _P, C.c_int64, _P]),
+    "yrtmi_histo_pair_bins": (C.c_int, [_P, _P, _P, C.c_int64, _P]),
+    "yrtmi_fill_i32": (C.c_int, [_P, _P, C.c_int64, C.c_int32]),
+    "yrtmi_mem_info": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "yrtmi_time_average_move": (C.c_int, [_P, C.POINTER(Image), _P, _P, _P, _P, C.c_int]),
     "yrtmi_count_segments": (C.c_int, [_P, _P, C.POINTER(Image), C.POINTER(Lors),
                                        C.POINTER(C.c_int64)]),

@@ -300,7 +300,7 @@ class Plan:
         self.ctx, self.proj, self.im, self.lors = ctx, proj, im, lors
         h = _P()
         ci, cl = im.c(), lors.c()
-        opt = capi.PlanOptions(int(chunk_records), int(tile_shape), int(persistent))
+        opt = capi.PlanOptions(int(chunk_records), int(tile_shape), int(persistent), 0)
         capi.check(capi.lib().yrtmi_plan_create(ctx.h, proj.h, C.byref(ci), C.byref(cl),
                                                 C.byref(opt), C.byref(h)))
         self.h = h
@@ -510,6 +510,36 @@ class Context:
                 self.h, proj.h, C.byref(ci), _ptr(img), C.byref(cl),
                 C.byref(ca) if ca is not None else None, _ptr(att_img), _ptr(sens)))
         return img
+
+    def histo_subset_nonzero(self, values, num_subsets, subset):
+        """The nonzero bins of histogram subset `subset` of `num_subsets`
+        (BinIteratorRangeHistogram3D order) and their values: (int32 bin ids,
+        float32 values), each of the nonzero count (yrtmi_histo_subset_nonzero;
+        the reference GPU EM skips zero-measurement bins)."""
+        _check_dtype(values, torch.float32, "values")
+        cap = capi.lib().yrtmi_histo_subset_size(self.h, num_subsets, subset)
+        if cap < 0:
+            raise capi.YrtmiError("histogram not set up or bad subset")
+        dev = values.device
+        bins = torch.empty(max(cap, 1), dtype=torch.int32, device=dev)
+        meas = torch.empty(max(cap, 1), dtype=torch.float32, device=dev)
+        n = C.c_int64()
+        capi.check(capi.lib().yrtmi_histo_subset_nonzero(self.h, _ptr(values), num_subsets,
+                                                         subset, _ptr(bins), _ptr(meas),
+                                                         C.byref(n)))
+        return bins[:n.value].clone(), meas[:n.value].clone()
+
+    def histo_pair_bins(self, d1, d2):
+        """The Histogram3D bin of each detector pair (int32 tensor of bin
+        ids; yrtmi_histo_pair_bins, Histogram3D::getBinIdFromDetPair)."""
+        _check_dtype(d1, torch.int32, "d1")
+        _check_dtype(d2, torch.int32, "d2")
+        if d1.numel() != d2.numel():
+            raise ValueError("histo_pair_bins: d1 and d2 differ in length")
+        out = torch.empty(d1.numel(), dtype=torch.int32, device=d1.device)
+        capi.check(capi.lib().yrtmi_histo_pair_bins(self.h, _ptr(d1), _ptr(d2), d1.numel(),
+                                                    _ptr(out)))
+        return out
 
     def attenuation_factors(self, att_im, att_img, lors, out):
         if att_img.numel() != att_im.nvox or out.numel() < lors.n:

@@ -125,11 +125,17 @@ def _hit_crystals(xp, p, u, sc):
 
 
 def sample_events(sc, im, activity, n, seed, tof_fwhm_ps=None, device=None,
-                  batch=1 << 24):
+                  batch=1 << 24, motion_xf=None):
     """Draw n detected coincidences from `activity` (nz, ny, nx).  Runs with
     torch on `device` (GPU) when given, else with numpy.  Events outside the
     axial FOV or the ring-difference limit are redrawn.  Returns a dict of
-    d1, d2 (uint32 / int32 tensors), tof (float32) and the emission points."""
+    d1, d2 (uint32 / int32 tensors), tof (float32) and the emission points.
+
+    motion_xf: a rigid transform (12 floats, transform_t rows) that maps the
+    scanner coordinates of this stretch of the acquisition to the reference
+    position (p_ref = R p + t, as ProjectionData::getLOR applies it): the
+    emission points are drawn in the reference position and moved to where
+    the object was, p = R^T (p_ref - t), before the detectors are hit."""
     if device is not None:
         import torch
         xp = torch
@@ -173,6 +179,15 @@ def sample_events(sc, im, activity, n, seed, tof_fwhm_ps=None, device=None,
         sz = xp.sqrt(1 - cz * cz)
         u = xp.stack([sz * xp.cos(ph), sz * xp.sin(ph), cz], 1)
         p = xp.stack([px, py, pz], 1)
+        if motion_xf is not None:
+            M = np.asarray(motion_xf, np.float64).reshape(3, 4)
+            Rt = M[:, :3].T
+            if device is not None:
+                Rt = torch.as_tensor(Rt, device=device)
+                t = torch.as_tensor(M[:, 3], device=device)
+            else:
+                t = M[:, 3]
+            p = (p - t) @ Rt.T
         da, va = _hit_crystals(xp, p, u, sc)
         db, vb = _hit_crystals(xp, p, -u, sc)
         ra, rb = da // sc.detsPerRing, db // sc.detsPerRing
