@@ -149,8 +149,18 @@ def parse():
                     help="skip the EM-update parity sample against the oracle")
     ap.add_argument("--oracle-events", type=int, default=1_000_000,
                     help="events of the oracle parity sample")
+    ap.add_argument("--no-dropin", action="store_true",
+                    help="motion workload: skip the OSEM_MI (reconstruct --gpu) leg")
     ap.add_argument("--quiet", action="store_true")
     return ap.parse_args()
+
+
+def strided_sample(torch, n, k, dev):
+    """k indices spread evenly over [0, n), first 0 and last n - 1, in exact
+    integer arithmetic (a float32 linspace rounds indices near 1e8 up to n)."""
+    k = max(1, min(int(k), int(n)))
+    i = torch.arange(k, device=dev, dtype=torch.int64)
+    return (i * (int(n) - 1)) // max(k - 1, 1)
 
 
 def log(args, *a):
@@ -306,6 +316,7 @@ def bench_listmode(args, torch, dev, stream):
     log(args, f"sensitivity: {n_lors} LORs x 4 rays in {sens_s:.2f}s")
     # hard threshold (voxels <= 1 -> 0), list-mode scaling 1/S, mask, init
     ctx.threshold(sens, sens.clone(), im.nvox, 1.0, 0.0, 0.0, 1.0, 0.0)
+    sens_thr = sens.clone() if motion and not args.no_dropin else None
     sens *= float(np.float32(1.0) / np.float32(S))
     x = torch.full(im.shape, 0.125, device=dev)
     ctx.threshold(x, sens, im.nvox, 0.0, 0.0, 0.0, 1.0, 0.0)
@@ -459,6 +470,13 @@ def bench_listmode(args, torch, dev, stream):
         plain = plain_per_frame(args, torch, ctx, proj, im, d1, d2, tof, mot, bounds, fcounts,
                                 x, x0, u, sens, shape, vbuf, xb, iters_cmp)
         log(args, "plain per-frame: " + json.dumps(plain))
+    dropin = None
+    if motion and tiled and sens_thr is not None:
+        del graph
+        graph = None
+        plans = []
+        dropin = dropin_osem(args, torch, d1, d2, tof, mot, xf, sens_thr, S, xb, iters_cmp)
+        log(args, "drop-in OSEM_MI: " + json.dumps(dropin))
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -517,6 +535,60 @@ def bench_listmode(args, torch, dev, stream):
         if "per_frame_fused_s" in plain:
             out["speedup_vs_plain_fused"] = plain["per_frame_fused_s"] / s_per_iter
         out["parity"]["batched_vs_plain_rel_l2"] = plain["rel_l2_vs_batched"]
+    if dropin is not None:
+        out["dropin"] = dropin
+    return out
+
+
+def dropin_osem(args, torch, d1, d2, tof, mot, xf, sens_thr, S, xb, iters):
+    """The same reconstruction through the C++ OSEM (OSEM_MI: the reconstruct
+    command's --gpu path): events aliased on the device in acquisition order
+    (event i in subset i mod S), the bench's thresholded sensitivity image,
+    graph-replayed subsets, autotuned launch shape, plans within 90 % of the
+    free device memory.  Its time per iteration is (t(iters) - t(1)) /
+    (iters - 1), so the per-call setup (plans, locality order, autotune)
+    cancels; its image after `iters` iterations is compared with the batched
+    one (EM from a uniform start does not depend on the start's value)."""
+    import pyyrtmi as Y
+    n = d1.numel()
+    # back to acquisition order (the bench keeps each subset contiguous)
+    perm = torch.cat([torch.arange(s, n, S, device=d1.device) for s in range(S)])
+    acq = {}
+    for k, v in (("d1", d1), ("d2", d2), ("tof", tof), ("motion", mot)):
+        a = torch.empty_like(v)
+        a[perm] = v
+        acq[k] = a
+    del perm
+    ysc = Y.Scanner("cfg2", 64 * 3.2, 3.2, 3.2, 20.0, 400.0, 512, 64, 1, 63, 1, 1)
+    yim = Y.ImageParams.fromParams(256, 256, 128, 1.6, 1.6, 1.6)
+    osem = Y.createOSEM(ysc)
+    octx = osem.getContext()
+    lm = Y.ListModeDeviceAlias(octx, n, acq["d1"], acq["d2"], tof=acq["tof"],
+                               motion=acq["motion"])
+    lm.setMotionTransforms(np.ascontiguousarray(xf, np.float32).reshape(-1))
+    osem.setImageParams(yim)
+    osem.setNumRays(4)
+    osem.addTOF(250.0, 5)
+    osem.num_OSEM_subsets = S
+    osem.setDataInput(lm)
+    simg = Y.ImageDeviceAlias(octx, yim, sens_thr)
+    osem.setSensitivityImages([simg])
+    t, img = {}, None
+    for k in (1, iters):
+        osem.num_MLEM_iterations = k
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        img = osem.reconstruct()
+        torch.cuda.synchronize()
+        t[k] = time.perf_counter() - t0
+    x = img.transferToHostMemory().reshape(tuple(xb.shape))
+    per_iter = (t[iters] - t[1]) / (iters - 1)
+    out = {"s_per_osem_iter": per_iter, "value": 2.0 * n / per_iter,
+           "call_s": {str(k): v for k, v in t.items()},
+           "planned_subsets": osem.numPlannedSubsets(),
+           "rel_l2_vs_batched": float(np.linalg.norm(x - xb.cpu().numpy()) /
+                                      np.linalg.norm(xb.cpu().numpy()))}
+    del lm, osem, acq
     return out
 
 
@@ -618,7 +690,7 @@ def oracle_sample(args, torch, ctx, proj, sc, im, pp, d1, d2, tof, mot, xf, boun
     sys.path.insert(0, os.path.join(REPO, "tests"))
     from helpers import oracle_image, oracle_proj, oracle_scanner, rel_l2
     k = min(args.oracle_events, bounds[1])
-    idx = torch.linspace(0, bounds[1] - 1, k, device=dev).long()
+    idx = strided_sample(torch, bounds[1], k, dev)
     a, b, t = d1[idx].contiguous(), d2[idx].contiguous(), tof[idx].contiguous()
     m = None if mot is None else mot[idx].contiguous()
     L = yrtmi.LorSet.pairs(a, b, tof=t, motion_frame=m)
@@ -896,13 +968,18 @@ def bench_dd(args, torch, dev, stream):
     for _ in range(args.warmup):
         batched()
     ctx.synchronize()
-    with ctx.graph() as g:
-        batched()
-    res["batched_s"] = timed(g.launch, args.steps, reset=False)
+    if args.no_graph:
+        g = None
+        step = batched
+    else:
+        with ctx.graph() as g:
+            batched()
+        step = g.launch
+    res["batched_s"] = timed(step, args.steps, reset=False)
     # the configuration's iterations from the start, for the comparisons
     xF.copy_(x0)
     for _ in range(iters):
-        g.launch()
+        step()
     ctx.synchronize()
     xb = xF.clone()
     evs = [tuple(torch.cuda.Event(enable_timing=True) for _ in range(3)) for _ in range(S)]
@@ -929,7 +1006,7 @@ def bench_dd(args, torch, dev, stream):
     log(args, "; ".join(f"{k_['kernel']}: {k_['avg_launch_ms']:.1f} ms, "
                         f"{k_['achieved']:.0f} GB/s alg" for k_ in kernels))
     dom = max(kernels, key=lambda k_: k_["avg_launch_ms"])
-    del g
+    del g, step
     plans = None
 
     if not args.no_plain:
@@ -1020,7 +1097,7 @@ def dd_oracle_sample(args, torch, ctx, proj, sc, im, pp, bins, meas, x, dev):
     sys.path.insert(0, os.path.join(REPO, "tests"))
     from helpers import oracle_image, oracle_proj, rel_l2
     k = min(args.oracle_events // 10, bins.numel())
-    idx = torch.linspace(0, bins.numel() - 1, k, device=dev).long()
+    idx = strided_sample(torch, bins.numel(), k, dev)
     b, m = bins[idx].contiguous(), meas[idx].contiguous()
     L = yrtmi.LorSet.bins_(b)
     y = torch.empty(k, device=dev)
@@ -1070,7 +1147,7 @@ def dd_cpu_baseline(args, torch, sc, im, pp, bins, meas, x):
     xh = x.cpu().numpy().reshape(im.shape)
 
     def run(k):
-        idx = torch.linspace(0, bins.numel() - 1, k, device=bins.device).long()
+        idx = strided_sample(torch, bins.numel(), k, bins.device)
         OL, mh = dd_bins_oracle(sc, bins[idx], meas[idx])
         uh = np.zeros(im.shape, np.float32)
         t0 = time.perf_counter()
@@ -1354,7 +1431,7 @@ def cpu_baseline(args, sc, im, pp, d1, d2, tof, bounds, x, mot=None, xf=None):
     xh = x.cpu().numpy()
 
     def run(k):
-        idx = torch.linspace(0, bounds[1] - 1, k, device=d1.device).long()
+        idx = strided_sample(torch, bounds[1], k, d1.device)
         a, b, t = (d1[idx].cpu().numpy().astype(np.uint32),
                    d2[idx].cpu().numpy().astype(np.uint32), tof[idx].cpu().numpy())
         m = None if mot is None else mot[idx].cpu().numpy()

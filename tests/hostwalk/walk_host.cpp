@@ -74,7 +74,7 @@ double tiles(const DImage& P, float3 a, float3 b, const DTof& T, float tofv, int
 	{
 		const int x0 = e.vx / TX * TX, y0 = e.vy / TY * TY, z0 = e.vz / TZ * TZ;
 		const WalkBox B{x0, std::min(x0 + TX, P.nx), y0, std::min(y0 + TY, P.ny),
-		                z0, std::min(z0 + TZ, P.nz), P.nx, P.nxy, 0};
+		                z0, std::min(z0 + TZ, P.nz), 1, P.nx, P.nxy, 0};
 		for (int pass = 0; pass < 2; pass++)
 		{
 			SidRay q;

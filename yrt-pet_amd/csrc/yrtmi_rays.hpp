@@ -59,6 +59,9 @@ __device__ __forceinline__ void getAlpha(T r0, T r1, T p1, T p2, T inv, T& amin,
 	}
 }
 
+// sqrt(log2(e) / 2): exp(-x^2 / 2) = 2^(-(x kTofC)^2)
+constexpr double kTofC = 0.84932180028801904272;
+
 // Siddon ray after its setup: the clipped alpha range, the next plane
 // crossing of each axis and what the walk needs to advance them
 // (ProjectorSiddon::project_helper<IS_FWD, FLAG_INCR=true, FLAG_TOF>,
@@ -71,12 +74,15 @@ struct SidRay
 	float axn, ayn, azn;  // alpha of the next plane per axis (FMAX: flat)
 	float xc, yc, zc;     // coordinate of that plane
 	float dnorm;
-	// TOF: the Gaussian's argument at a segment [a0, a1] is x_c = (a0 + a1)
-	// * tA - tB (tA = d / (2 sigma), tB = pc / sigma, pc = d/2 + tof_mm, both
-	// rounded once from FP64) and its weight (a1 - a0) * wn * exp(-x_c^2 /
-	// 2), wn = d * norm: the reference's value to a few FP32 ulps (it forms
-	// (a_mid d - pc) / sigma in FP32).
-	float tA, tB, wn;
+	// A segment [a0, a1] weighs (a1 - a0) * g * ws: ws = d (no TOF) or
+	// d * norm (TOF), a constant of the ray that the walk's callers apply once
+	// per ray; g = 1 (no TOF) or the TOF Gaussian exp(-x_c^2 / 2) at the
+	// segment's midpoint, x_c = (a_mid d - pc) / sigma, pc = d/2 + tof_mm.
+	// The walk forms u = (a0 + a1) tA - tB = x_c sqrt(log2(e) / 2) with one
+	// FMA (tA = kTofC d / (2 sigma), tB = kTofC pc / sigma, rounded once
+	// from FP64) and g = 2^(-u^2): the reference's value to a few FP32 ulps
+	// (it forms x_c in FP32).
+	float tA, tB, ws;
 	int dirx, diry, dirz;
 	// index of the first plane per axis, counted from the image face the
 	// ray enters through (x_start + k * dir * v)
@@ -103,7 +109,7 @@ __device__ __forceinline__ bool sidSetup(const DImage& P, float3 p1, float3 p2,
 	r.azn = FMAX;
 	r.tA = 0.f;
 	r.tB = 0.f;
-	r.wn = 0.f;
+	r.ws = 0.f;
 	r.kx = r.ky = r.kz = 0;
 	r.x1d = r.y1d = r.z1d = r.ddx = r.ddy = r.ddz = 0.0;
 	if constexpr (S64)
@@ -174,9 +180,8 @@ __device__ __forceinline__ bool sidSetup(const DImage& P, float3 p1, float3 p2,
 				amin = fmax(amin, 0.5 + (tmm - (double)tof.trunc) / dnormd);
 				amax = fmin(amax, 0.5 + (tmm + (double)tof.trunc) / dnormd);
 			}
-			r.tB = (float)((0.5 * dnormd + tmm) / (double)tof.sigma);
-			r.tA = (float)(0.5 * dnormd / (double)tof.sigma);
-			r.wn = (float)(dnormd * (double)tof.norm);
+			r.tB = (float)((0.5 * dnormd + tmm) / (double)tof.sigma * kTofC);
+			r.tA = (float)(0.5 * dnormd / (double)tof.sigma * kTofC);
 		}
 		if ((invx >= 0.0 && x1d > hx) || (invx < 0.0 && x1d < -hx) ||
 		    (invy >= 0.0 && y1d > hy) || (invy < 0.0 && y1d < -hy) ||
@@ -215,6 +220,7 @@ __device__ __forceinline__ bool sidSetup(const DImage& P, float3 p1, float3 p2,
 		r.invy = (float)invy;
 		r.invz = (float)invz;
 		r.dnorm = (float)dnormd;
+		r.ws = TOF ? (float)(dnormd * (double)tof.norm) : r.dnorm;
 		r.amax = (float)amax;
 		r.a_cur = (float)amin;
 	}
@@ -280,12 +286,11 @@ __device__ __forceinline__ bool sidSetup(const DImage& P, float3 p1, float3 p2,
 				amin = fmaxf(amin, lo);
 				amax = fminf(amax, hi);
 			}
-			// x_c = (a0 + a1) tA - tB, weight wn * exp(-x_c^2 / 2) per unit
-			// alpha, from FP64 (the S64 form)
-			r.tB = (float)((0.5 * (double)r.dnorm + (double)tmm) / (double)tof.sigma);
-			r.tA = (float)(0.5 * (double)r.dnorm / (double)tof.sigma);
-			r.wn = (float)((double)r.dnorm * (double)tof.norm);
+			// the Gaussian's constants from FP64 (the S64 form)
+			r.tB = (float)((0.5 * (double)r.dnorm + (double)tmm) / (double)tof.sigma * kTofC);
+			r.tA = (float)(0.5 * (double)r.dnorm / (double)tof.sigma * kTofC);
 		}
+		r.ws = TOF ? (float)((double)r.dnorm * (double)tof.norm) : r.dnorm;
 		r.xc = (r.invx > 0.0f) ? x0 : x1;
 		r.yc = (r.invy > 0.0f) ? y0 : y1;
 		r.zc = (r.invz > 0.0f) ? z0 : z1;
@@ -333,7 +338,7 @@ __device__ __forceinline__ bool sidSetup(const DImage& P, float3 p1, float3 p2,
 __device__ __forceinline__ void sidStore(float4* e, const SidRay& r)
 {
 	e[0] = make_float4(r.invx, r.invy, r.invz, r.amax);
-	e[1] = make_float4(r.dnorm, r.tA, r.tB, r.wn);
+	e[1] = make_float4(r.dnorm, r.tA, r.tB, r.ws);
 }
 
 __device__ __forceinline__ void sidLoad(float3 p1, float3 p2, const float4& s0,
@@ -349,7 +354,7 @@ __device__ __forceinline__ void sidLoad(float3 p1, float3 p2, const float4& s0,
 	r.dnorm = s1.x;
 	r.tA = s1.y;
 	r.tB = s1.z;
-	r.wn = s1.w;
+	r.ws = s1.w;
 	r.dirx = (r.invx >= 0.0f) ? 1 : -1;
 	r.diry = (r.invy >= 0.0f) ? 1 : -1;
 	r.dirz = (r.invz >= 0.0f) ? 1 : -1;
@@ -463,7 +468,9 @@ __device__ __forceinline__ bool sidPlaneShift(const DImage& P, const SidRay& r, 
 // shift (sidPlaneShift), counted from the entry face of the image as the
 // walk counts it (x_start + k * dir * v).  On voxel sizes that are sums of
 // few powers of two this reproduces the walk's accumulated plane positions
-// exactly; otherwise to an ulp.
+// exactly; otherwise to an ulp, and a plane recomputed an ulp before a0 is
+// taken at a0: the walk's first step there is an empty segment, then the
+// move across it, as where the planes coincide.
 template <bool S64>
 __device__ __forceinline__ void sidRestart(const DImage& P, SidRay& r, float a0,
                                            int vx, int vy, int vz, int dx, int dy,
@@ -484,7 +491,7 @@ __device__ __forceinline__ void sidRestart(const DImage& P, SidRay& r, float a0,
 		if (k != r.kx)  // else: the setup's first plane, kept as set up
 		{
 			r.xc = c;
-			r.axn = (c - r.p1.x) * r.invx;
+			r.axn = fmaxf((c - r.p1.x) * r.invx, a0);
 		}
 	}
 	if (r.p1.y != r.p2.y)
@@ -500,7 +507,7 @@ __device__ __forceinline__ void sidRestart(const DImage& P, SidRay& r, float a0,
 		if (k != r.ky)
 		{
 			r.yc = c;
-			r.ayn = (c - r.p1.y) * r.invy;
+			r.ayn = fmaxf((c - r.p1.y) * r.invy, a0);
 		}
 	}
 	if (r.p1.z != r.p2.z)
@@ -516,36 +523,40 @@ __device__ __forceinline__ void sidRestart(const DImage& P, SidRay& r, float a0,
 		if (k != r.kz)
 		{
 			r.zc = c;
-			r.azn = (c - r.p1.z) * r.invz;
+			r.azn = fmaxf((c - r.p1.z) * r.invz, a0);
 		}
 	}
 }
 
-// Voxel box a walk stays in: [lo, hi) per axis and the linear index
-// base + vz*sz + vy*sy + vx of a voxel (the image itself: base 0, sy = nx,
-// sz = nx*ny; an LDS tile: its own strides and origin).
+// Voxel box a walk stays in: [lo, hi) per axis, and the index
+// base + vz*sz + vy*sy + vx*sx of a voxel in the caller's units, modulo 2^32
+// (the image: voxels from its origin; an LDS tile: bytes from the tile's).
 struct WalkBox
 {
 	int x0, x1, y0, y1, z0, z1;
-	int sy, sz, base;
+	int sx, sy, sz, base;
 };
 
 __device__ __forceinline__ WalkBox imageBox(const DImage& P)
 {
-	return WalkBox{0, P.nx, 0, P.ny, 0, P.nz, P.nx, P.nxy, 0};
+	return WalkBox{0, P.nx, 0, P.ny, 0, P.nz, 1, P.nx, P.nxy, 0};
 }
 
 // FP32 walk (ProjectorSiddon.cpp:622-766) from r.a_cur to r.amax.  With
 // `first` the walk starts at the ray's entry and its first voxel is the one
 // holding the midpoint of the first nonempty segment; otherwise it starts in
 // (vx, vy, vz) (after sidRestart).  It ends when the voxel leaves `B`.
-// fn(vx, vy, vz, idx, a_start, w) is called for every segment.
+// fn(vx, vy, vz, idx, a_start, w) is called for every segment, w without
+// the ray's constant factor r.ws.
 //
 // Each step takes the nearest next plane m = min(axn, ayn, azn); every axis
 // whose plane is at m advances (the reference's three `<=` tests: ties
 // advance together), and the segment ends at min(m, amax).  The steady
 // loop is branch-free per axis (selects), so a wave runs one instruction
-// stream whatever the axes its lanes cross.
+// stream whatever the axes its lanes cross.  The voxel is kept as its index
+// and, per axis, the crossings left before it leaves the box (the walk ends
+// when one goes negative); its coordinates are derived from those where fn
+// uses them.
 template <bool TOF, bool S64, class Fn>
 __device__ __forceinline__ void sidWalk(const DImage& P, const SidRay& r,
                                         const DTof& tof, const WalkBox& B,
@@ -556,7 +567,6 @@ __device__ __forceinline__ void sidWalk(const DImage& P, const SidRay& r,
 	const float sdx = (float)r.dirx * P.vx, sdy = (float)r.diry * P.vy,
 	            sdz = (float)r.dirz * P.vz;
 	const float amaxf = r.amax;
-	const float dnorm = r.dnorm;
 	float axn = r.axn, ayn = r.ayn, azn = r.azn;
 	float xc = r.xc, yc = r.yc, zc = r.zc;
 	float a_cur = r.a_cur, a_next = 0.f;
@@ -574,27 +584,48 @@ __device__ __forceinline__ void sidWalk(const DImage& P, const SidRay& r,
 		axn = mx ? (xn - r.p1.x) * r.invx : axn;
 		ayn = my ? (yn - r.p1.y) * r.invy : ayn;
 		azn = mz ? (zn - r.p1.z) * r.invz : azn;
-		a_next = fminf(m, amaxf);
+		// min(m, amax) as an integer min of the bit patterns: equal to the
+		// float min here (amax > 0, m no NaN: a negative m has the smaller
+		// pattern as a signed integer, and non-negative floats order as
+		// their patterns), and one instruction where fminf would first
+		// quiet a possible signalling NaN
+		a_next = __int_as_float(min(__float_as_int(m), __float_as_int(amaxf)));
 	};
-	// the voxel in box coordinates (lx, ly, lz) = (vx - x0, vy - y0, vz - z0);
-	// its index base + vz * sz + vy * sy + vx is off + lx, off = base +
-	// vz * sz + vy * sy + x0
-	int lx = 0, ly = 0, lz = 0, off = 0;
-	auto emit = [&](int off) {
-		float w;
+	// the voxel: crossings left per axis in its direction of travel, and
+	// its index
+	int cx = 0, cy = 0, cz = 0;
+	uint32_t idx = 0u;
+	const uint32_t xs = (uint32_t)(r.dirx * B.sx), ys = (uint32_t)(r.diry * B.sy),
+	               zs = (uint32_t)(r.dirz * B.sz);
+	auto enter = [&]() {
+		cx = (r.dirx > 0) ? B.x1 - 1 - vx : vx - B.x0;
+		cy = (r.diry > 0) ? B.y1 - 1 - vy : vy - B.y0;
+		cz = (r.dirz > 0) ? B.z1 - 1 - vz : vz - B.z0;
+		idx = (uint32_t)B.base + (uint32_t)vz * (uint32_t)B.sz + (uint32_t)vy * (uint32_t)B.sy +
+		      (uint32_t)vx * (uint32_t)B.sx;
+	};
+	auto emit = [&]() {
+		float w = a_next - a_cur;
 		if (TOF)
 		{
 			// TimeOfFlightHelper::getWeight (TimeOfFlight.hpp:175-185): the
-			// Gaussian at the segment's midpoint, its argument
-			// (a_mid d - pc) / sigma formed as (a0 + a1) tA - tB with one FMA
-			// and exp(-x^2/2) = 2^(-x^2 log2(e) / 2)
-			const float xc = fmaf(a_cur + a_next, r.tA, -r.tB);
-			w = (a_next - a_cur) * r.wn * __builtin_amdgcn_exp2f(-0.72134752f * xc * xc);
+			// Gaussian at the segment's midpoint, u = (a0 + a1) tA - tB
+			const float u = fmaf(a_cur + a_next, r.tA, -r.tB);
+			w = w * __builtin_amdgcn_exp2f(-(u * u));
 		}
-		else
-			w = (a_next - a_cur) * dnorm;
-		fn(lx + B.x0, ly + B.y0, lz + B.z0, off + lx, a_cur, w);
+		const int gx = (r.dirx > 0) ? B.x1 - 1 - cx : B.x0 + cx;
+		const int gy = (r.diry > 0) ? B.y1 - 1 - cy : B.y0 + cy;
+		const int gz = (r.dirz > 0) ? B.z1 - 1 - cz : B.z0 + cz;
+		fn(gx, gy, gz, idx, a_cur, w);
 		a_cur = a_next;
+	};
+	// the voxel across the planes of the last step: crossings left and
+	// index after the move (computed beside the step's compares)
+	auto moved = [&](int& nx, int& ny, int& nz, uint32_t& ni) {
+		nx = cx - (mx ? 1 : 0);
+		ny = cy - (my ? 1 : 0);
+		nz = cz - (mz ? 1 : 0);
+		ni = idx + (mx ? xs : 0u) + (my ? ys : 0u) + (mz ? zs : 0u);
 	};
 	if (first)
 	{
@@ -610,38 +641,29 @@ __device__ __forceinline__ void sidWalk(const DImage& P, const SidRay& r,
 		sidMidVoxel<S64>(P, r, a_cur, a_next, vx, vy, vz);
 		if (vx < B.x0 || vx >= B.x1 || vy < B.y0 || vy >= B.y1 || vz < B.z0 || vz >= B.z1)
 			return;
-		lx = vx - B.x0;
-		ly = vy - B.y0;
-		lz = vz - B.z0;
-		off = B.base + vz * B.sz + vy * B.sy + B.x0;
-		emit(off);
+		enter();
+		int nx, ny, nz;
+		uint32_t ni;
+		moved(nx, ny, nz, ni);
+		emit();
+		cx = nx;
+		cy = ny;
+		cz = nz;
+		idx = ni;
 	}
 	else
+		enter();
+	while (a_cur < amaxf && (cx | cy | cz) >= 0)
 	{
-		lx = vx - B.x0;
-		ly = vy - B.y0;
-		lz = vz - B.z0;
-		off = B.base + vz * B.sz + vy * B.sy + B.x0;
-	}
-	const int ys = r.diry * B.sy, zs = r.dirz * B.sz;
-	const unsigned bx = (unsigned)(B.x1 - B.x0), by = (unsigned)(B.y1 - B.y0),
-	               bz = (unsigned)(B.z1 - B.z0);
-	// each pass moves the voxel across the planes of the previous step (none
-	// before the first: the masks start false)
-	while (a_cur < amaxf)
-	{
-		lx += mx ? r.dirx : 0;
-		ly += my ? r.diry : 0;
-		lz += mz ? r.dirz : 0;
-		off += (my ? ys : 0) + (mz ? zs : 0);
-		if (((unsigned)lx >= bx) | ((unsigned)ly >= by) | ((unsigned)lz >= bz))
-			break;
 		step();
-		// after the first voxel the planes only come after a_cur; a walk
-		// re-entered at a tile boundary can meet one recomputed an ulp early,
-		// which is still a crossing: an empty segment, then the move
-		a_next = fmaxf(a_next, a_cur);
-		emit(off);
+		int nx, ny, nz;
+		uint32_t ni;
+		moved(nx, ny, nz, ni);
+		emit();
+		cx = nx;
+		cy = ny;
+		cz = nz;
+		idx = ni;
 	}
 }
 
@@ -664,15 +686,16 @@ __device__ __forceinline__ float siddonRay(const DImage& P,
 		// gathers hit L2; keeping several in flight per lane measured slower)
 		float acc = 0.f;
 		sidWalk<TOF, S64>(P, r, tof, imageBox(P), true, 0, 0, 0,
-		                  [&](int, int, int, int idx, float, float w) {
+		                  [&](int, int, int, uint32_t idx, float, float w) {
 			                  acc = fmaf(w, img[idx], acc);
 		                  });
-		return acc;
+		return acc * r.ws;
 	}
+	const float vs = val * r.ws;
 	sidWalk<TOF, S64>(P, r, tof, imageBox(P), true, 0, 0, 0,
-	                  [&](int, int, int, int idx, float, float w) {
+	                  [&](int, int, int, uint32_t idx, float, float w) {
 		                  if (MODE == MODE_BACK)
-			                  unsafeAtomicAdd(img + idx, val * w);
+			                  unsafeAtomicAdd(img + idx, vs * w);
 		                  else
 			                  cnt++;
 	                  });

@@ -352,9 +352,13 @@ __device__ __forceinline__ void tileChunk(const KArgs& a, const DTiles& T,
 	B.x1 = min(B.x0 + TX, a.im.nx);
 	B.y1 = min(B.y0 + TY, a.im.ny);
 	B.z1 = min(B.z0 + TZ, a.im.nz);
-	B.sy = TX;
-	B.sz = TX * TY;
-	B.base = -(B.z0 * B.sz + B.y0 * B.sy + B.x0);
+	// voxel index in bytes of the LDS tile's words (8: fixed point, 4: FP32)
+	auto wordBytes = [&](int esz) {
+		B.sx = esz;
+		B.sy = TX * esz;
+		B.sz = TX * TY * esz;
+		B.base = -(B.z0 * B.sz + B.y0 * B.sy + B.x0 * B.sx);
+	};
 	constexpr int NW = (SHAPE == 2) ? NV / 2 : NV;  // 64-bit words of the LDS tile
 	for (int i = threadIdx.x; i < NW; i += NT)
 		acc64[i] = 0ull;
@@ -457,7 +461,8 @@ __device__ __forceinline__ void tileChunk(const KArgs& a, const DTiles& T,
 			ddVisit<TOF>(a.im, D, a.pr, xlo, min(xhi, (D.flag_y ? B.y1 : B.x1) - 1), ylo, yhi,
 			             B.z0, B.z1 - 1, [&](int, int xyi, int yxi, int zi, float w) {
 				             const int gx = D.flag_y ? yxi : xyi, gy = D.flag_y ? xyi : yxi;
-				             add(zi * B.sz + gy * B.sy + gx + B.base, gx, gy, zi, val * w);
+				             add((uint32_t)(B.base + zi * B.sz + gy * B.sy + gx * B.sx), gx, gy, zi,
+				                 val * w);
 			             });
 		}
 		else
@@ -486,9 +491,10 @@ __device__ __forceinline__ void tileChunk(const KArgs& a, const DTiles& T,
 				return;
 			if (!first)
 				sidRestart<S64>(a.im, s, rc.al, vx, vy, vz, dx, dy, dz);
+			const float vs = val * s.ws;
 			sidWalk<TOF, S64>(a.im, s, a.pr.tof, B, first, vx, vy, vz,
-			                  [&](int gx, int gy, int gz, int idx, float, float w) {
-				                  add(idx, gx, gy, gz, val * w);
+			                  [&](int gx, int gy, int gz, uint32_t idx, float, float w) {
+				                  add(idx, gx, gy, gz, vs * w);
 			                  });
 		}
 	};
@@ -520,18 +526,27 @@ __device__ __forceinline__ void tileChunk(const KArgs& a, const DTiles& T,
 	constexpr double kBias = 6755399441055744.0;  // 1.5 * 2^52
 	constexpr unsigned long long kBiasBits = 0x4338000000000000ull;
 	if (fixed)
-		pipeline([&](int idx, int gx, int gy, int gz, float c) {
+	{
+		wordBytes(8);
+		pipeline([&](uint32_t idx, int gx, int gy, int gz, float c) {
 			if (fabsf(c) < climit)
 			{
 				const double d = fma((double)c, scale, kBias);
-				ldsAdd64(&acc64[idx], (long long)((unsigned long long)__double_as_longlong(d) -
-				                                  kBiasBits));
+				ldsAdd64(reinterpret_cast<unsigned long long*>(
+				             reinterpret_cast<char*>(acc64) + idx),
+				         (long long)((unsigned long long)__double_as_longlong(d) - kBiasBits));
 			}
 			else  // out of the fixed-point range, or not finite
 				unsafeAtomicAdd(yf + (int64_t)gz * a.im.nxy + (int64_t)gy * a.im.nx + gx, c);
 		});
+	}
 	else
-		pipeline([&](int idx, int, int, int, float c) { ldsAdd(&acc[idx], c); });
+	{
+		wordBytes(4);
+		pipeline([&](uint32_t idx, int, int, int, float c) {
+			ldsAdd(reinterpret_cast<float*>(reinterpret_cast<char*>(acc) + idx), c);
+		});
+	}
 	__syncthreads();
 	for (int i = threadIdx.x; i < NV; i += NT)
 	{
@@ -793,8 +808,18 @@ hipError_t launchRayPrep(const KArgs& a, float4* tab, hipStream_t st)
 	if (total == 0)
 		return hipSuccess;
 	const unsigned g = (unsigned)std::min<int64_t>(std::max<int64_t>((total + 255) / 256, 1), 65536);
+	// Siddon: the FP64 set-up stores the end points only (two float4 per
+	// ray); the FP32 set-up stores its set-up too (four), with the TOF
+	// constants when the projector has TOF
 	if (a.pr.type != 1)
-		kRayPrep<false, false, false><<<g, 256, 0, st>>>(a, tab);
+	{
+		if (a.pr.setup64)
+			kRayPrep<false, false, true><<<g, 256, 0, st>>>(a, tab);
+		else if (a.pr.useTof)
+			kRayPrep<false, true, false><<<g, 256, 0, st>>>(a, tab);
+		else
+			kRayPrep<false, false, false><<<g, 256, 0, st>>>(a, tab);
+	}
 	else if (a.pr.useTof)
 	{
 		if (a.pr.setup64)
