@@ -228,6 +228,19 @@ int yrtmi_back(yrtmi_ctx* ctx, const yrtmi_proj* p, const yrtmi_image* im,
 int yrtmi_em_subset(yrtmi_ctx* ctx, const yrtmi_proj* p, const yrtmi_image* im,
                     const float* x, float* u, const yrtmi_lors* lors,
                     const yrtmi_corrections* corr);
+/* As yrtmi_em_subset, allocating nothing.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under graph capture,
+ * tight serving loops).  The per-XCD partial images (per-XCD accumulation,
+ * yrtmi_proj_set_accumulation) live in the library's static scratch array
+ * of 16 Mi floats (64 MiB) per device, held until the process ends: an image
+ * of more than 2 Mi voxels x frames (8 XCDs; the partial images are rounded
+ * to 64 floats each) is refused with YRTMI_EINVAL.  The scratch is cleared
+ * by a fill kernel at every call (recordable).  Every call on one device
+ * shares that device's array: calls that overlap on one device (two
+ * streams, two host threads) race.  Direct accumulation uses no scratch. */
+int yrtmi_em_subset_noalloc(yrtmi_ctx* ctx, const yrtmi_proj* p, const yrtmi_image* im,
+                            const float* x, float* u, const yrtmi_lors* lors,
+                            const yrtmi_corrections* corr);
 
 /* Sensitivity: img += A^T(acf .* sens) over `lors` (typically a whole
  * histogram subset), the projector used WITHOUT TOF.  att_img (nullable,

@@ -18,6 +18,12 @@
 
 using namespace yrtmi;
 
+// The scratch array of yrtmi_em_subset_noalloc (its per-XCD partial
+// images): 16 Mi floats (64 MiB) on each device the library runs on, held
+// until the process ends.
+constexpr int64_t kScratchFloats = int64_t(16) << 20;
+static __device__ __align__(256) float scratch[kScratchFloats];
+
 namespace
 {
 thread_local std::string g_err;
@@ -214,6 +220,40 @@ int buildLors(yrtmi_ctx* c, const yrtmi_lors* l, DLors& d, DScanner& s)
 	return YRTMI_OK;
 }
 
+// Scratch of yrtmi_em_subset_noalloc: its per-XCD partial images, in a
+// file-scope device array (64 MiB on every device the library runs on, held
+// until the process ends) instead of memory the library allocates.  Host
+// code never uses the array's name (on the host it is a stand-in, not a
+// device address): yrtmi_ctx_create takes each device's address of it once
+// with hipGetSymbolAddress, with that device current, and the entry point
+// uses that address.
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratchMutex;
+float* g_scratchPtr[kMaxScratchDevices] = {};
+
+// this device's address of `scratch` (the device is current)
+int scratchSetup(int device)
+{
+	if (device < 0 || device >= kMaxScratchDevices)
+		return fail(YRTMI_EINVAL, "device index beyond the scratch table");
+	std::lock_guard<std::mutex> lk(g_scratchMutex);
+	if (g_scratchPtr[device] != nullptr)
+		return YRTMI_OK;
+	float* ptr = nullptr;
+	const hipError_t e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+	if (e != hipSuccess || ptr == nullptr)
+		return hipFail(e != hipSuccess ? e : hipErrorInvalidSymbol,
+		               "scratch: hipGetSymbolAddress");
+	g_scratchPtr[device] = ptr;
+	return YRTMI_OK;
+}
+
+float* scratchOf(int device)
+{
+	std::lock_guard<std::mutex> lk(g_scratchMutex);
+	return (device >= 0 && device < kMaxScratchDevices) ? g_scratchPtr[device] : nullptr;
+}
+
 // Make sure the context holds nxcc zeroed partial images of at least n
 // floats each.  Allocates only when a larger image than before is seen.
 int ensurePartials(yrtmi_ctx* c, int64_t n)
@@ -281,6 +321,12 @@ int yrtmi_ctx_create(int device, void* stream, yrtmi_ctx** out)
 	hipError_t e = hipSetDevice(device);
 	if (e == hipSuccess)
 		e = hipMalloc(&c->counter, sizeof(unsigned long long));
+	if (e == hipSuccess && scratchSetup(device) != YRTMI_OK)
+	{
+		(void)hipFree(c->counter);
+		delete c;
+		return YRTMI_EHIP;
+	}
 	if (e == hipSuccess)
 	{
 		int nx = 1;
@@ -896,6 +942,67 @@ int yrtmi_em_subset(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
 	r = foldAccum(c, a, u, ny);
 	if (r)
 		return r;
+	return YRTMI_OK;
+}
+
+int yrtmi_em_subset_noalloc(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
+                            const float* x, float* u, const yrtmi_lors* l,
+                            const yrtmi_corrections* corr)
+{
+	KArgs a;
+	int r = prepare(c, p, im, l, a);
+	if (r)
+		return r;
+	if (x == nullptr || u == nullptr)
+		return fail(YRTMI_EINVAL, "em_subset_noalloc: null image");
+	a.x = x;
+	a.y = u;
+	a.c.g = 1.0f;
+	a.c.denomThr = 1e-35f;
+	if (corr != nullptr)
+	{
+		a.c.meas = corr->meas;
+		a.c.sens = corr->sens;
+		a.c.att = corr->att;
+		a.c.randoms = corr->randoms;
+		a.c.scatter = corr->scatter;
+		a.c.g = corr->global_scale;
+		a.c.denomThr = corr->denom_threshold;
+		a.c.precorr = corr->precorr;
+	}
+	const int64_t ny = a.im.nvox * a.im.nt;
+	a.part = nullptr;
+	if (p->accum == YRTMI_ACCUM_XCD && c->nxcc > 1)
+	{
+		// the partial images in the scratch array: nxcc of stride floats
+		const int64_t stride = (ny + 63) / 64 * 64;
+		if (stride * c->nxcc > kScratchFloats)
+		{
+			char msg[200];
+			std::snprintf(msg, sizeof msg,
+			              "em_subset_noalloc: %lld voxels x frames need %lld floats of "
+			              "scratch (%d partial images), above its %lld",
+			              (long long)ny, (long long)(stride * c->nxcc), c->nxcc,
+			              (long long)kScratchFloats);
+			return fail(YRTMI_EINVAL, msg);
+		}
+		float* ptr = scratchOf(c->device);
+		if (ptr == nullptr)
+			return fail(YRTMI_ESTATE, "em_subset_noalloc: no scratch on this device");
+		if (l->n == 0)
+			return YRTMI_OK;
+		// the array keeps the last call's contents: clear what this call
+		// accumulates into (a fill kernel, also under graph recording)
+		HIP_TRY(launchFill(ptr, stride * c->nxcc, 0.f, c->stream));
+		a.part = ptr;
+		a.partStride = stride;
+		a.nxcc = c->nxcc;
+	}
+	if (l->n == 0)
+		return YRTMI_OK;
+	HIP_TRY(launchEvents(MODE_EM, a, p->lpe, p->block, p->grid, c->stream));
+	if (a.part != nullptr)
+		HIP_TRY(launchFoldPartials(u, a.part, a.partStride, a.nxcc, ny, c->stream));
 	return YRTMI_OK;
 }
 

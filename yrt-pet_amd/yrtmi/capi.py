@@ -107,6 +107,8 @@ _SIGS = {
     "yrtmi_back": (C.c_int, [_P, _P, C.POINTER(Image), _P, C.POINTER(Lors), _P]),
     "yrtmi_em_subset": (C.c_int, [_P, _P, C.POINTER(Image), _P, _P,
                                   C.POINTER(Lors), C.POINTER(Corrections)]),
+    "yrtmi_em_subset_noalloc": (C.c_int, [_P, _P, C.POINTER(Image), _P, _P,
+                                          C.POINTER(Lors), C.POINTER(Corrections)]),
     "yrtmi_sensitivity": (C.c_int, [_P, _P, C.POINTER(Image), _P, C.POINTER(Lors),
                                     C.POINTER(Image), _P, _P]),
     "yrtmi_attenuation_factors": (C.c_int, [_P, C.POINTER(Image), _P,
