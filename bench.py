@@ -86,6 +86,7 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_PROFILES = {
     "motion": os.path.join(REPO, "profiles", "em_motion_tof_250M_pmc_traffic.md"),
     "listmode": os.path.join(REPO, "profiles", "em_listmode_tof_100M_pmc_traffic_sq.md"),
+    "dd": os.path.join(REPO, "profiles", "dd_psf_histo_16f_pmc.md"),
 }
 
 
@@ -449,7 +450,7 @@ def bench_listmode(args, torch, dev, stream):
     iters_cmp = 3
     xb = None
     parity = {}
-    if tiled and (not args.no_plain or not args.no_oracle_sample):
+    if tiled and (not args.no_plain or not args.no_oracle_sample or sens_thr is not None):
         x.copy_(x0)
         for _ in range(iters_cmp):
             if graph is not None:
@@ -545,10 +546,12 @@ def dropin_osem(args, torch, d1, d2, tof, mot, xf, sens_thr, S, xb, iters):
     command's --gpu path): events aliased on the device in acquisition order
     (event i in subset i mod S), the bench's thresholded sensitivity image,
     graph-replayed subsets, autotuned launch shape, plans within 90 % of the
-    free device memory.  Its time per iteration is (t(iters) - t(1)) /
-    (iters - 1), so the per-call setup (plans, locality order, autotune)
-    cancels; its image after `iters` iterations is compared with the batched
-    one (EM from a uniform start does not depend on the start's value)."""
+    free device memory.  Its time per iteration is that of the graph-replayed
+    iterations, which OSEM_MI times itself (timeIterations: the device
+    synchronized after each); `setup_s` is the rest of the call (plans,
+    locality order, autotune).  Its image after `iters` iterations is
+    compared with the batched one (EM from a uniform start does not depend on
+    the start's value)."""
     import pyyrtmi as Y
     n = d1.numel()
     # back to acquisition order (the bench keeps each subset contiguous)
@@ -573,18 +576,21 @@ def dropin_osem(args, torch, d1, d2, tof, mot, xf, sens_thr, S, xb, iters):
     osem.setDataInput(lm)
     simg = Y.ImageDeviceAlias(octx, yim, sens_thr)
     osem.setSensitivityImages([simg])
-    t, img = {}, None
-    for k in (1, iters):
-        osem.num_MLEM_iterations = k
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        img = osem.reconstruct()
-        torch.cuda.synchronize()
-        t[k] = time.perf_counter() - t0
+    osem.num_MLEM_iterations = iters
+    osem.timeIterations = True
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    img = osem.reconstruct()
+    torch.cuda.synchronize()
+    call_s = time.perf_counter() - t0
+    its = list(osem.iterationSeconds())
     x = img.transferToHostMemory().reshape(tuple(xb.shape))
-    per_iter = (t[iters] - t[1]) / (iters - 1)
+    # the graph-replayed iterations (after the first, call by call, and the
+    # second, recorded then launched)
+    replay = its[2:] if len(its) > 2 else its[-1:]
+    per_iter = float(np.mean(replay))
     out = {"s_per_osem_iter": per_iter, "value": 2.0 * n / per_iter,
-           "call_s": {str(k): v for k, v in t.items()},
+           "iteration_s": its, "call_s": call_s, "setup_s": call_s - sum(its),
            "planned_subsets": osem.numPlannedSubsets(),
            "rel_l2_vs_batched": float(np.linalg.norm(x - xb.cpu().numpy()) /
                                       np.linalg.norm(xb.cpu().numpy()))}
@@ -1045,6 +1051,12 @@ def bench_dd(args, torch, dev, stream):
         args, torch, ctx, proj, sc, im1, pp, per[0][0][0], per[0][0][1], xb[0:1], dev)
     cpu = None if args.no_cpu_baseline else dd_cpu_baseline(
         args, torch, sc, im1, pp, per[0][0][0], per[0][0][1], xb[0:1])
+    # measured traffic applies to the profiled configuration only
+    traffic, traffic_src = None, None
+    if (args.dd_min == 1e6 and args.dd_max == 40e6 and S == 12 and args.order == "lor"):
+        traffic = profiled_traffic(TRAFFIC_PROFILES["dd"], dom["kernel"])
+        if traffic is not None:
+            traffic_src = os.path.relpath(TRAFFIC_PROFILES["dd"], REPO)
     out = {
         "metric": "fwd+back event-projections/s, s/OSEM iter; batched DD+PSF histogram frames "
                   "vs plain per-frame",
@@ -1061,7 +1073,8 @@ def bench_dd(args, torch, dev, stream):
                                f"[{args.dd_min:g}, {args.dd_max:g}] counts, {S} subsets",
                    "frame_counts": counts, "nonzero_bins": n, "footprint_voxels": sum(fp)},
         "roofline": {"bound": "hbm", "achieved": dom["achieved"], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": dom["frac"], "traffic": None,
+                     "unit": "GB/s", "frac": dom["frac"], "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "kernel": dom["kernel"], "avg_launch_ms": dom["avg_launch_ms"],
                      "alg_bytes_per_launch": dom["alg_bytes_per_launch"],
                      "footprint_voxels_per_bin": sum(fp) / n},

@@ -52,7 +52,7 @@ double tiles(const DImage& P, float3 a, float3 b, const DTof& T, float tofv, int
 	int last = -1;
 	SidRay w = r;
 	sidWalk<TOF, S64>(P, w, T, imageBox(P), true, 0, 0, 0,
-	                  [&](int vx, int vy, int vz, int idx, float a0, float wt) {
+	                  [&](int vx, int vy, int vz, int idx, float a0, float wt, uint32_t) {
 		                  full[idx] += wt;
 		                  out[1]++;
 		                  const int t = ((vz / TZ) * 4096 + vy / TY) * 4096 + vx / TX;
@@ -66,27 +66,31 @@ double tiles(const DImage& P, float3 a, float3 b, const DTof& T, float tofv, int
 	// the plan's ray-table entry of an FP32 set-up (kRayPrep)
 	float4 st[2];
 	sidStore(st, r);
-	// the tiled walks, the FP32 set-up's non-first records re-entered from
-	// the stored set-up (kTileBack) and, beside them, from a fresh set-up:
-	// out[3] counts voxels where the two differ at all
+	// the tiled walks, the FP32 set-up's records re-entered from the stored
+	// set-up (kTileBack) and, beside them, from a fresh set-up: out[3]
+	// counts voxels where the two differ at all
 	std::map<int, double> fresh;
 	for (const Ent& e : ents)
 	{
 		const int x0 = e.vx / TX * TX, y0 = e.vy / TY * TY, z0 = e.vz / TZ * TZ;
 		const WalkBox B{x0, std::min(x0 + TX, P.nx), y0, std::min(y0 + TY, P.ny),
 		                z0, std::min(z0 + TZ, P.nz), 1, P.nx, P.nxy, 0};
+		// FP32 set-up: every record re-entered (the first at the ray's start
+		// alpha and first voxel), as kTileBack does; FP64: the first record
+		// walks from the entry
+		const bool entry = S64 && e.first;
 		for (int pass = 0; pass < 2; pass++)
 		{
 			SidRay q;
-			if (pass == 0 && !S64 && !e.first)
+			if (pass == 0 && !S64)
 				sidLoad(a, b, st[0], st[1], q);
 			else
 				sidSetup<TOF, S64>(P, a, b, T, tofv, q);
-			if (!e.first)
+			if (!entry)
 				sidRestart<S64>(P, q, e.a, e.vx, e.vy, e.vz, dx, dy, dz);
 			std::map<int, double>& m = pass == 0 ? tiled : fresh;
-			sidWalk<TOF, S64>(P, q, T, B, e.first, e.vx, e.vy, e.vz,
-			                  [&](int, int, int, int idx, float, float wt) {
+			sidWalk<TOF, S64>(P, q, T, B, entry, e.vx, e.vy, e.vz,
+			                  [&](int, int, int, int idx, float, float wt, uint32_t) {
 				                  m[idx] += wt;
 				                  if (wt > 0.f && pass == 0)
 					                  out[2]++;

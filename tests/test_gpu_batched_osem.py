@@ -135,7 +135,7 @@ def test_histogram_configuration_full_size(gpu):
     check("cfg1_full_size", x, xo, TOL_IMG)
 
 
-def listmode_osem(ysc, yim, ev, S, iters, **opts):
+def listmode_osem(ysc, yim, ev, S, iters, out=None, **opts):
     osem = Y.createOSEM(ysc)
     ctx = osem.getContext()
     lm = Y.ListModeDeviceOwned(ctx, ev["d1"].astype(np.uint32), ev["d2"].astype(np.uint32),
@@ -150,6 +150,8 @@ def listmode_osem(ysc, yim, ev, S, iters, **opts):
     osem.setDataInput(lm)
     osem.generateSensitivityImages()
     x = osem.reconstruct().transferToHostMemory()
+    if out is not None:
+        out["iteration_s"] = list(osem.iterationSeconds())
     return x, osem.numPlannedSubsets()
 
 
@@ -179,6 +181,13 @@ def test_memory_budget_graphs_and_autotune(gpu):
     x_plain, _ = listmode_osem(ysc, yim, ev, S, iters, useGraphs=False, useAutotune=False)
     check("no_graph_no_tune", x_plain, xo, TOL_IMG)
     check("graphs_autotune_vs_calls", x_def, x_plain, 1e-5)
+    # iterations timed by OSEM_MI itself: one time per iteration, the image
+    # as without timing
+    timed = {}
+    x_timed, _ = listmode_osem(ysc, yim, ev, S, iters, out=timed, timeIterations=True)
+    print(f"iteration seconds {timed['iteration_s']}", flush=True)
+    assert len(timed["iteration_s"]) == iters and min(timed["iteration_s"]) > 0
+    check("timed_vs_untimed", x_timed, x_def, 1e-5)
     # a budget of one byte: no plan fits, every subset runs unplanned
     x_none, np_none = listmode_osem(ysc, yim, ev, S, iters, memoryBudget=1)
     assert np_none == 0

@@ -391,6 +391,8 @@ PYBIND11_MODULE(_pyyrtmi, m)
 	    .def_readwrite("useAutotune", &OSEM_MI::useAutotune)
 	    .def_readwrite("memoryBudget", &OSEM_MI::memoryBudget)
 	    .def("numPlannedSubsets", &OSEM_MI::numPlannedSubsets)
+	    .def_readwrite("timeIterations", &OSEM_MI::timeIterations)
+	    .def("iterationSeconds", &OSEM_MI::iterationSeconds)
 	    .def("numHistogramFrames", &OSEM_MI::numHistogramFrames);
 
 	m.def(

@@ -4,6 +4,7 @@
 #include "yrtmi_io.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <fstream>
 #include <sstream>
@@ -1391,11 +1392,27 @@ std::unique_ptr<ImageDevice> OSEM_MI::reconstruct()
 	// the first iteration call by call (it also makes any per-XCD partial
 	// images of unplanned subsets), the next recorded once as a graph and
 	// replayed for every later one
+	m_iterSeconds.clear();
+	auto clock = std::chrono::steady_clock::now();
+	auto lap = [&]() {
+		if (!timeIterations)
+			return;
+		m_ctx->synchronize();
+		const auto now = std::chrono::steady_clock::now();
+		m_iterSeconds.push_back(std::chrono::duration<double>(now - clock).count());
+		clock = now;
+	};
+	if (timeIterations)
+	{
+		m_ctx->synchronize();
+		clock = std::chrono::steady_clock::now();
+	}
 	int it = 0;
 	if (num_MLEM_iterations > 0)
 	{
 		iteration();
 		it = 1;
+		lap();
 	}
 	if (useGraphs && num_MLEM_iterations > 1)
 	{
@@ -1416,11 +1433,17 @@ std::unique_ptr<ImageDevice> OSEM_MI::reconstruct()
 		check(yrtmi_graph_end(c, &g), "graph end");
 		std::unique_ptr<yrtmi_graph, int (*)(yrtmi_graph*)> gh(g, yrtmi_graph_destroy);
 		for (; it < num_MLEM_iterations; it++)
+		{
 			check(yrtmi_graph_launch(c, g), "graph launch");
+			lap();
+		}
 		m_ctx->synchronize();
 	}
 	for (; it < num_MLEM_iterations; it++)
+	{
 		iteration();
+		lap();
+	}
 	m_ctx->synchronize();
 	return x;
 }

@@ -411,6 +411,11 @@ public:
 	size_t memoryBudget = 0;
 	// Subsets that got a tile plan in the last reconstruct().
 	int numPlannedSubsets() const { return m_numPlanned; }
+	// Time each iteration of reconstruct() (wall seconds, the device
+	// synchronized after each; off by default: it adds a synchronization per
+	// iteration).  iterationSeconds() holds the last reconstruct()'s.
+	bool timeIterations = false;
+	const std::vector<double>& iterationSeconds() const { return m_iterSeconds; }
 
 	const Context& getContext() const { return *m_ctx; }
 
@@ -430,6 +435,7 @@ private:
 	const ListModeDevice* m_lm = nullptr;
 	std::vector<const Histogram3DDevice*> m_histos;
 	int m_numPlanned = 0;
+	std::vector<double> m_iterSeconds;
 	int m_tileShape = -1, m_persistent = 0;  // from the autotuner
 	float m_globalScale = 1.0f;
 	const ImageDevice* m_att = nullptr;

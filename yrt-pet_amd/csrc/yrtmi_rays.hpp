@@ -546,8 +546,11 @@ __device__ __forceinline__ WalkBox imageBox(const DImage& P)
 // `first` the walk starts at the ray's entry and its first voxel is the one
 // holding the midpoint of the first nonempty segment; otherwise it starts in
 // (vx, vy, vz) (after sidRestart).  It ends when the voxel leaves `B`.
-// fn(vx, vy, vz, idx, a_start, w) is called for every segment, w without
-// the ray's constant factor r.ws.
+// fn(vx, vy, vz, idx, a_start, w, nidx) is called for every segment, w
+// without the ray's constant factor r.ws, nidx the index of the walk's next
+// voxel when that is inside `B` (else idx again): a caller may fetch it one
+// segment ahead.  onEnter(idx) is called with the first voxel's index before
+// its segment.
 //
 // Each step takes the nearest next plane m = min(axn, ayn, azn); every axis
 // whose plane is at m advances (the reference's three `<=` tests: ties
@@ -557,11 +560,11 @@ __device__ __forceinline__ WalkBox imageBox(const DImage& P)
 // and, per axis, the crossings left before it leaves the box (the walk ends
 // when one goes negative); its coordinates are derived from those where fn
 // uses them.
-template <bool TOF, bool S64, class Fn>
+template <bool TOF, bool S64, class Fn, class Enter>
 __device__ __forceinline__ void sidWalk(const DImage& P, const SidRay& r,
                                         const DTof& tof, const WalkBox& B,
                                         bool first, int vx, int vy, int vz,
-                                        Fn&& fn)
+                                        Fn&& fn, Enter&& onEnter)
 {
 #pragma clang fp contract(off)
 	const float sdx = (float)r.dirx * P.vx, sdy = (float)r.diry * P.vy,
@@ -604,7 +607,7 @@ __device__ __forceinline__ void sidWalk(const DImage& P, const SidRay& r,
 		idx = (uint32_t)B.base + (uint32_t)vz * (uint32_t)B.sz + (uint32_t)vy * (uint32_t)B.sy +
 		      (uint32_t)vx * (uint32_t)B.sx;
 	};
-	auto emit = [&]() {
+	auto emit = [&](int nx, int ny, int nz, uint32_t ni) {
 		float w = a_next - a_cur;
 		if (TOF)
 		{
@@ -616,7 +619,7 @@ __device__ __forceinline__ void sidWalk(const DImage& P, const SidRay& r,
 		const int gx = (r.dirx > 0) ? B.x1 - 1 - cx : B.x0 + cx;
 		const int gy = (r.diry > 0) ? B.y1 - 1 - cy : B.y0 + cy;
 		const int gz = (r.dirz > 0) ? B.z1 - 1 - cz : B.z0 + cz;
-		fn(gx, gy, gz, idx, a_cur, w);
+		fn(gx, gy, gz, idx, a_cur, w, ((nx | ny | nz) >= 0) ? ni : idx);
 		a_cur = a_next;
 	};
 	// the voxel across the planes of the last step: crossings left and
@@ -642,29 +645,44 @@ __device__ __forceinline__ void sidWalk(const DImage& P, const SidRay& r,
 		if (vx < B.x0 || vx >= B.x1 || vy < B.y0 || vy >= B.y1 || vz < B.z0 || vz >= B.z1)
 			return;
 		enter();
+		onEnter(idx);
 		int nx, ny, nz;
 		uint32_t ni;
 		moved(nx, ny, nz, ni);
-		emit();
+		emit(nx, ny, nz, ni);
 		cx = nx;
 		cy = ny;
 		cz = nz;
 		idx = ni;
 	}
 	else
+	{
 		enter();
-	while (a_cur < amaxf && (cx | cy | cz) >= 0)
+		onEnter(idx);
+	}
+	// one exit test (the loop stays one block, so a fetch one segment ahead
+	// is not moved past it)
+	while ((a_cur < amaxf) & ((cx | cy | cz) >= 0))
 	{
 		step();
 		int nx, ny, nz;
 		uint32_t ni;
 		moved(nx, ny, nz, ni);
-		emit();
+		emit(nx, ny, nz, ni);
 		cx = nx;
 		cy = ny;
 		cz = nz;
 		idx = ni;
 	}
+}
+
+template <bool TOF, bool S64, class Fn>
+__device__ __forceinline__ void sidWalk(const DImage& P, const SidRay& r,
+                                        const DTof& tof, const WalkBox& B,
+                                        bool first, int vx, int vy, int vz,
+                                        Fn&& fn)
+{
+	sidWalk<TOF, S64>(P, r, tof, B, first, vx, vy, vz, fn, [](uint32_t) {});
 }
 
 // Siddon ray over the whole image.  `img` points at the frame's voxels.
@@ -682,18 +700,21 @@ __device__ __forceinline__ float siddonRay(const DImage& P,
 		return 0.f;
 	if constexpr (MODE == MODE_FWD)
 	{
-		// one FMA per segment in the walk's order (in locality order the
-		// gathers hit L2; keeping several in flight per lane measured slower)
-		float acc = 0.f;
-		sidWalk<TOF, S64>(P, r, tof, imageBox(P), true, 0, 0, 0,
-		                  [&](int, int, int, uint32_t idx, float, float w) {
-			                  acc = fmaf(w, img[idx], acc);
-		                  });
+		// one FMA per segment in the walk's order, the voxel's value
+		// fetched one segment ahead (its gather overlaps a whole step)
+		float acc = 0.f, xv = 0.f;
+		sidWalk<TOF, S64>(
+		    P, r, tof, imageBox(P), true, 0, 0, 0,
+		    [&](int, int, int, uint32_t, float, float w, uint32_t nidx) {
+			    acc = fmaf(w, xv, acc);
+			    xv = img[nidx];
+		    },
+		    [&](uint32_t idx) { xv = img[idx]; });
 		return acc * r.ws;
 	}
 	const float vs = val * r.ws;
 	sidWalk<TOF, S64>(P, r, tof, imageBox(P), true, 0, 0, 0,
-	                  [&](int, int, int, uint32_t idx, float, float w) {
+	                  [&](int, int, int, uint32_t idx, float, float w, uint32_t) {
 		                  if (MODE == MODE_BACK)
 			                  unsafeAtomicAdd(img + idx, vs * w);
 		                  else

@@ -129,7 +129,7 @@ __device__ __forceinline__ void binRay(const DImage& im, const DTof& tof, const 
 	float pa = 0.f;
 	uint32_t pv = 0;
 	sidWalk<TOF, S64>(im, s, tof, imageBox(im), true, 0, 0, 0,
-	                  [&](int vx, int vy, int vz, int, float a0, float) {
+	                  [&](int vx, int vy, int vz, uint32_t, float a0, float, uint32_t) {
 		                  const int tl = tileOf(T, frame, vx, vy, vz);
 		                  if (tl != last)
 		                  {
@@ -478,24 +478,24 @@ __device__ __forceinline__ void tileChunk(const KArgs& a, const DTiles& T,
 			          vz = B.z0 + (int)((pv >> 10) & 31u);
 			const int dx = (int)((pv >> 15) & 15u) - 8, dy = (int)((pv >> 19) & 15u) - 8,
 			          dz = (int)((pv >> 23) & 15u) - 8;
-			const bool first = (pv >> 31) != 0;
-			if (!first && (vx >= B.x1 || vy >= B.y1 || vz >= B.z1))
+			// FP32 set-up: every record, a ray's first included (its voxel is
+			// the walk's first, its alpha the ray's start), is re-entered from
+			// the ray's stored set-up, one instruction stream for the wave;
+			// FP64 set-up: the first record walks from the ray's entry
+			const bool entry = S64 && (pv >> 31) != 0;
+			if (!entry && (vx >= B.x1 || vy >= B.y1 || vz >= B.z1))
 				return;
 			SidRay s;
-			if (!S64 && !first)
-			{
-				// re-entered at the record: the ray's stored set-up
+			if (!S64)
 				sidLoad(q1, q2, q.e[2 % NE], q.e[3 % NE], s);
-			}
 			else if (!sidSetup<TOF, S64>(a.im, q1, q2, a.pr.tof, qb.z, s))
 				return;
-			if (!first)
+			if (!entry)
 				sidRestart<S64>(a.im, s, rc.al, vx, vy, vz, dx, dy, dz);
 			const float vs = val * s.ws;
-			sidWalk<TOF, S64>(a.im, s, a.pr.tof, B, first, vx, vy, vz,
-			                  [&](int gx, int gy, int gz, uint32_t idx, float, float w) {
-				                  add(idx, gx, gy, gz, vs * w);
-			                  });
+			sidWalk<TOF, S64>(a.im, s, a.pr.tof, B, entry, vx, vy, vz,
+			                  [&](int gx, int gy, int gz, uint32_t idx, float, float w,
+			                      uint32_t) { add(idx, gx, gy, gz, vs * w); });
 		}
 	};
 	auto pipeline = [&](auto&& add) {
@@ -648,7 +648,8 @@ __device__ __forceinline__ float emRatioOut(const KArgs& a, int64_t e, float s, 
 // Siddon: LPE lanes per LOR splitting its rays; the rays read from the plan's
 // table (TAB) or made from the resolved LOR.
 template <bool TOF, int LPE, bool S64, bool TAB>
-__global__ void __launch_bounds__(256) kEmRatio(KArgs a, float* __restrict__ v)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S64 ? 4 : 5, 8)))
+kEmRatio(KArgs a, float* __restrict__ v)
 {
 	const int64_t total = a.L.n * LPE;
 	const int64_t stride = (int64_t)gridDim.x * blockDim.x;
