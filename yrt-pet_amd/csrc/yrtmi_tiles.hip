@@ -227,6 +227,9 @@ __device__ __forceinline__ int lengthClass(const DPlan& pl, int segs)
 	return c;
 }
 
+// Each workgroup bins one contiguous range of rays (they are in locality
+// order), so a bin's records from one workgroup come from neighbouring rays
+// and the tile backprojection's reads of their ray-table entries coalesce.
 // PASS 0: each workgroup counts the records of its rays per bin (LDS),
 // keeps its counts in blockCnt[block][bin] and adds them to count[bin].
 // PASS 1 (same grid, same rays per workgroup): each workgroup reserves its
@@ -243,8 +246,10 @@ __global__ void __launch_bounds__(256) kBin(DImage im, DProj pr, int64_t nRays, 
 	const int nB = T.nTiles * pl.C;
 	int* cnt = sm;
 	int64_t* base = reinterpret_cast<int64_t*>(sm + ((nB + 1) & ~1));
-	const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-	const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	const int64_t per = (nRays + gridDim.x - 1) / gridDim.x;
+	const int64_t tEnd = min(nRays, (int64_t)(blockIdx.x + 1) * per);
+	const int64_t t0 = (int64_t)blockIdx.x * per + threadIdx.x;
+	const int64_t stride = blockDim.x;
 	unsigned int* myCnt = pl.blockCnt + (int64_t)blockIdx.x * nB;
 	auto binOne = [&](uint32_t t, auto&& onRecord) {
 		if constexpr (DD)
@@ -257,7 +262,7 @@ __global__ void __launch_bounds__(256) kBin(DImage im, DProj pr, int64_t nRays, 
 		for (int i = threadIdx.x; i < nB; i += blockDim.x)
 			cnt[i] = 0;
 		__syncthreads();
-		for (int64_t t = t0; t < nRays; t += stride)
+		for (int64_t t = t0; t < tEnd; t += stride)
 			binOne((uint32_t)t, [&](int tile, float, uint32_t, int segs) {
 				atomicAdd(&cnt[tile * pl.C + lengthClass(pl, segs)], 1);
 			});
@@ -278,7 +283,7 @@ __global__ void __launch_bounds__(256) kBin(DImage im, DProj pr, int64_t nRays, 
 		cnt[i] = c;
 	}
 	__syncthreads();
-	for (int64_t t = t0; t < nRays; t += stride)
+	for (int64_t t = t0; t < tEnd; t += stride)
 		binOne((uint32_t)t, [&](int tile, float a0, uint32_t pv, int segs) {
 			const int b = tile * pl.C + lengthClass(pl, segs);
 			const int left = atomicSub(&cnt[b], 1);
