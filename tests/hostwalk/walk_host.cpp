@@ -5,6 +5,7 @@
 // TOF and the projection-space PSF), whole and cut into tiles.
 #include "yrtmi_rays.hpp"
 
+#include <array>
 #include <map>
 #include <vector>
 
@@ -169,6 +170,44 @@ double hw_tiles(const float* imp, const float* p1, const float* p2, int tof, int
 	if (s64)
 		return tiles<false, true>(P, a, b, T, tofv, TX, TY, TZ, out);
 	return tiles<false, false>(P, a, b, T, tofv, TX, TY, TZ, out);
+}
+
+// Bricked forward (siddonRay<MODE_FWD, ., ., BRICK>) of one ray over
+// `brk` (the bricked copy of `img`) beside the x-major forward over `img`:
+// out[0] = bricked, out[1] = x-major; returns the segments whose bricked
+// index is not brickIndex of their voxel, or whose fetched-ahead index is
+// not the next voxel's (when that voxel is in the image).
+int hw_brick(const float* imp, const float* img, const float* brk, const float* p1,
+             const float* p2, int tof, const float* tofc, float tofv, float* out)
+{
+	const DImage P = image(imp);
+	const DTof T{tofc[0], tofc[1], tofc[2], rcpNearest(tofc[0])};
+	const float3 a = make_float3(p1[0], p1[1], p1[2]), b = make_float3(p2[0], p2[1], p2[2]);
+	uint32_t c = 0;
+	float* bi = const_cast<float*>(brk);
+	float* xi = const_cast<float*>(img);
+	out[0] = tof ? siddonRay<MODE_FWD, true, false, true>(P, bi, a, b, 0.f, T, tofv, c)
+	             : siddonRay<MODE_FWD, false, false, true>(P, bi, a, b, 0.f, T, tofv, c);
+	out[1] = tof ? siddonRay<MODE_FWD, true, false>(P, xi, a, b, 0.f, T, tofv, c)
+	             : siddonRay<MODE_FWD, false, false>(P, xi, a, b, 0.f, T, tofv, c);
+	SidRay r;
+	if (!(tof ? sidSetup<true, false>(P, a, b, T, tofv, r) : sidSetup<false, false>(P, a, b, T, tofv, r)))
+		return 0;
+	int bad = 0;
+	std::vector<std::array<int, 4>> seg;
+	auto fn = [&](int gx, int gy, int gz, uint32_t idx, float, float, uint32_t nidx) {
+		if (idx != brickIndex(P.nx, P.ny, gx, gy, gz))
+			bad++;
+		seg.push_back({gx, gy, gz, (int)nidx});
+	};
+	if (tof)
+		sidWalk<true, false, true>(P, r, T, imageBox(P), true, 0, 0, 0, fn, [](uint32_t) {});
+	else
+		sidWalk<false, false, true>(P, r, T, imageBox(P), true, 0, 0, 0, fn, [](uint32_t) {});
+	for (size_t i = 0; i + 1 < seg.size(); i++)
+		if ((uint32_t)seg[i][3] != brickIndex(P.nx, P.ny, seg[i + 1][0], seg[i + 1][1], seg[i + 1][2]))
+			bad++;
+	return bad;
 }
 
 // Distance-driven LOR (ddSetup + ddVisit): forward projection of `img`

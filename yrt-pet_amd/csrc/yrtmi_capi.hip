@@ -72,6 +72,9 @@ struct yrtmi_ctx
 	float* part = nullptr;
 	int64_t partStride = 0;
 	int nxcc = 1;
+	// bricked copy of the image the Siddon EM ratio projects, grown on demand
+	float* brick = nullptr;
+	int64_t brickCap = 0;
 	hipStream_t owned = nullptr;  // stream created by yrtmi_ctx_own_stream
 	bool capturing = false;       // between yrtmi_graph_begin and _end
 };
@@ -274,6 +277,42 @@ int ensurePartials(yrtmi_ctx* c, int64_t n)
 	return YRTMI_OK;
 }
 
+// The buffer of the bricked image the Siddon EM ratio's forward reads, at
+// least `need` floats (grown outside graph recording only; tile plans size
+// it for their image, so planned EM calls never allocate).
+int ensureBrick(yrtmi_ctx* c, int64_t need)
+{
+	if (c->brick == nullptr || c->brickCap < need)
+	{
+		int r = notCapturing(c, "bricked image (run one EM ratio of this size before recording)");
+		if (r)
+			return r;
+		HIP_TRY(hipStreamSynchronize(c->stream));
+		HIP_TRY(hipFree(c->brick));
+		c->brick = nullptr;
+		c->brickCap = 0;
+		HIP_TRY(hipMalloc(&c->brick, sizeof(float) * need));
+		c->brickCap = need;
+	}
+	return YRTMI_OK;
+}
+
+// The bricked copy of a.x for the Siddon EM ratio's forward; DD reads a.x
+// itself.
+int brickImage(yrtmi_ctx* c, KArgs& a)
+{
+	if (a.pr.type == 1)
+		return YRTMI_OK;
+	const int64_t bvox = brickFrameVoxels(a.im.nx, a.im.ny, a.im.nz);
+	const int r = ensureBrick(c, bvox * a.im.nt);
+	if (r)
+		return r;
+	HIP_TRY(launchToBricks(a.im, a.x, c->brick, c->stream));
+	a.xb = c->brick;
+	a.bvox = bvox;
+	return YRTMI_OK;
+}
+
 // Set up per-XCD accumulation for a backprojection of n voxels into y, or
 // leave the kernel writing y directly.
 int setupAccum(yrtmi_ctx* c, const yrtmi_proj* p, KArgs& a, int64_t n)
@@ -358,6 +397,7 @@ int yrtmi_ctx_destroy(yrtmi_ctx* c)
 	(void)hipFree(c->gatherErr);
 	(void)hipFree(c->counter);
 	(void)hipFree(c->part);
+	(void)hipFree(c->brick);
 	if (c->owned != nullptr)
 		(void)hipStreamDestroy(c->owned);
 	delete c;
@@ -1493,6 +1533,13 @@ int yrtmi_plan_create(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
 	const bool dd = p->d.type == YRTMI_PROJ_DD;
 	if (im->nx > 1024 || im->ny > 1024 || im->nz > 1024)
 		return fail(YRTMI_EINVAL, "plan: image dimensions above 1024");
+	if (!dd)
+	{
+		// the bricked image of the planned EM ratio, sized here
+		r = ensureBrick(c, brickFrameVoxels(a.im.nx, a.im.ny, a.im.nz) * a.im.nt);
+		if (r)
+			return r;
+	}
 	const int R = p->d.numRays;
 	if ((l->n * (int64_t)R) >= (int64_t(1) << 32))
 		return fail(YRTMI_EINVAL, "plan: n * num_rays must be below 2^32");
@@ -1811,6 +1858,9 @@ int yrtmi_em_ratio(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
 		return fail(YRTMI_EINVAL, "em_ratio: null image or output");
 	a.x = x;
 	setCorr(a, corr);
+	r = brickImage(c, a);
+	if (r)
+		return r;
 	HIP_TRY(launchEmRatio(a, p->lpe, v, c->stream));
 	return YRTMI_OK;
 }
@@ -1833,6 +1883,9 @@ int yrtmi_em_ratio_planned(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image*
 	a.x = x;
 	setCorr(a, corr);
 	a.rays = planRays(q);
+	r = brickImage(c, a);
+	if (r)
+		return r;
 	HIP_TRY(launchEmRatio(a, p->lpe, v, c->stream));
 	return YRTMI_OK;
 }
@@ -1856,6 +1909,9 @@ int yrtmi_em_subset_planned(yrtmi_ctx* c, const yrtmi_proj* p,
 	a.x = x;
 	setCorr(a, corr);
 	a.rays = planRays(q);
+	r = brickImage(c, a);
+	if (r)
+		return r;
 	HIP_TRY(launchEmRatio(a, p->lpe, q->v, c->stream));
 	HIP_TRY(launchTileBack(a, q->T, planRead(q), q->v, u, q->persistent, c->stream));
 	return YRTMI_OK;
@@ -2207,6 +2263,11 @@ int yrtmi_autotune(yrtmi_ctx* c, yrtmi_proj* p, const yrtmi_image* im,
 	a.x = x;
 	a.c.g = 1.0f;
 	a.c.denomThr = 1e-35f;
+	{
+		const int rb = brickImage(c, a);
+		if (rb)
+			return done(rb);
+	}
 	// time fn(): one untimed run, then the mean of two timed runs
 	auto timeIt = [&](auto&& fn, float& ms) -> hipError_t {
 		hipError_t e = fn();

@@ -216,6 +216,10 @@ struct KArgs
 	// end points after offsets, TOF value, frame), or null: the forward of
 	// the EM ratio then reads it instead of resolving each LOR.
 	const float4* rays;
+	// The Siddon EM ratio's forward reads the bricked copy of x (xb, bvox
+	// floats per frame, brickIndex), made by launchToBricks.
+	const float* xb;
+	int64_t bvox;
 };
 
 // Hardware id of the XCD (accelerator complex die) running this wave.

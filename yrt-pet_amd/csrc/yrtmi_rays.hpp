@@ -542,6 +542,26 @@ __device__ __forceinline__ WalkBox imageBox(const DImage& P)
 	return WalkBox{0, P.nx, 0, P.ny, 0, P.nz, 1, P.nx, P.nxy, 0};
 }
 
+// Bricked copy of an image frame for the forward's gathers: bricks of
+// 4 x 4 x 2 voxels (128 bytes, one cache line), x-major within a brick and
+// brick-x-major across bricks, the frame padded to whole bricks.  Rays a
+// voxel or two apart (the rays of one LOR) then mostly read the same line
+// in one load instruction, where rows of 32 voxels along x split them.
+__host__ __device__ constexpr int brickNx(int nx) { return (nx + 3) >> 2; }
+__host__ __device__ constexpr int brickNy(int ny) { return (ny + 3) >> 2; }
+__host__ __device__ constexpr int brickNz(int nz) { return (nz + 1) >> 1; }
+__host__ __device__ inline int64_t brickFrameVoxels(int nx, int ny, int nz)
+{
+	return (int64_t)brickNx(nx) * brickNy(ny) * brickNz(nz) * 32;
+}
+__device__ __forceinline__ uint32_t brickIndex(int nx, int ny, int vx, int vy, int vz)
+{
+	const uint32_t b = ((uint32_t)(vz >> 1) * (uint32_t)brickNy(ny) + (uint32_t)(vy >> 2)) *
+	                       (uint32_t)brickNx(nx) +
+	                   (uint32_t)(vx >> 2);
+	return b * 32u + (uint32_t)(((vz & 1) << 4) | ((vy & 3) << 2) | (vx & 3));
+}
+
 // FP32 walk (ProjectorSiddon.cpp:622-766) from r.a_cur to r.amax.  With
 // `first` the walk starts at the ray's entry and its first voxel is the one
 // holding the midpoint of the first nonempty segment; otherwise it starts in
@@ -560,7 +580,13 @@ __device__ __forceinline__ WalkBox imageBox(const DImage& P)
 // and, per axis, the crossings left before it leaves the box (the walk ends
 // when one goes negative); its coordinates are derived from those where fn
 // uses them.
-template <bool TOF, bool S64, class Fn, class Enter>
+//
+// BRICK: the index is the voxel's in the bricked frame (brickIndex; `B`
+// must be the image): a crossing moves it by the in-brick stride, or across
+// to the next brick when the voxel was the brick's last in the direction of
+// travel, which the crossings left tell (per axis, (off - c) mod the brick
+// size is 0 there, off = n along +, 0 along -).
+template <bool TOF, bool S64, bool BRICK = false, class Fn, class Enter>
 __device__ __forceinline__ void sidWalk(const DImage& P, const SidRay& r,
                                         const DTof& tof, const WalkBox& B,
                                         bool first, int vx, int vy, int vz,
@@ -598,14 +624,25 @@ __device__ __forceinline__ void sidWalk(const DImage& P, const SidRay& r,
 	// its index
 	int cx = 0, cy = 0, cz = 0;
 	uint32_t idx = 0u;
-	const uint32_t xs = (uint32_t)(r.dirx * B.sx), ys = (uint32_t)(r.diry * B.sy),
-	               zs = (uint32_t)(r.dirz * B.sz);
+	const uint32_t xs = (uint32_t)(r.dirx * (BRICK ? 1 : B.sx)),
+	               ys = (uint32_t)(r.diry * (BRICK ? 4 : B.sy)),
+	               zs = (uint32_t)(r.dirz * (BRICK ? 16 : B.sz));
+	// BRICK: the strides across a brick boundary and the offsets of the
+	// boundary test
+	const uint32_t bxs = (uint32_t)(r.dirx * 29),
+	               bys = (uint32_t)(r.diry * (brickNx(P.nx) * 32 - 12)),
+	               bzs = (uint32_t)(r.dirz * (brickNx(P.nx) * brickNy(P.ny) * 32 - 16));
+	const int ox = (r.dirx > 0) ? P.nx : 0, oy = (r.diry > 0) ? P.ny : 0,
+	          oz = (r.dirz > 0) ? P.nz : 0;
 	auto enter = [&]() {
 		cx = (r.dirx > 0) ? B.x1 - 1 - vx : vx - B.x0;
 		cy = (r.diry > 0) ? B.y1 - 1 - vy : vy - B.y0;
 		cz = (r.dirz > 0) ? B.z1 - 1 - vz : vz - B.z0;
-		idx = (uint32_t)B.base + (uint32_t)vz * (uint32_t)B.sz + (uint32_t)vy * (uint32_t)B.sy +
-		      (uint32_t)vx * (uint32_t)B.sx;
+		if constexpr (BRICK)
+			idx = brickIndex(P.nx, P.ny, vx, vy, vz);
+		else
+			idx = (uint32_t)B.base + (uint32_t)vz * (uint32_t)B.sz +
+			      (uint32_t)vy * (uint32_t)B.sy + (uint32_t)vx * (uint32_t)B.sx;
 	};
 	auto emit = [&](int nx, int ny, int nz, uint32_t ni) {
 		float w = a_next - a_cur;
@@ -628,7 +665,15 @@ __device__ __forceinline__ void sidWalk(const DImage& P, const SidRay& r,
 		nx = cx - (mx ? 1 : 0);
 		ny = cy - (my ? 1 : 0);
 		nz = cz - (mz ? 1 : 0);
-		ni = idx + (mx ? xs : 0u) + (my ? ys : 0u) + (mz ? zs : 0u);
+		if constexpr (BRICK)
+		{
+			const uint32_t sx = (((ox - cx) & 3) == 0) ? bxs : xs;
+			const uint32_t sy = (((oy - cy) & 3) == 0) ? bys : ys;
+			const uint32_t sz = (((oz - cz) & 1) == 0) ? bzs : zs;
+			ni = idx + (mx ? sx : 0u) + (my ? sy : 0u) + (mz ? sz : 0u);
+		}
+		else
+			ni = idx + (mx ? xs : 0u) + (my ? ys : 0u) + (mz ? zs : 0u);
 	};
 	if (first)
 	{
@@ -682,13 +727,14 @@ __device__ __forceinline__ void sidWalk(const DImage& P, const SidRay& r,
                                         bool first, int vx, int vy, int vz,
                                         Fn&& fn)
 {
-	sidWalk<TOF, S64>(P, r, tof, B, first, vx, vy, vz, fn, [](uint32_t) {});
+	sidWalk<TOF, S64, false>(P, r, tof, B, first, vx, vy, vz, fn, [](uint32_t) {});
 }
 
 // Siddon ray over the whole image.  `img` points at the frame's voxels.
 // MODE_FWD returns the line integral; MODE_BACK adds val*w atomically;
 // MODE_COUNT counts applied segments.
-template <int MODE, bool TOF, bool S64>
+// BRICK (forward only): `img` is the frame's bricked copy (brickIndex).
+template <int MODE, bool TOF, bool S64, bool BRICK = false>
 __device__ __forceinline__ float siddonRay(const DImage& P,
                                            float* __restrict__ img, float3 p1,
                                            float3 p2, float val,
@@ -703,7 +749,7 @@ __device__ __forceinline__ float siddonRay(const DImage& P,
 		// one FMA per segment in the walk's order, the voxel's value
 		// fetched one segment ahead (its gather overlaps a whole step)
 		float acc = 0.f, xv = 0.f;
-		sidWalk<TOF, S64>(
+		sidWalk<TOF, S64, BRICK>(
 		    P, r, tof, imageBox(P), true, 0, 0, 0,
 		    [&](int, int, int, uint32_t, float, float w, uint32_t nidx) {
 			    acc = fmaf(w, xv, acc);

@@ -685,7 +685,7 @@ kEmRatio(KArgs a, float* __restrict__ v)
 		float s = 0.f;
 		if (!skip)
 		{
-			const float* img = a.x + (int64_t)l.frame * a.im.nvox;
+			const float* img = a.xb + (int64_t)l.frame * a.bvox;
 			for (int r = lane; r < R; r += LPE)
 			{
 				float3 q1, q2;
@@ -706,8 +706,8 @@ kEmRatio(KArgs a, float* __restrict__ v)
 					q2 = make_float3(q2.x - a.im.ox, q2.y - a.im.oy, q2.z - a.im.oz);
 				}
 				uint32_t c = 0;
-				s += siddonRay<MODE_FWD, TOF, S64>(a.im, const_cast<float*>(img), q1, q2,
-				                                   0.f, a.pr.tof, l.tof, c);
+				s += siddonRay<MODE_FWD, TOF, S64, true>(a.im, const_cast<float*>(img), q1, q2,
+				                                         0.f, a.pr.tof, l.tof, c);
 			}
 		}
 #pragma unroll
@@ -1030,8 +1030,41 @@ static hipError_t ratioT(const KArgs& a, int lpe, dim3 g, float* v, hipStream_t 
 	                         : ratioLpe<TOF, S64, false>(a, lpe, g, v, st);
 }
 
+__global__ void __launch_bounds__(256)
+    kToBricks(const float* __restrict__ x, float* __restrict__ xb, int nx, int ny, int nz,
+              int64_t nvox, int64_t bvox, int64_t total)
+{
+	const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+	const int bnx = brickNx(nx), bny = brickNy(ny);
+	for (int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < total; o += stride)
+	{
+		const int64_t f = o / bvox;
+		const int64_t r = o - f * bvox;
+		const int64_t b = r >> 5;
+		const int w = (int)(r & 31);
+		const int64_t t = b / bnx;
+		const int vx = (int)(b - t * bnx) * 4 + (w & 3);
+		const int vy = (int)(t % bny) * 4 + ((w >> 2) & 3);
+		const int vz = (int)(t / bny) * 2 + (w >> 4);
+		xb[o] = (vx < nx && vy < ny && vz < nz)
+		            ? x[f * nvox + ((int64_t)vz * ny + vy) * nx + vx]
+		            : 0.f;
+	}
+}
+
+hipError_t launchToBricks(const DImage& im, const float* x, float* xb, hipStream_t st)
+{
+	const int64_t bvox = brickFrameVoxels(im.nx, im.ny, im.nz);
+	const int64_t total = bvox * im.nt;
+	const unsigned g = (unsigned)std::min<int64_t>(std::max<int64_t>((total + 255) / 256, 1), 65536);
+	kToBricks<<<g, 256, 0, st>>>(x, xb, im.nx, im.ny, im.nz, im.nvox, bvox, total);
+	return hipGetLastError();
+}
+
 hipError_t launchEmRatio(const KArgs& a, int lpe, float* v, hipStream_t st)
 {
+	if (a.pr.type != 1 && (a.xb == nullptr || a.bvox != brickFrameVoxels(a.im.nx, a.im.ny, a.im.nz)))
+		return hipErrorInvalidValue;
 	if (a.pr.type == 1)
 		lpe = 1;
 	const int64_t threads = a.L.n * lpe;

@@ -129,7 +129,11 @@ hipError_t launchBin(int pass, const KArgs& a, const DTiles& T, const DPlan& pl,
 // workgroup per chunk, or `persistent` workgroups on the chunk queue.
 hipError_t launchTileBack(const KArgs& a, const DTiles& T, const DPlanRead& pl,
                           const float* v, float* y, int persistent, hipStream_t st);
-// v[e] = EM ratio of LOR e (0 where skipped), Siddon or DD (a.pr.type).
+// xb (nt frames of brickFrameVoxels floats) = bricked copy of x (nt frames
+// of the image's voxels), padding voxels 0.
+hipError_t launchToBricks(const DImage& im, const float* x, float* xb, hipStream_t st);
+// v[e] = EM ratio of LOR e (0 where skipped), Siddon or DD (a.pr.type);
+// Siddon reads a.xb (launchToBricks of a.x).
 hipError_t launchEmRatio(const KArgs& a, int lpe, float* v, hipStream_t st);
 // v[e] = sensitivity value of LOR e (factor x attenuation).
 hipError_t launchSensValue(const KArgs& a, float* v, hipStream_t st);
