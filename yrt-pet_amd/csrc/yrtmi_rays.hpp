@@ -568,9 +568,9 @@ __device__ __forceinline__ uint32_t brickIndex(int nx, int ny, int vx, int vy, i
 // (vx, vy, vz) (after sidRestart).  It ends when the voxel leaves `B`.
 // fn(vx, vy, vz, idx, a_start, w, nidx) is called for every segment, w
 // without the ray's constant factor r.ws, nidx the index of the walk's next
-// voxel when that is inside `B` (else idx again): a caller may fetch it one
-// segment ahead.  onEnter(idx) is called with the first voxel's index before
-// its segment.
+// voxel when that is inside `B` (else idx again, whose line is cached): a
+// forward may fetch it one segment ahead.  onEnter(idx) is called with the
+// first voxel's index before its segment.
 //
 // Each step takes the nearest next plane m = min(axn, ayn, azn); every axis
 // whose plane is at m advances (the reference's three `<=` tests: ties
@@ -632,8 +632,10 @@ __device__ __forceinline__ void sidWalk(const DImage& P, const SidRay& r,
 	const uint32_t bxs = (uint32_t)(r.dirx * 29),
 	               bys = (uint32_t)(r.diry * (brickNx(P.nx) * 32 - 12)),
 	               bzs = (uint32_t)(r.dirz * (brickNx(P.nx) * brickNy(P.ny) * 32 - 16));
-	const int ox = (r.dirx > 0) ? P.nx : 0, oy = (r.diry > 0) ? P.ny : 0,
-	          oz = (r.dirz > 0) ? P.nz : 0;
+	// the crossing counter's phase at a brick's last voxel: (off - c) mod
+	// the brick size is 0 there, i.e. c mod the size equals off mod the size
+	const int ox = ((r.dirx > 0) ? P.nx : 0) & 3, oy = ((r.diry > 0) ? P.ny : 0) & 3,
+	          oz = ((r.dirz > 0) ? P.nz : 0) & 1;
 	auto enter = [&]() {
 		cx = (r.dirx > 0) ? B.x1 - 1 - vx : vx - B.x0;
 		cy = (r.diry > 0) ? B.y1 - 1 - vy : vy - B.y0;
@@ -662,18 +664,18 @@ __device__ __forceinline__ void sidWalk(const DImage& P, const SidRay& r,
 	// the voxel across the planes of the last step: crossings left and
 	// index after the move (computed beside the step's compares)
 	auto moved = [&](int& nx, int& ny, int& nz, uint32_t& ni) {
-		nx = cx - (mx ? 1 : 0);
-		ny = cy - (my ? 1 : 0);
-		nz = cz - (mz ? 1 : 0);
 		if constexpr (BRICK)
 		{
-			const uint32_t sx = (((ox - cx) & 3) == 0) ? bxs : xs;
-			const uint32_t sy = (((oy - cy) & 3) == 0) ? bys : ys;
-			const uint32_t sz = (((oz - cz) & 1) == 0) ? bzs : zs;
+			const uint32_t sx = ((cx & 3) == ox) ? bxs : xs;
+			const uint32_t sy = ((cy & 3) == oy) ? bys : ys;
+			const uint32_t sz = ((cz & 1) == oz) ? bzs : zs;
 			ni = idx + (mx ? sx : 0u) + (my ? sy : 0u) + (mz ? sz : 0u);
 		}
 		else
 			ni = idx + (mx ? xs : 0u) + (my ? ys : 0u) + (mz ? zs : 0u);
+		nx = cx - (mx ? 1 : 0);
+		ny = cy - (my ? 1 : 0);
+		nz = cz - (mz ? 1 : 0);
 	};
 	if (first)
 	{
