@@ -1049,8 +1049,10 @@ def bench_dd(args, torch, dev, stream):
                   f"rel L2 vs batched {rel['plain_vs_batched']:.2e}")
     parity = None if args.no_oracle_sample else dd_oracle_sample(
         args, torch, ctx, proj, sc, im1, pp, per[0][0][0], per[0][0][1], xb[0:1], dev)
+    # the CPU baseline samples the largest frame's first subset (~cpu_seconds)
+    fbig = max(range(F), key=lambda f: per[f][0][0].numel())
     cpu = None if args.no_cpu_baseline else dd_cpu_baseline(
-        args, torch, sc, im1, pp, per[0][0][0], per[0][0][1], xb[0:1])
+        args, torch, sc, im1, pp, per[fbig][0][0], per[fbig][0][1], xb[fbig:fbig + 1], fbig)
     # measured traffic applies to the profiled configuration only
     traffic, traffic_src = None, None
     if (args.dd_min == 1e6 and args.dd_max == 40e6 and S == 12 and args.order == "lor"):
@@ -1148,10 +1150,10 @@ def dd_oracle_sample(args, torch, ctx, proj, sc, im, pp, bins, meas, x, dev):
     return res
 
 
-def dd_cpu_baseline(args, torch, sc, im, pp, bins, meas, x):
+def dd_cpu_baseline(args, torch, sc, im, pp, bins, meas, x, frame=0):
     """The oracle's FP32 restatement of the reference's DD + PSF EM subset
     (std::thread, one per hardware thread up to 16) on an evenly strided
-    sample of frame 0's first subset, sized to ~cpu_seconds of work."""
+    sample of one frame's first subset, sized to ~cpu_seconds of work."""
     import oracle as O
     sys.path.insert(0, os.path.join(REPO, "tests"))
     from helpers import oracle_image, oracle_proj
@@ -1175,7 +1177,7 @@ def dd_cpu_baseline(args, torch, sc, im, pp, bins, meas, x):
     return {"value": 2.0 * k / dt, "unit": "event-projections/s", "cores": nthreads,
             "kind": "port",
             "sample": f"DD + PSF EM subset (forward+backprojection) of {k} nonzero bins evenly "
-                      f"strided over frame 0's first subset, oracle FP32 restatement of the "
+                      f"strided over frame {frame}'s first subset, oracle FP32 restatement of the "
                       f"reference CPU path, {nthreads} std::threads",
             "seconds": dt, **host_cpu()}
 
