@@ -85,7 +85,7 @@ HBM_PEAK_GBS = 8000.0
 # each file gives its commands.
 TRAFFIC_PROFILES = {
     "motion": os.path.join(REPO, "profiles", "em_motion_tof_250M_pmc_traffic.md"),
-    "listmode": os.path.join(REPO, "profiles", "em_listmode_tof_100M_pmc_traffic_sq.md"),
+    "listmode": os.path.join(REPO, "profiles", "em_listmode_tof_100M_pmc_traffic.md"),
     "dd": os.path.join(REPO, "profiles", "dd_psf_histo_16f_pmc.md"),
 }
 

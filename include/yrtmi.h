@@ -525,6 +525,10 @@ int yrtmi_em_subset_planned(yrtmi_ctx* ctx, const yrtmi_proj* p,
  * EM ratio of LOR i (m / corrected forward projection, times the
  * multiplicative corrections), 0 where the reference skips the LOR.
  * yrtmi_em_subset_planned = yrtmi_em_ratio_planned + yrtmi_back_planned(v).
+ * Siddon projects a bricked copy of x made at each call into a buffer the
+ * context keeps (4 x 4 x 2-voxel bricks; same values, same sums): it grows
+ * on the first call of a larger image, or when a plan is created, which a
+ * caller recording a graph must do before recording.
  * Replaces the forward half of OSEM_CPU::computeEMUpdateImage
  * (src/recon/OSEM_CPU.cpp:329-374). */
 int yrtmi_em_ratio(yrtmi_ctx* ctx, const yrtmi_proj* p, const yrtmi_image* im,

@@ -237,7 +237,7 @@ __device__ __forceinline__ int lengthClass(const DPlan& pl, int segs)
 // each record into its range (an LDS countdown per bin, so a record beyond
 // the reservation is caught without a global read).  DD records have no
 // alpha.
-// LDS: cnt[nB] ints, then (PASS 1) base[nB] 64-bit slots.
+// LDS: cnt[nB] ints, then (PASS 1) base[nB] 32-bit record indices.
 template <bool DD, bool TOF, bool S64, int PASS>
 __global__ void __launch_bounds__(256) kBin(DImage im, DProj pr, int64_t nRays, DTiles T,
                                             DPlan pl)
@@ -245,7 +245,7 @@ __global__ void __launch_bounds__(256) kBin(DImage im, DProj pr, int64_t nRays, 
 	extern __shared__ int sm[];
 	const int nB = T.nTiles * pl.C;
 	int* cnt = sm;
-	int64_t* base = reinterpret_cast<int64_t*>(sm + ((nB + 1) & ~1));
+	uint32_t* base = reinterpret_cast<uint32_t*>(sm + nB);
 	const int64_t per = (nRays + gridDim.x - 1) / gridDim.x;
 	const int64_t tEnd = min(nRays, (int64_t)(blockIdx.x + 1) * per);
 	const int64_t t0 = (int64_t)blockIdx.x * per + threadIdx.x;
@@ -278,8 +278,9 @@ __global__ void __launch_bounds__(256) kBin(DImage im, DProj pr, int64_t nRays, 
 	for (int i = threadIdx.x; i < nB; i += blockDim.x)
 	{
 		const int c = (int)myCnt[i];
-		base[i] = (c != 0) ? pl.tileOff[i] + (int64_t)atomicAdd(&pl.cursor[i], (unsigned int)c)
-		                   : 0;
+		base[i] = (c != 0) ? (uint32_t)(pl.tileOff[i] +
+		                                (int64_t)atomicAdd(&pl.cursor[i], (unsigned int)c))
+		                   : 0u;
 		cnt[i] = c;
 	}
 	__syncthreads();
@@ -289,7 +290,7 @@ __global__ void __launch_bounds__(256) kBin(DImage im, DProj pr, int64_t nRays, 
 			const int left = atomicSub(&cnt[b], 1);
 			if (left > 0)
 			{
-				const int64_t j = base[b] + left - 1;
+				const int64_t j = (int64_t)base[b] + left - 1;
 				pl.key[j] = (uint32_t)t;
 				if (!DD)
 					pl.alpha[j] = a0;

@@ -20,11 +20,12 @@ __host__ __device__ constexpr int tileThreads(int shape)
 {
 	return shape == 0 ? 256 : (shape == 2 ? 1024 : 512);
 }
-// Binning keeps a counter (count pass) and a counter plus a 64-bit base
-// (fill pass) per (tile, length class) in LDS; length classes are added
-// while tiles x classes <= kMaxBins / 2.
-constexpr int kMaxBins = 8192;
-constexpr int kMaxClasses = 8;
+// Binning keeps a counter (count pass) and a counter plus a 32-bit base
+// (fill pass; a plan holds fewer than 2^32 records) per (tile, length
+// class) in LDS; length classes are added while tiles x classes <=
+// kMaxBins / 2.
+constexpr int kMaxBins = 16384;
+constexpr int kMaxClasses = 16;
 
 // Entry record of a ray in a tile: voxel within the tile (5 bits per axis),
 // the ray's plane shift per axis (sidPlaneShift, 4 bits each, offset 8) and
@@ -115,10 +116,10 @@ struct DPlanRead
 	const int* vexp;
 };
 
-// LDS of the fill pass: an int counter and a 64-bit base per bin.
+// LDS of the fill pass: an int counter and a 32-bit base per bin.
 inline size_t binLdsBytes(int nB)
 {
-	return (size_t)((nB + 1) & ~1) * sizeof(int) + (size_t)nB * sizeof(int64_t);
+	return (size_t)nB * (sizeof(int) + sizeof(uint32_t));
 }
 // The plan's ray table for a.L (before the binning passes).
 hipError_t launchRayPrep(const KArgs& a, float4* tab, hipStream_t st);
