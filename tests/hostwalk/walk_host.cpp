@@ -234,18 +234,25 @@ float hw_dd(const float* imp, const float* img, const float* p1, const float* p2
 	if (psf != nullptr)
 	{
 		const int64_t nk = (int64_t)numS * K;
-		taps.resize(2 * nk);
+		std::vector<float> raw(2 * nk);
 		for (int64_t i = 0; i < numS; i++)
 			for (int64_t j = 0; j < K; j++)
 			{
-				taps[i * K + j] = psf[i * K + j];
-				taps[nk + i * K + (K - 1 - j)] = psf[i * K + j];
+				raw[i * K + j] = psf[i * K + j];
+				raw[nk + i * K + (K - 1 - j)] = psf[i * K + j];
 			}
 		cum.resize(2 * (size_t)numS * (K + 2));
-		psfPrefixSums(taps.data(), numS, K, kSpacing, cum.data());
-		psfPrefixSums(taps.data() + nk, numS, K, kSpacing, cum.data() + (size_t)numS * (K + 2));
+		psfPrefixSums(raw.data(), numS, K, kSpacing, cum.data());
+		psfPrefixSums(raw.data() + nk, numS, K, kSpacing, cum.data() + (size_t)numS * (K + 2));
+		// the device layout: rows of K + 4, the taps between two zeros each side
+		const int64_t KP = psfRowTaps(K), nkp = (int64_t)numS * KP;
+		taps.assign(2 * nkp, 0.f);
+		for (int64_t h = 0; h < 2; h++)
+			for (int64_t i = 0; i < numS; i++)
+				for (int64_t j = 0; j < K; j++)
+					taps[h * nkp + i * KP + 2 + j] = raw[h * nk + i * K + j];
 		pr.psf.k = taps.data();
-		pr.psf.kflip = taps.data() + nk;
+		pr.psf.kflip = taps.data() + nkp;
 		pr.psf.cum = cum.data();
 		pr.psf.cumFlip = cum.data() + (size_t)numS * (K + 2);
 		pr.psf.numS = numS;

@@ -93,7 +93,7 @@ struct yrtmi_proj
 	yrtmi_ctx* ctx = nullptr;
 	int device = 0;
 	DProj d{};
-	float* psf = nullptr;  // numS*K taps followed by the flipped copy
+	float* psf = nullptr;  // numS padded rows of taps, then the flipped copy's
 	double* psfCum = nullptr;  // numS*(K+2) trapezoid prefix sums, then the flipped copy's
 	int lpe = 1;
 	int block = 256;
@@ -781,9 +781,16 @@ int yrtmi_proj_create(yrtmi_ctx* c, const yrtmi_projector_desc* d,
 		psfPrefixSums(tab.data(), d->psf_num_s, K, d->psf_k_spacing, cum.data());
 		psfPrefixSums(tab.data() + nk, d->psf_num_s, K, d->psf_k_spacing,
 		              cum.data() + (size_t)d->psf_num_s * KC);
-		hipError_t e = hipMalloc(&p->psf, tab.size() * sizeof(float));
+		// the device rows: the taps with two zeros on either side
+		const int64_t KP = psfRowTaps((int)K), nkp = (int64_t)d->psf_num_s * KP;
+		std::vector<float> padded(2 * nkp, 0.f);
+		for (int64_t h = 0; h < 2; h++)
+			for (int64_t i = 0; i < d->psf_num_s; i++)
+				for (int64_t j = 0; j < K; j++)
+					padded[h * nkp + i * KP + 2 + j] = tab[h * nk + i * K + j];
+		hipError_t e = hipMalloc(&p->psf, padded.size() * sizeof(float));
 		if (e == hipSuccess)
-			e = hipMemcpy(p->psf, tab.data(), tab.size() * sizeof(float),
+			e = hipMemcpy(p->psf, padded.data(), padded.size() * sizeof(float),
 			              hipMemcpyHostToDevice);
 		if (e == hipSuccess)
 			e = hipMalloc(&p->psfCum, cum.size() * sizeof(double));
@@ -798,7 +805,7 @@ int yrtmi_proj_create(yrtmi_ctx* c, const yrtmi_projector_desc* d,
 			return hipFail(e, "yrtmi_proj_create (PSF upload)");
 		}
 		q.psf.k = p->psf;
-		q.psf.kflip = p->psf + nk;
+		q.psf.kflip = p->psf + nkp;
 		q.psf.cum = p->psfCum;
 		q.psf.cumFlip = p->psfCum + (size_t)d->psf_num_s * KC;
 		q.psf.rks = rcpNearest(d->psf_k_spacing);
@@ -808,6 +815,8 @@ int yrtmi_proj_create(yrtmi_ctx* c, const yrtmi_projector_desc* d,
 		q.psf.kSpacing = d->psf_k_spacing;
 		// getHalfWidth_mm (ProjectionPsfManager.cpp:52-56)
 		q.psf.halfWidth = (float)((d->psf_ksize + 1) / 2) * d->psf_k_spacing;
+		const int64_t lds = nkp * (int64_t)sizeof(float) + (int64_t)d->psf_num_s * KC * 8;
+		q.psf.ldsBytes = lds <= kPsfLdsMax ? (int32_t)lds : 0;
 	}
 	p->lpe = (q.numRays >= 4) ? 4 : (q.numRays >= 2 ? 2 : 1);
 	*out = p;

@@ -131,8 +131,11 @@ struct DTof64
 // Projection-space PSF tables (ProjectionPsfManager.cpp:27-103).
 struct DPsf
 {
-	const float* k;      // numS x K
-	const float* kflip;  // numS x K, reversed taps (backprojection)
+	// numS rows of K + 4 floats: the row's taps with two zeros on either
+	// side (entry i + 2 = tap i), so that psfWeight reads the taps of any
+	// clamped index without a test
+	const float* k;
+	const float* kflip;  // the same of the reversed taps (backprojection)
 	// numS x (K + 2) prefix sums of the kernel's trapezoids (FP64): row r,
 	// entry m + 1 = sum over xi in [-1, m) of kSpacing * (tap(xi) +
 	// tap(xi + 1)) / 2, taps outside [0, K) being 0; cumFlip for kflip
@@ -141,7 +144,16 @@ struct DPsf
 	int32_t numS, K;
 	float sStep, kSpacing, halfWidth;
 	float rks;  // RN(1 / kSpacing)
+	// bytes of one direction's tables (the prefix sums, then the padded
+	// taps) when the DD kernels stage them in LDS, else 0 (too large: they
+	// read them from global memory)
+	int32_t ldsBytes;
 };
+
+// Row stride of DPsf::k and DPsf::kflip.
+__host__ __device__ constexpr int psfRowTaps(int K) { return K + 4; }
+// LDS tables of at most this many bytes are staged.
+constexpr int kPsfLdsMax = 16384;
 
 struct DProj
 {
@@ -439,11 +451,6 @@ __device__ __forceinline__ float gaussW(float xc, float norm)
 // Projection-PSF integral (ProjectorUtils.hpp:43-98 via
 // ProjectionPsfManager::getWeight, ProjectionPsfManager.cpp:93-103).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float psfTap(const float* k, int K, int i)
-{
-	return (i < 0 || i >= K) ? 0.f : k[i];
-}
-
 // x / y from ry = RN(1/y): q = RN(x*ry) corrected by one FMA residual step
 // (P. Markstein, IBM J. Res. Dev. 34(1), 1990): RN(x/y) for finite x, y away
 // from the overflow and underflow thresholds.
@@ -458,28 +465,34 @@ __device__ __forceinline__ float divByRecip(float x, float y, float ry)
 // trapezoids at both ends as the reference computes them, the whole
 // trapezoids between them from the row's FP64 prefix table `cum` (the
 // reference adds them one by one in FP32; the two sums differ by rounding).
-__device__ float psfWeight(const DPsf& P, const float* k, const double* cum, float x0, float x1)
+// kp: the row's padded taps (kp[i + 2] = tap i, zero for i outside [0, K)).
+// Every index is clamped into the padded row, so the six table reads have
+// no tests and are issued together: a tap index below -2 or above K reads
+// zeros, as the unpadded taps' bounds tests give, and the prefix entries
+// read for an empty middle range are not used.
+__device__ __forceinline__ float psfWeight(const DPsf& P, const float* kp, const double* cum,
+                                           float x0, float x1)
 {
 #pragma clang fp contract(off)
 	const int hw = (P.K + 1) / 2;
 	const float ks = P.kSpacing;
+	// (K odd: hw = half + 1, the reference's second range test is the same)
 	if (x0 > hw * ks || x1 < -hw * ks || x0 >= x1)
 		return 0.f;
 	const int K = P.K;
 	const int half = (K - 1) / 2;
-	if (x0 > (half + 1) * ks || x1 < -(half + 1) * ks)
-		return 0.f;
 	const float x0s = divByRecip(x0, ks, P.rks);
 	const int x0i = (int)floorf(x0s + (float)half);
-	const float tau0 = x0s + (float)(-(-half + x0i));
-	const float v0lo = psfTap(k, K, x0i);
-	const float v0hi = (x0i < -1 || x0i >= K - 1) ? 0.f : k[x0i + 1];
-	const float v0 = v0lo * (1.f - tau0) + v0hi * tau0;
 	const float x1s = divByRecip(x1, ks, P.rks);
 	const int x1i = (int)floorf(x1s + (float)half);
+	const int j0 = min(max(x0i, -2), K) + 2, j1 = min(max(x1i, -2), K) + 2;
+	const int ia = max(x0i + 1, -1), ib = min(x1i, K);
+	const float v0lo = kp[j0], v0hi = kp[j0 + 1];
+	const float v1lo = kp[j1], v1hi = kp[j1 + 1];
+	const double ca = cum[min(ia, K) + 1], cb = cum[max(ib, -1) + 1];
+	const float tau0 = x0s + (float)(-(-half + x0i));
+	const float v0 = v0lo * (1.f - tau0) + v0hi * tau0;
 	const float tau1 = x1s - (float)(-half + x1i);
-	const float v1lo = psfTap(k, K, x1i);
-	const float v1hi = (x1i < -1 || x1i >= K - 1) ? 0.f : k[x1i + 1];
 	const float v1 = v1lo * (1.f - tau1) + v1hi * tau1;
 	float out = 0.f;
 	if (x0i == x1i)
@@ -492,9 +505,8 @@ __device__ float psfWeight(const DPsf& P, const float* k, const double* cum, flo
 		out += 0.5f * (v1lo + v1) * ks * tau1;
 		// whole trapezoids xi in [x0i + 1, x1i); those below xi = -1 and
 		// from xi = K on are zero (x0 may lie far below the kernel)
-		const int ia = max(x0i + 1, -1), ib = min(x1i, K);
 		if (ib > ia)
-			out += (float)(cum[ib + 1] - cum[ia + 1]);
+			out += (float)(cb - ca);
 	}
 	return out;
 }

@@ -782,7 +782,7 @@ struct DDRay
 {
 	bool usePsf;
 	int psfRow;         // row of the PSF table for this LOR
-	const float* pk;    // PSF taps (flipped for backprojection)
+	const float* pk;    // PSF taps, padded (flipped for backprojection)
 	const double* pcum; // the row's trapezoid prefix sums (flipped likewise)
 	float ext, d_norm, d1_i, d2_i;
 	// reciprocals of the slice-invariant denominators of ddSlice: the
@@ -853,7 +853,7 @@ __device__ __forceinline__ void ddSetup(const DImage& P, float3 d1, float3 d2, f
 				s = pr.psf.numS - 1;
 		}
 		psfRow = s;
-		pk = (flipPsf ? pr.psf.kflip : pr.psf.k) + s * pr.psf.K;
+		pk = (flipPsf ? pr.psf.kflip : pr.psf.k) + s * psfRowTaps(pr.psf.K);
 		pcum = (flipPsf ? pr.psf.cumFlip : pr.psf.cum) + s * (pr.psf.K + 2);
 		ext = pr.psf.halfWidth;
 	}
@@ -1123,7 +1123,8 @@ __device__ __forceinline__ void ddLoad(const DImage& P, const DProj& pr, const f
 	R.flag_y = ((rg >> 22) & 1u) != 0;
 	R.psfRow = __float_as_int(e[5].z);
 	R.usePsf = pr.psf.k != nullptr;
-	R.pk = R.usePsf ? (flipPsf ? pr.psf.kflip : pr.psf.k) + R.psfRow * pr.psf.K : nullptr;
+	R.pk = R.usePsf ? (flipPsf ? pr.psf.kflip : pr.psf.k) + R.psfRow * psfRowTaps(pr.psf.K)
+	                : nullptr;
 	R.pcum = R.usePsf ? (flipPsf ? pr.psf.cumFlip : pr.psf.cum) + R.psfRow * (pr.psf.K + 2)
 	                  : nullptr;
 	R.ext = R.usePsf ? pr.psf.halfWidth : 0.f;

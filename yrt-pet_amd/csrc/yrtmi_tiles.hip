@@ -335,11 +335,40 @@ __global__ void __launch_bounds__(256) kBin(DImage im, DProj pr, int64_t nRays, 
 // up (ddSetup) and its footprint swept over the slices, minor-axis voxels and
 // z planes of the tile (ddVisit), the voxel weights exactly those of the
 // direct sweep.  A voxel again takes one contribution per record.
+// The projector's PSF tables of one direction (flip: the backprojection's),
+// the prefix sums then the padded taps, copied into the workgroup's dynamic
+// LDS (DPsf::ldsBytes, given by the DD kernels' launches) by all its
+// threads; the caller waits at a barrier before reading them.  Returns the
+// LDS prefix sums, or null when the tables stay in global memory.
+__device__ __forceinline__ const double* psfStage(const DPsf& P, bool flip, double* l)
+{
+	if (P.ldsBytes == 0)
+		return nullptr;
+	const int nc = P.numS * (P.K + 2), nk = P.numS * psfRowTaps(P.K);
+	const double* c = flip ? P.cumFlip : P.cum;
+	const float* k = flip ? P.kflip : P.k;
+	for (int i = threadIdx.x; i < nc; i += blockDim.x)
+		l[i] = c[i];
+	float* lk = reinterpret_cast<float*>(l + nc);
+	for (int i = threadIdx.x; i < nk; i += blockDim.x)
+		lk[i] = k[i];
+	return l;
+}
+
+// A DD ray's PSF rows pointed at the staged tables.
+__device__ __forceinline__ void psfRowsLds(const DPsf& P, const double* lc, DDRay& D)
+{
+	const float* lk = reinterpret_cast<const float*>(lc + P.numS * (P.K + 2));
+	D.pcum = lc + D.psfRow * (P.K + 2);
+	D.pk = lk + D.psfRow * psfRowTaps(P.K);
+}
+
 template <bool DD, bool TOF, bool S64, int SHAPE>
 __device__ __forceinline__ void tileChunk(const KArgs& a, const DTiles& T,
                                           const DPlanRead& pl, const TileChunk ch,
                                           const float* __restrict__ v,
-                                          float* __restrict__ y, unsigned long long* acc64)
+                                          float* __restrict__ y, unsigned long long* acc64,
+                                          const double* psfL)
 {
 	float* acc = reinterpret_cast<float*>(acc64);
 	constexpr int TX = tileDim(SHAPE, 0), TY = tileDim(SHAPE, 1), TZ = tileDim(SHAPE, 2);
@@ -454,24 +483,6 @@ __device__ __forceinline__ void tileChunk(const KArgs& a, const DTiles& T,
 		float val = q.val;
 		if ((uint64_t)rc.key >= nRays || val == 0.f)
 			return;
-		if constexpr (DD)
-		{
-			if (ddFrame(q.e[5]) != f)
-				return;
-			DDRay D;
-			ddLoad(a.im, a.pr, q.e, true, D);
-			// the record's slices: [first, last] of the tile's slab
-			const int mb = D.flag_y ? B.y0 : B.x0;
-			const int xlo = mb + (int)(rc.pv & 63u), xhi = mb + (int)((rc.pv >> 6) & 63u);
-			const int ylo = D.flag_y ? B.x0 : B.y0, yhi = (D.flag_y ? B.x1 : B.y1) - 1;
-			ddVisit<TOF>(a.im, D, a.pr, xlo, min(xhi, (D.flag_y ? B.y1 : B.x1) - 1), ylo, yhi,
-			             B.z0, B.z1 - 1, [&](int, int xyi, int yxi, int zi, float w) {
-				             const int gx = D.flag_y ? yxi : xyi, gy = D.flag_y ? xyi : yxi;
-				             add((uint32_t)(B.base + zi * B.sz + gy * B.sy + gx * B.sx), gx, gy, zi,
-				                 val * w);
-			             });
-		}
-		else
 		{
 			const float4 qa = q.e[0], qb = q.e[1];
 			if (__float_as_int(qb.w) != f)
@@ -504,23 +515,102 @@ __device__ __forceinline__ void tileChunk(const KArgs& a, const DTiles& T,
 			                      uint32_t) { add(idx, gx, gy, gz, vs * w); });
 		}
 	};
-	auto pipeline = [&](auto&& add) {
-		int i = threadIdx.x;
-		Rec r0, r1;
-		Ray q0;
-		readRecord(i, r0);
-		readRay(r0.key, q0);
-		readRecord(i + NT, r1);
-		for (; i < ch.count; i += NT)
+	// DD: a wave's 64 records are swept one (record, slice) pair per lane.
+	// The pairs are numbered by the wave's prefix sum of the records' slice
+	// counts (last - first + 1 of the record's word), and the lane of pair s
+	// finds its record, the last whose first pair is at most s, by a binary
+	// search over the prefix sums; so the lanes differ in the footprint of
+	// their slice only, not in their record's number of slices.  Each pair
+	// reads its LOR's set-up (the pairs of one record read the same lines in
+	// one load).  The sums and searches run in the wave's uniform control
+	// flow (the trip counts are wave-uniform), with no workgroup barrier.
+	auto pipelineDD = [&](auto&& add) {
+		const int lane = (int)(threadIdx.x & 63u);
+		for (int i0 = (int)threadIdx.x - lane; i0 < ch.count; i0 += NT)
 		{
-			Ray q1;
-			readRay(r1.key, q1);
-			Rec r2;
-			readRecord(i + 2 * NT, r2);
-			walk(r0, q0, add);
-			r0 = r1;
-			q0 = q1;
-			r1 = r2;
+			Rec rc;
+			readRecord(i0 + lane, rc);
+			const int ns = (uint64_t)rc.key < nRays
+			                   ? (int)((rc.pv >> 6) & 63u) - (int)(rc.pv & 63u) + 1
+			                   : 0;
+			int incl = ns;
+#pragma unroll
+			for (int d = 1; d < 64; d <<= 1)
+			{
+				const int t = __shfl_up(incl, (unsigned int)d, 64);
+				if (lane >= d)
+					incl += t;
+			}
+			const int excl = incl - ns;
+			const int total = __shfl(incl, 63, 64);
+			for (int s0 = 0; s0 < total; s0 += 64)
+			{
+				const int s = s0 + lane;
+				int r = 0;
+#pragma unroll
+				for (int b = 32; b > 0; b >>= 1)
+					r += (__shfl(excl, r + b, 64) <= s) ? b : 0;
+				const uint32_t key = (uint32_t)__shfl((int)rc.key, r, 64);
+				const uint32_t pv = (uint32_t)__shfl((int)rc.pv, r, 64);
+				const int slot = s - __shfl(excl, r, 64);
+				if (s >= total)
+					continue;
+				const float val = v[key];
+				float4 e[6];
+#pragma unroll
+				for (int j = 0; j < 6; j++)
+					e[j] = pl.ray[6 * (uint64_t)key + j];
+				if (val == 0.f || ddFrame(e[5]) != f)
+					continue;
+				DDRay D;
+				ddLoad(a.im, a.pr, e, true, D);
+				// the pair's slice: first + slot of the tile's slab
+				const int xyi = (D.flag_y ? B.y0 : B.x0) + (int)(pv & 63u) + slot;
+				if (xyi >= (D.flag_y ? B.y1 : B.x1))
+					continue;
+				const int ylo = D.flag_y ? B.x0 : B.y0, yhi = (D.flag_y ? B.x1 : B.y1) - 1;
+				auto sweep = [&](const DDRay& Q) {
+					ddVisit<TOF>(a.im, Q, a.pr, xyi, xyi, ylo, yhi, B.z0, B.z1 - 1,
+					             [&](int, int xs, int yxi, int zi, float w) {
+						             const int gx = Q.flag_y ? yxi : xs,
+						                       gy = Q.flag_y ? xs : yxi;
+						             add((uint32_t)(B.base + zi * B.sz + gy * B.sy + gx * B.sx),
+						                 gx, gy, zi, val * w);
+					             });
+				};
+				// (two copies of the sweep: one reads the PSF rows from LDS)
+				if (psfL != nullptr)
+				{
+					psfRowsLds(a.pr.psf, psfL, D);
+					sweep(D);
+				}
+				else
+					sweep(D);
+			}
+		}
+	};
+	auto pipeline = [&](auto&& add) {
+		if constexpr (DD)
+			pipelineDD(add);
+		else
+		{
+			int i = threadIdx.x;
+			Rec r0, r1;
+			Ray q0;
+			readRecord(i, r0);
+			readRay(r0.key, q0);
+			readRecord(i + NT, r1);
+			for (; i < ch.count; i += NT)
+			{
+				Ray q1;
+				readRay(r1.key, q1);
+				Rec r2;
+				readRecord(i + 2 * NT, r2);
+				walk(r0, q0, add);
+				r0 = r1;
+				q0 = q1;
+				r1 = r2;
+			}
 		}
 	};
 	// c * 2^k rounded to an integer: one FP64 fma with the bias 1.5 * 2^52
@@ -576,9 +666,16 @@ __global__ void __launch_bounds__(tileThreads(SHAPE))
 	// NV 64-bit words (fixed point), or NV floats for the 32^3 shape
 	constexpr int NV = tileDim(SHAPE, 0) * tileDim(SHAPE, 1) * tileDim(SHAPE, 2);
 	__shared__ unsigned long long acc[SHAPE == 2 ? NV / 2 : NV];
+	extern __shared__ double psfLds[];
+	const double* psfL = nullptr;
+	if constexpr (DD)
+	{
+		psfL = psfStage(a.pr.psf, true, psfLds);
+		__syncthreads();
+	}
 	if (pl.queue == nullptr)
 	{
-		tileChunk<DD, TOF, S64, SHAPE>(a, T, pl, pl.chunks[blockIdx.x], v, y, acc);
+		tileChunk<DD, TOF, S64, SHAPE>(a, T, pl, pl.chunks[blockIdx.x], v, y, acc, psfL);
 		return;
 	}
 	// the next chunk's index is passed through the tile's first word (read
@@ -593,7 +690,7 @@ __global__ void __launch_bounds__(tileThreads(SHAPE))
 		__syncthreads();
 		if (c >= pl.nChunks)
 			break;
-		tileChunk<DD, TOF, S64, SHAPE>(a, T, pl, pl.chunks[c], v, y, acc);
+		tileChunk<DD, TOF, S64, SHAPE>(a, T, pl, pl.chunks[c], v, y, acc, psfL);
 	}
 	if (threadIdx.x == 0)
 	{
@@ -724,6 +821,9 @@ kEmRatio(KArgs a, float* __restrict__ v)
 template <bool TOF, bool S64, bool TAB>
 __global__ void __launch_bounds__(256) kEmRatioDD(KArgs a, float* __restrict__ v)
 {
+	extern __shared__ double psfLds[];
+	const double* psfL = psfStage(a.pr.psf, false, psfLds);
+	__syncthreads();
 	const int64_t stride = (int64_t)gridDim.x * blockDim.x;
 	for (int64_t e = xcdRange(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; e < a.L.n;
 	     e += stride)
@@ -762,8 +862,16 @@ __global__ void __launch_bounds__(256) kEmRatioDD(KArgs a, float* __restrict__ v
 				const float3 d2 = make_float3(l.p2.x - a.im.ox, l.p2.y - a.im.oy, l.p2.z - a.im.oz);
 				ddSetup<TOF, S64>(a.im, d1, d2, l.n1, l.n2, a.pr, l.tof, false, D);
 			}
-			const float s =
-			    ddSweepFwd<TOF>(a.im, a.x + (int64_t)frame * a.im.nvox, D, a.pr);
+			const float* img = a.x + (int64_t)frame * a.im.nvox;
+			// (two copies of the sweep: one reads the PSF rows from LDS)
+			float s;
+			if (psfL != nullptr)
+			{
+				psfRowsLds(a.pr.psf, psfL, D);
+				s = ddSweepFwd<TOF>(a.im, img, D, a.pr);
+			}
+			else
+				s = ddSweepFwd<TOF>(a.im, img, D, a.pr);
 			out = emRatioOut(a, e, s, m, sf, af);
 		}
 		v[e] = out;
@@ -867,7 +975,8 @@ template <bool DD, bool TOF, bool S64, int SHAPE>
 static void tileBackT(const KArgs& a, const DTiles& T, const DPlanRead& pl,
                       const float* v, float* y, unsigned grid, hipStream_t st)
 {
-	kTileBack<DD, TOF, S64, SHAPE><<<grid, tileThreads(SHAPE), 0, st>>>(a, T, pl, v, y);
+	kTileBack<DD, TOF, S64, SHAPE>
+	    <<<grid, tileThreads(SHAPE), DD ? a.pr.psf.ldsBytes : 0, st>>>(a, T, pl, v, y);
 }
 
 template <bool DD, bool TOF, bool S64>
@@ -1022,9 +1131,9 @@ static hipError_t ratioT(const KArgs& a, int lpe, dim3 g, float* v, hipStream_t 
 	if (a.pr.type == 1)
 	{
 		if (a.rays != nullptr)
-			kEmRatioDD<TOF, S64, true><<<g, 256, 0, st>>>(a, v);
+			kEmRatioDD<TOF, S64, true><<<g, 256, a.pr.psf.ldsBytes, st>>>(a, v);
 		else
-			kEmRatioDD<TOF, S64, false><<<g, 256, 0, st>>>(a, v);
+			kEmRatioDD<TOF, S64, false><<<g, 256, a.pr.psf.ldsBytes, st>>>(a, v);
 		return hipGetLastError();
 	}
 	return a.rays != nullptr ? ratioLpe<TOF, S64, true>(a, lpe, g, v, st)
