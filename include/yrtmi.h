@@ -430,7 +430,7 @@ int yrtmi_permute(yrtmi_ctx* ctx, const void* src, int elem_bytes,
  * parameter, TOF terms: 64 bytes per LOR and ray in all), so that the
  * backprojection reads one table entry per record instead of resolving the
  * LOR and redoing its set-up.
- * DD: a plan keeps each LOR's footprint set-up (96 bytes per LOR) and one
+ * DD: a plan keeps each LOR's footprint set-up (128 bytes per LOR) and one
  * record per tile the footprint reaches (8 bytes: the LOR and the slices of
  * the tile's slab that reach it, records of a tile ordered by their voxel
  * count); the backprojection sweeps each record's slices over the tile with

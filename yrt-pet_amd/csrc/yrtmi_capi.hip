@@ -1563,9 +1563,9 @@ int yrtmi_plan_create(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
 		return fail(YRTMI_EINVAL, "plan: max_bytes must be >= 0");
 	// the plan's memory within the budget: the value buffer and ray table
 	// now, the records after the count pass
-	// ray table: six float4 per DD LOR (its set-up), two per Siddon ray (its
+	// ray table: kDDVec float4 per DD LOR (its set-up), two per Siddon ray (its
 	// end points) and two more with the FP32 set-up (rayTabVec)
-	const int64_t tabVec = dd ? 6 : rayTabVec(p->d.setup64 != 0);
+	const int64_t tabVec = dd ? kDDVec : rayTabVec(p->d.setup64 != 0);
 	const int64_t tabBytes = tabVec * (int64_t)sizeof(float4) * l->n * (int64_t)R;
 	if (o.max_bytes > 0 && (int64_t)sizeof(float) * l->n + tabBytes > o.max_bytes)
 		return fail(YRTMI_ENOMEM, "plan: the ray table exceeds the memory budget");

@@ -1055,15 +1055,18 @@ __device__ __forceinline__ void ddSetup(const DImage& P, float3 d1, float3 d2, f
 	ddRecips(R);
 }
 
-// The per-LOR part of a DDRay in six float4 (a plan's DD ray table), and
-// back: the LOR's set-up is computed once (ddSetup) and the sweeps that use
-// it read it.  The image- and PSF-derived fields are recomputed from the
+// The per-LOR part of a DDRay in kDDVec float4 (a plan's DD ray table, 128
+// bytes: one cache line), and back: the LOR's set-up is computed once
+// (ddSetup) and the sweeps that use it read it, its five reciprocals
+// included.  The image- and PSF-derived fields are recomputed from the
 // image and projector exactly as ddSetup computes them, so a loaded DDRay
 // equals the one ddSetup makes for the same LOR.  frame -1: skipped LOR.
+constexpr int kDDVec = 8;
+
 __device__ __forceinline__ void ddStore(float4* __restrict__ tab, uint64_t t, const DDRay& R,
                                         int32_t frame)
 {
-	float4* o = tab + 6 * t;
+	float4* o = tab + kDDVec * t;
 	o[0] = make_float4(R.d1_xy_lo, R.d1_xy_hi, R.d2_xy_lo, R.d2_xy_hi);
 	o[1] = make_float4(R.d1_yx_lo, R.d1_yx_hi, R.d2_yx_lo, R.d2_yx_hi);
 	o[2] = make_float4(R.d1zloI, R.d1zloZ, R.d1zhiI, R.d1zhiZ);
@@ -1075,12 +1078,14 @@ __device__ __forceinline__ void ddStore(float4* __restrict__ tab, uint64_t t, co
 	const uint32_t lo = empty ? 1u : (uint32_t)R.xy_i_0, hi = empty ? 0u : (uint32_t)R.xy_i_1;
 	const uint32_t rg = lo | (hi << 11) | (R.flag_y ? 1u << 22 : 0u);
 	o[5] = make_float4(R.pc, __uint_as_float(rg), __int_as_float(R.psfRow), __int_as_float(frame));
+	o[6] = make_float4(R.r_xy_lo, R.r_xy_hi, R.r_z_lo, R.r_z_hi);
+	o[7] = make_float4(R.r_i, R.inv_dyx, __int_as_float(R.rOk ? 1 : 0), 0.f);
 }
 
 // An empty entry: frame -1 and an empty slice range.
 __device__ __forceinline__ void ddStoreSkip(float4* __restrict__ tab, uint64_t t)
 {
-	float4* o = tab + 6 * t;
+	float4* o = tab + kDDVec * t;
 	const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
 	o[0] = z;
 	o[1] = z;
@@ -1088,12 +1093,14 @@ __device__ __forceinline__ void ddStoreSkip(float4* __restrict__ tab, uint64_t t
 	o[3] = z;
 	o[4] = z;
 	o[5] = make_float4(0.f, __uint_as_float(1u), 0.f, __int_as_float(-1));
+	o[6] = z;
+	o[7] = z;
 }
 
 __device__ __forceinline__ int32_t ddFrame(const float4& e5) { return __float_as_int(e5.w); }
 
-__device__ __forceinline__ void ddLoad(const DImage& P, const DProj& pr, const float4 (&e)[6],
-                                       bool flipPsf, DDRay& R)
+__device__ __forceinline__ void ddLoad(const DImage& P, const DProj& pr,
+                                       const float4 (&e)[kDDVec], bool flipPsf, DDRay& R)
 {
 #pragma clang fp contract(off)
 	R.d1_xy_lo = e[0].x;
@@ -1137,12 +1144,17 @@ __device__ __forceinline__ void ddLoad(const DImage& P, const DProj& pr, const f
 	R.strideXY = fy ? P.nx : 1;
 	R.strideYX = fy ? 1 : P.nx;
 	R.yx_off = (float)(R.nyx - 1) / 2.f;
-	R.inv_dyx = 1.0f / R.dyx;
+	R.inv_dyx = e[7].y;
 	R.z_off = (float)(P.nz - 1) * 0.5f;
 	R.inv_dz = 1.0f / P.vz;
 	R.half_dyx = R.dyx * 0.5f;
 	R.hvz = P.vz * 0.5f;
-	ddRecips(R);
+	R.r_xy_lo = e[6].x;
+	R.r_xy_hi = e[6].y;
+	R.r_z_lo = e[6].z;
+	R.r_z_hi = e[6].w;
+	R.r_i = e[7].x;
+	R.rOk = __float_as_int(e[7].z) != 0;
 }
 
 // Footprint range of slice xyi: minor-axis voxels [yx0, yx1], z voxels
@@ -1235,15 +1247,18 @@ constexpr int DD_ZW = 4;
 
 // The footprint voxels of slices [xlo, xhi] of the major axis, minor-axis
 // index in [ylo, yhi] and z in [zlo, zhi] (each also clamped to the ray's own
-// range): fn(io, xyi, yxi, zi, w), io the voxel's index in the frame.  The
+// range): fn(io, xyi, yxi, zi, w, xv), io the voxel's index in the frame.  The
 // whole image (ranges covering it) is ProjectorDD's sweep.  The weight of
 // voxel (xyi, yxi, zi) is ProjectorDD's: (w_yx / width_yx) * (w_z / width_z)
 // * dxy / |cos|, times the TOF weight of the slice; the z factor depends on
 // the slice and zi only, so it is computed once per slice and z plane.
-template <bool TOF, class Fn>
-__device__ __forceinline__ void ddVisit(const DImage& P, const DDRay& R, const DProj& pr,
-                                        int xlo, int xhi, int ylo, int yhi, int zlo, int zhi,
-                                        Fn&& fn)
+// PRE: xv = img[io], the loads of a minor-axis voxel's z planes issued
+// before its weight is computed (the PSF integral then covers their
+// latency); else xv = 0.
+template <bool TOF, bool PRE, class Fn>
+__device__ __forceinline__ void ddVisitX(const DImage& P, const DDRay& R, const DProj& pr,
+                                         int xlo, int xhi, int ylo, int yhi, int zlo, int zhi,
+                                         const float* __restrict__ img, Fn&& fn)
 {
 #pragma clang fp contract(off)
 	const int x0 = max(R.xy_i_0, xlo), x1 = min(R.xy_i_1, xhi);
@@ -1283,11 +1298,16 @@ __device__ __forceinline__ void ddVisit(const DImage& P, const DDRay& R, const D
 			const float p1 = pix_yx + R.half_dyx;
 			if (!(q.r1 >= p0 && q.r0 < p1))
 				continue;
+			const int ib = xyi * R.strideXY + yxi * R.strideYX;
+			float xv[DD_ZW];
+#pragma unroll
+			for (int j = 0; j < DD_ZW; j++)
+				xv[j] = (PRE && nzv <= DD_ZW && ((zmask >> j) & 1u)) ? img[(zi0 + j) * P.nxy + ib]
+				                                                      : 0.f;
 			const float wxy =
 			    R.usePsf ? psfWeight(pr.psf, R.pk, R.pcum, p0 - q.r1ov, p1 - q.r0ov)
 			             : (fminf(p1, q.r1ov) - fmaxf(p0, q.r0ov));
 			const float wxy_s = ddDivW(wxy, q.widthFrac_yx, ry, oky);
-			const int ib = xyi * R.strideXY + yxi * R.strideYX;
 			if (nzv <= DD_ZW)
 			{
 #pragma unroll
@@ -1299,7 +1319,7 @@ __device__ __forceinline__ void ddVisit(const DImage& P, const DDRay& R, const D
 					float w = wxy_s * wz[j] * R.dxy_cos_theta;
 					if (TOF)
 						w *= q.tw;
-					fn(zi * P.nxy + ib, xyi, yxi, zi, w);
+					fn(zi * P.nxy + ib, xyi, yxi, zi, w, xv[j]);
 				}
 			}
 			else
@@ -1316,21 +1336,35 @@ __device__ __forceinline__ void ddVisit(const DImage& P, const DDRay& R, const D
 					float w = wxy_s * wz_s * R.dxy_cos_theta;
 					if (TOF)
 						w *= q.tw;
-					fn(zi * P.nxy + ib, xyi, yxi, zi, w);
+					const int io = zi * P.nxy + ib;
+					fn(io, xyi, yxi, zi, w, PRE ? img[io] : 0.f);
 				}
 			}
 		}
 	}
 }
 
-// Forward sweep of a set-up DD LOR over the whole image.
+// ddVisitX without the image loads: fn(io, xyi, yxi, zi, w).
+template <bool TOF, class Fn>
+__device__ __forceinline__ void ddVisit(const DImage& P, const DDRay& R, const DProj& pr,
+                                        int xlo, int xhi, int ylo, int yhi, int zlo, int zhi,
+                                        Fn&& fn)
+{
+	ddVisitX<TOF, false>(P, R, pr, xlo, xhi, ylo, yhi, zlo, zhi, nullptr,
+	                     [&](int io, int xyi, int yxi, int zi, float w, float) {
+		                     fn(io, xyi, yxi, zi, w);
+	                     });
+}
+
+// Forward sweep of a set-up DD LOR over the whole image: the sum of w * x
+// in the sweep's voxel order.
 template <bool TOF>
 __device__ __forceinline__ float ddSweepFwd(const DImage& P, const float* __restrict__ img,
                                             const DDRay& R, const DProj& pr)
 {
 	float acc = 0.f;
-	ddVisit<TOF>(P, R, pr, 0, 1 << 30, 0, 1 << 30, 0, 1 << 30,
-	             [&](int io, int, int, int, float w) { acc = fmaf(w, img[io], acc); });
+	ddVisitX<TOF, true>(P, R, pr, 0, 1 << 30, 0, 1 << 30, 0, 1 << 30, img,
+	                    [&](int, int, int, int, float w, float xv) { acc = fmaf(w, xv, acc); });
 	return acc;
 }
 
@@ -1341,17 +1375,16 @@ __device__ float ddRay(const DImage& P, float* __restrict__ img, float3 d1,
 {
 	DDRay R;
 	ddSetup<TOF, S64>(P, d1, d2, n1, n2, pr, tofv, !(MODE == MODE_FWD || MODE == MODE_COUNT), R);
-	float acc = 0.f;
+	if constexpr (MODE == MODE_FWD)
+		return ddSweepFwd<TOF>(P, img, R, pr);
 	ddVisit<TOF>(P, R, pr, 0, 1 << 30, 0, 1 << 30, 0, 1 << 30,
 	             [&](int io, int, int, int, float w) {
-		             if (MODE == MODE_FWD)
-			             acc = fmaf(w, img[io], acc);
-		             else if (MODE == MODE_BACK)
+		             if (MODE == MODE_BACK)
 			             unsafeAtomicAdd(img + io, val * w);
 		             else
 			             cnt++;
 	             });
-	return acc;
+	return 0.f;
 }
 
 }  // namespace yrtmi

@@ -159,8 +159,11 @@ __device__ __forceinline__ void binRayDD(const DImage& im, const DProj& pr, cons
                                          const float4* __restrict__ tab, uint32_t t,
                                          Fn&& onRecord)
 {
-	const float4* te = tab + 6 * (uint64_t)t;
-	const float4 e[6] = {te[0], te[1], te[2], te[3], te[4], te[5]};
+	const float4* te = tab + kDDVec * (uint64_t)t;
+	float4 e[kDDVec];
+#pragma unroll
+	for (int i = 0; i < kDDVec; i++)
+		e[i] = te[i];
 	const int frame = ddFrame(e[5]);
 	if (frame < 0)
 		return;
@@ -446,8 +449,8 @@ __device__ __forceinline__ void tileChunk(const KArgs& a, const DTiles& T,
 		float al;
 	};
 	// Siddon: the ray's end points, TOF value and frame (two float4), with
-	// the FP32 set-up two more (sidStore); DD: six (ddStore)
-	constexpr int NE = DD ? 6 : rayTabVec(S64);
+	// the FP32 set-up two more (sidStore); DD: kDDVec (ddStore)
+	constexpr int NE = DD ? kDDVec : rayTabVec(S64);
 	struct Ray
 	{
 		float4 e[NE];
@@ -556,10 +559,10 @@ __device__ __forceinline__ void tileChunk(const KArgs& a, const DTiles& T,
 				if (s >= total)
 					continue;
 				const float val = v[key];
-				float4 e[6];
+				float4 e[kDDVec];
 #pragma unroll
-				for (int j = 0; j < 6; j++)
-					e[j] = pl.ray[6 * (uint64_t)key + j];
+				for (int j = 0; j < kDDVec; j++)
+					e[j] = pl.ray[kDDVec * (uint64_t)key + j];
 				if (val == 0.f || ddFrame(e[5]) != f)
 					continue;
 				DDRay D;
@@ -833,10 +836,10 @@ __global__ void __launch_bounds__(256) kEmRatioDD(KArgs a, float* __restrict__ v
 		const float af = a.c.att ? a.c.att[e] : 1.0f;
 		int frame;
 		Lor l;
-		float4 de[6];
+		float4 de[kDDVec];
 		if (TAB)
 		{
-			de[5] = a.rays[6 * e + 5];
+			de[5] = a.rays[kDDVec * e + 5];
 			frame = ddFrame(de[5]);
 		}
 		else
@@ -852,8 +855,9 @@ __global__ void __launch_bounds__(256) kEmRatioDD(KArgs a, float* __restrict__ v
 			if (TAB)
 			{
 #pragma unroll
-				for (int i = 0; i < 5; i++)
-					de[i] = a.rays[6 * e + i];
+				for (int i = 0; i < kDDVec; i++)
+					if (i != 5)
+						de[i] = a.rays[kDDVec * e + i];
 				ddLoad(a.im, a.pr, de, false, D);
 			}
 			else
