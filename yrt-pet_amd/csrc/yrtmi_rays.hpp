@@ -601,18 +601,43 @@ __device__ __forceinline__ void sidWalk(const DImage& P, const SidRay& r,
 	float a_cur = r.a_cur, a_next = 0.f;
 	// planes crossed by the last step (ties advance together)
 	bool mx = false, my = false, mz = false;
+	// FP32 set-up: an axis's next crossing is (plane - origin) * factor of
+	// its plane alone (the set-up computes it so, sidRestart too where it
+	// does not clamp a plane to the restart's alpha, and a clamped plane's
+	// axis moves in the first step, its crossing being the least), so every
+	// axis's is recomputed each step, the planes that did not move giving
+	// the same value.  A flat axis (next crossing FMAX, factor 0) takes the
+	// origin -FMAX and the factor 1: plane + FMAX rounds to FMAX.
+	const float FMAX = 3.402823466e+38f;
+	const float oxr = (r.p1.x == r.p2.x) ? -FMAX : r.p1.x,
+	            oyr = (r.p1.y == r.p2.y) ? -FMAX : r.p1.y,
+	            ozr = (r.p1.z == r.p2.z) ? -FMAX : r.p1.z;
+	const float fxr = (r.p1.x == r.p2.x) ? 1.f : r.invx, fyr = (r.p1.y == r.p2.y) ? 1.f : r.invy,
+	            fzr = (r.p1.z == r.p2.z) ? 1.f : r.invz;
 	auto step = [&]() {
 		const float m = fminf(axn, fminf(ayn, azn));
 		mx = axn == m;
 		my = ayn == m;
 		mz = azn == m;
-		const float xn = xc + sdx, yn = yc + sdy, zn = zc + sdz;
-		xc = mx ? xn : xc;
-		yc = my ? yn : yc;
-		zc = mz ? zn : zc;
-		axn = mx ? (xn - r.p1.x) * r.invx : axn;
-		ayn = my ? (yn - r.p1.y) * r.invy : ayn;
-		azn = mz ? (zn - r.p1.z) * r.invz : azn;
+		if constexpr (!S64)
+		{
+			xc = mx ? xc + sdx : xc;
+			yc = my ? yc + sdy : yc;
+			zc = mz ? zc + sdz : zc;
+			axn = (xc - oxr) * fxr;
+			ayn = (yc - oyr) * fyr;
+			azn = (zc - ozr) * fzr;
+		}
+		else
+		{
+			const float xn = xc + sdx, yn = yc + sdy, zn = zc + sdz;
+			xc = mx ? xn : xc;
+			yc = my ? yn : yc;
+			zc = mz ? zn : zc;
+			axn = mx ? (xn - r.p1.x) * r.invx : axn;
+			ayn = my ? (yn - r.p1.y) * r.invy : ayn;
+			azn = mz ? (zn - r.p1.z) * r.invz : azn;
+		}
 		// min(m, amax) as an integer min of the bit patterns: equal to the
 		// float min here (amax > 0, m no NaN: a negative m has the smaller
 		// pattern as a signed integer, and non-negative floats order as
@@ -707,8 +732,9 @@ __device__ __forceinline__ void sidWalk(const DImage& P, const SidRay& r,
 		enter();
 		onEnter(idx);
 	}
-	// one exit test (the loop stays one block, so a fetch one segment ahead
-	// is not moved past it)
+	// one exit test per segment; unrolled twice, so the loop-carried index
+	// and alpha need no register copies
+#pragma unroll 2
 	while ((a_cur < amaxf) & ((cx | cy | cz) >= 0))
 	{
 		step();
