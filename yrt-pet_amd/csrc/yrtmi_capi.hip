@@ -1789,6 +1789,12 @@ int yrtmi_plan_create(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
 			same = (unsigned long long)cur[b] == cnt[b];
 		if (!same)
 			return cleanup(fail(YRTMI_ESTATE, "plan: fill pass disagrees with count pass"));
+		// each chunk's records by their fine length class
+		e = launchChunkSort(dd, pl, q->chunks, q->nChunks, c->stream);
+		if (e == hipSuccess)
+			e = hipStreamSynchronize(c->stream);
+		if (e != hipSuccess)
+			return cleanup(hipFail(e, "plan: chunk sort"));
 	}
 	return cleanup(YRTMI_OK);
 }

@@ -297,11 +297,124 @@ __global__ void __launch_bounds__(256) kBin(DImage im, DProj pr, int64_t nRays, 
 				pl.key[j] = (uint32_t)t;
 				if (!DD)
 					pl.alpha[j] = a0;
-				pl.vox[j] = pv;
+				pl.vox[j] = withFineClass(DD, pv, segs, T.tx + T.ty + T.tz);
 			}
 			else
 				atomicOr(pl.err, 1);
 		});
+}
+
+// Records of every chunk ordered by their fine length class (withFineClass),
+// once when the plan is built: the tile backprojection's waves then take
+// records of about the same length, where the plan's bins can hold only a
+// few classes per tile.  Each window of kSortWin records of a chunk is
+// sorted in place, stably by (class, position): one workgroup per chunk,
+// each lane holding its records in registers between the reads and the
+// writes; a record's rank in its class within a group of 64 is counted
+// with wave ballots.  The tile sums do not depend on the records' order
+// (fixed point), so the backprojection is unchanged.
+constexpr int kSortWin = 2048;
+
+template <bool DD>
+__global__ void __launch_bounds__(256) kChunkSort(DPlan pl, const TileChunk* __restrict__ chunks)
+{
+	constexpr int G = kSortWin / 64, GPW = G / 4;  // groups of 64, per wave
+	__shared__ int cntG[G][16];
+	__shared__ int offG[G][16];
+	__shared__ int base[16];
+	const TileChunk ch = chunks[blockIdx.x];
+	const int lane = (int)(threadIdx.x & 63u), wv = (int)(threadIdx.x >> 6);
+	const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+	for (int w0 = 0; w0 < ch.count; w0 += kSortWin)
+	{
+		const int n = min(kSortWin, ch.count - w0);
+		const int64_t j0 = ch.begin + w0;
+		uint32_t k[GPW], v[GPW];
+		float al[GPW];
+		int cls[GPW];
+#pragma unroll
+		for (int i = 0; i < GPW; i++)
+		{
+			const int g = wv + 4 * i, r = g * 64 + lane;
+			k[i] = 0u;
+			v[i] = 0u;
+			al[i] = 0.f;
+			cls[i] = 16;  // no record
+			if (r < n)
+			{
+				k[i] = pl.key[j0 + r];
+				v[i] = pl.vox[j0 + r];
+				if (!DD)
+					al[i] = pl.alpha[j0 + r];
+				cls[i] = fineClassOf(DD, v[i]);
+			}
+			for (int c = 0; c < 16; c++)
+			{
+				const int m = __popcll(__ballot(cls[i] == c));
+				if (lane == 0)
+					cntG[g][c] = m;
+			}
+		}
+		__syncthreads();
+		if (threadIdx.x < 16)
+		{
+			const int c = (int)threadIdx.x;
+			int sum = 0;
+			for (int g = 0; g < G; g++)
+			{
+				offG[g][c] = sum;
+				sum += cntG[g][c];
+			}
+			base[c] = sum;
+		}
+		__syncthreads();
+		if (threadIdx.x == 0)
+		{
+			int sum = 0;
+			for (int c = 0; c < 16; c++)
+			{
+				const int t = base[c];
+				base[c] = sum;
+				sum += t;
+			}
+		}
+		__syncthreads();
+#pragma unroll
+		for (int i = 0; i < GPW; i++)
+		{
+			const int g = wv + 4 * i;
+			uint64_t mine = 0ull;
+			for (int c = 0; c < 16; c++)
+			{
+				const uint64_t m = __ballot(cls[i] == c);
+				if (cls[i] == c)
+					mine = m;
+			}
+			if (cls[i] < 16)
+			{
+				const int64_t j = j0 + base[cls[i]] + offG[g][cls[i]] + __popcll(mine & below);
+				pl.key[j] = k[i];
+				pl.vox[j] = v[i];
+				if (!DD)
+					pl.alpha[j] = al[i];
+			}
+		}
+		__syncthreads();
+	}
+}
+
+hipError_t launchChunkSort(bool dd, const DPlan& pl, const TileChunk* chunks, int64_t nChunks,
+                           hipStream_t st)
+{
+	for (int64_t c0 = 0; c0 < nChunks; c0 += 65535)
+	{
+		const unsigned g = (unsigned)std::min<int64_t>(65535, nChunks - c0);
+		if (dd)
+			kChunkSort<true><<<g, 256, 0, st>>>(pl, chunks + c0);
+		else
+			kChunkSort<false><<<g, 256, 0, st>>>(pl, chunks + c0);
+	}
+	return hipGetLastError();
 }
 
 // One chunk: accumulate its ray pieces in the LDS tile `acc`, then add the

@@ -4,6 +4,8 @@
 
 #include "yrtmi_rays.hpp"
 
+#include <algorithm>
+
 namespace yrtmi
 {
 
@@ -37,6 +39,28 @@ __host__ __device__ inline uint32_t packEntry(int lx, int ly, int lz, int dx, in
 	return (uint32_t)lx | ((uint32_t)ly << 5) | ((uint32_t)lz << 10) |
 	       ((uint32_t)(dx + 8) << 15) | ((uint32_t)(dy + 8) << 19) |
 	       ((uint32_t)(dz + 8) << 23) | (first ? 0x80000000u : 0u);
+}
+
+// A record's fine length class (0 .. 15), kept in entry-word bits that
+// the record's fields leave free (Siddon: 27..30, packEntry; DD: 12..15,
+// above the slice range) for the plan's per-chunk sort (kChunkSort).
+// Siddon: its segments in the tile in 16 equal steps of the most a tile
+// piece can hold (tx + ty + tz); DD: its footprint voxels per slice.
+__host__ __device__ inline uint32_t withFineClass(bool dd, uint32_t pv, int segs, int L)
+{
+	if (dd)
+	{
+		const int ns = (int)((pv >> 6) & 63u) - (int)(pv & 63u) + 1;
+		const int c = std::min(15, std::max(0, (segs + ns - 1) / std::max(ns, 1)));
+		return pv | ((uint32_t)c << 12);
+	}
+	const int c = std::min(15, std::max(0, segs * 16 / std::max(L, 1)));
+	return pv | ((uint32_t)c << 27);
+}
+
+__host__ __device__ inline int fineClassOf(bool dd, uint32_t pv)
+{
+	return (int)((pv >> (dd ? 12 : 27)) & 15u);
 }
 
 // Tiles of an image with nt frames: ntx * nty * ntz per frame, numbered
@@ -128,6 +152,8 @@ hipError_t launchBin(int pass, const KArgs& a, const DTiles& T, const DPlan& pl,
                      int grid, hipStream_t st);
 // y += backprojection of v (per LOR) through the plan's records: one
 // workgroup per chunk, or `persistent` workgroups on the chunk queue.
+hipError_t launchChunkSort(bool dd, const DPlan& pl, const TileChunk* chunks, int64_t nChunks,
+                           hipStream_t st);
 hipError_t launchTileBack(const KArgs& a, const DTiles& T, const DPlanRead& pl,
                           const float* v, float* y, int persistent, hipStream_t st);
 // xb (nt frames of brickFrameVoxels floats) = bricked copy of x (nt frames
