@@ -298,7 +298,8 @@ def bench_listmode(args, torch, dev, stream):
     for s in range(SS):
         Ls = yrtmi.LorSet.histo_subset(ctx, SS, s)
         if tiled:
-            pl = ctx.plan(psens, im, Ls, args.chunk, tile_shape=yrtmi.capi.TILE_32x32x8)
+            pl = ctx.plan(psens, im, Ls, args.chunk, tile_shape=yrtmi.capi.TILE_32x32x8,
+                          single_use=True)
             ctx.sensitivity(psens, im, sens, Ls, plan=pl)
             del pl
         else:
@@ -704,7 +705,7 @@ def oracle_sample(args, torch, ctx, proj, sc, im, pp, d1, d2, tof, mot, xf, boun
     ctx.forward(proj, im, x, L, y)
     v = torch.empty(k, device=dev)
     ug = torch.zeros(im.shape, device=dev)
-    pl = ctx.plan(proj, im, L, tile_shape=shape)
+    pl = ctx.plan(proj, im, L, tile_shape=shape, single_use=True)
     ctx.em_ratio(proj, im, x, L, v)
     ctx.back(proj, im, ug, L, v, plan=pl)
     ctx.synchronize()

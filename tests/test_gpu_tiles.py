@@ -260,7 +260,8 @@ def test_sensitivity_planned_histogram(gpu, att):
         act = synth.cylinder_spheres(im, 40.0, 30.0, 1.0, [], supersample=1)
         mu = np.where(act > 0, np.float32(0.096), np.float32(0.0)).astype(np.float32)
     mug = torch.from_numpy(mu).to(gpu) if att else None
-    plan = ctx.plan(p, im, L)
+    # a single-use plan (records not ordered by length): the same image
+    plan = ctx.plan(p, im, L, single_use=True)
     a = torch.zeros(im.shape, device=gpu)
     b = torch.zeros(im.shape, device=gpu)
     kw = dict(att_im=im if att else None, att_img=mug)

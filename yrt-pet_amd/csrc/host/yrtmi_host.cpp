@@ -591,13 +591,15 @@ int defaultTileShape(const OperatorProjectorMI& proj, const yrtmi_image& im)
 
 // A tile plan of `l`; with maxBytes > 0, null when the plan would hold more
 // device memory than that (or the allocation fails): the caller then uses
-// the unplanned kernels.
+// the unplanned kernels.  singleUse: a plan backprojected once (its records
+// are not ordered by length).
 std::unique_ptr<PlanHandle> makePlan(const Context& ctx, const OperatorProjectorMI& proj,
                                      const yrtmi_image& im, const yrtmi_lors& l,
-                                     int shape = -1, int persistent = 0, int64_t maxBytes = 0)
+                                     int shape = -1, int persistent = 0, int64_t maxBytes = 0,
+                                     bool singleUse = false)
 {
 	const yrtmi_plan_options o{0, shape >= 0 ? shape : defaultTileShape(proj, im), persistent,
-	                           maxBytes};
+	                           maxBytes, singleUse ? 1 : 0};
 	yrtmi_plan* q = nullptr;
 	const int rc = yrtmi_plan_create(ctx.get(), proj.get(), &im, &l, &o, &q);
 	if (rc == YRTMI_ENOMEM && maxBytes > 0)
@@ -953,7 +955,7 @@ const std::vector<std::unique_ptr<ImageDevice>>& OSEM_MI::generateSensitivityIma
 		}
 		if (planned())
 		{
-			auto plan = makePlan(*m_ctx, *proj, ci, l);
+			auto plan = makePlan(*m_ctx, *proj, ci, l, -1, 0, 0, true);
 			check(yrtmi_sensitivity_planned(m_ctx->get(), proj->get(), &ci,
 			                                img->getDevicePointer(), &l,
 			                                hw ? &ca : nullptr,

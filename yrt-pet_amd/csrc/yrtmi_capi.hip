@@ -1552,7 +1552,7 @@ int yrtmi_plan_create(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
 	const int R = p->d.numRays;
 	if ((l->n * (int64_t)R) >= (int64_t(1) << 32))
 		return fail(YRTMI_EINVAL, "plan: n * num_rays must be below 2^32");
-	const yrtmi_plan_options o = opt ? *opt : yrtmi_plan_options{0, 0, 0, 0};
+	const yrtmi_plan_options o = opt ? *opt : yrtmi_plan_options{0, 0, 0, 0, 0};
 	if (o.chunk_records < 0 || o.chunk_records > (1 << 24))
 		return fail(YRTMI_EINVAL, "plan: chunk_records out of range");
 	if (o.tile_shape < 0 || o.tile_shape >= kNumTileShapes)
@@ -1789,10 +1789,14 @@ int yrtmi_plan_create(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
 			same = (unsigned long long)cur[b] == cnt[b];
 		if (!same)
 			return cleanup(fail(YRTMI_ESTATE, "plan: fill pass disagrees with count pass"));
-		// each chunk's records by their fine length class
-		e = launchChunkSort(dd, pl, q->chunks, q->nChunks, c->stream);
-		if (e == hipSuccess)
-			e = hipStreamSynchronize(c->stream);
+		// each chunk's records by their fine length class (a plan used
+		// more than once)
+		if (!o.single_use)
+		{
+			e = launchChunkSort(dd, pl, q->chunks, q->nChunks, c->stream);
+			if (e == hipSuccess)
+				e = hipStreamSynchronize(c->stream);
+		}
 		if (e != hipSuccess)
 			return cleanup(hipFail(e, "plan: chunk sort"));
 	}
@@ -2331,7 +2335,7 @@ int yrtmi_autotune(yrtmi_ctx* c, yrtmi_proj* p, const yrtmi_image* im,
 	best->back_ms = 1e30f;
 	for (int shape = 0; shape < kNumTileShapes; shape++)
 	{
-		yrtmi_plan_options o{0, shape, 0, 0};
+		yrtmi_plan_options o{0, shape, 0, 0, 0};
 		yrtmi_plan* q = nullptr;
 		best->back_ms_shape[shape][0] = best->back_ms_shape[shape][1] = -1.f;
 		r = yrtmi_plan_create(c, p, im, l, &o, &q);

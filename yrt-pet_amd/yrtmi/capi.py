@@ -76,7 +76,7 @@ _lib = None
 _P = C.c_void_p
 class PlanOptions(C.Structure):
     _fields_ = [("chunk_records", C.c_int), ("tile_shape", C.c_int), ("persistent", C.c_int),
-                ("max_bytes", C.c_int64)]
+                ("max_bytes", C.c_int64), ("single_use", C.c_int)]
 
 
 TILE_16x16x16, TILE_32x32x16, TILE_32x32x32, TILE_32x32x8 = 0, 1, 2, 3

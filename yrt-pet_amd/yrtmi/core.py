@@ -296,11 +296,13 @@ class Plan:
     its ray table.  Geometry only; holds device memory until dropped.  Keeps
     the LOR set alive (the plan refers to its arrays)."""
 
-    def __init__(self, ctx, proj, im, lors, chunk_records=0, tile_shape=0, persistent=0):
+    def __init__(self, ctx, proj, im, lors, chunk_records=0, tile_shape=0, persistent=0,
+                 single_use=False):
         self.ctx, self.proj, self.im, self.lors = ctx, proj, im, lors
         h = _P()
         ci, cl = im.c(), lors.c()
-        opt = capi.PlanOptions(int(chunk_records), int(tile_shape), int(persistent), 0)
+        opt = capi.PlanOptions(int(chunk_records), int(tile_shape), int(persistent), 0,
+                               int(bool(single_use)))
         capi.check(capi.lib().yrtmi_plan_create(ctx.h, proj.h, C.byref(ci), C.byref(cl),
                                                 C.byref(opt), C.byref(h)))
         self.h = h
@@ -427,8 +429,10 @@ class Context:
                     back_ms_shape=[list(r) for r in t.back_ms_shape])
 
     def plan(self, proj: Projector, im: ImageParams, lors: LorSet, chunk_records=0,
-             tile_shape=0, persistent=0):
-        return Plan(self, proj, im, lors, chunk_records, tile_shape, persistent)
+             tile_shape=0, persistent=0, single_use=False):
+        """single_use: a plan backprojected once (e.g. for a sensitivity
+        image), built without ordering its records by length."""
+        return Plan(self, proj, im, lors, chunk_records, tile_shape, persistent, single_use)
 
     def back(self, proj: Projector, im: ImageParams, img, lors: LorSet, vals, plan=None):
         _check_dtype(img, torch.float32, "img")
