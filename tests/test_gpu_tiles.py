@@ -140,6 +140,32 @@ def test_back_planned_frames_motion_offset(gpu):
     check("frames_planned_vs_oracle", b.cpu().numpy(), ref, TOL_ORACLE)
 
 
+def test_plan_record_order_keeps_the_image(gpu):
+    """A plan orders each chunk's records by length (kChunkSort) unless it
+    is built for one use.  The records are the same, so the 64-bit
+    fixed-point tile sums are too: with one chunk per tile (chunk_records
+    16384, 32x32x8 tiles) the two plans' images are bit-identical.  The
+    chunks hold several 2048-record sort windows, the last one partial."""
+    sc, im, pp, ctx, p = setup(gpu, "dyadic", 4, 250.0, 0, 4)
+    n = 20000
+    ev = events(sc, n, 131, 250.0)
+    vals = np.random.default_rng(13).uniform(0.1, 1.0, n).astype(np.float32)
+    L = dev_lors(ev, gpu)
+    vg = torch.from_numpy(vals).to(gpu)
+    ordered = ctx.plan(p, im, L, 16384, tile_shape=3)
+    single = ctx.plan(p, im, L, 16384, tile_shape=3, single_use=True)
+    info = ordered.info()
+    assert info == single.info()
+    assert info["records"] > 2048 * info["chunks"]  # several windows per chunk
+    a = torch.zeros(im.shape, device=gpu)
+    b = torch.zeros(im.shape, device=gpu)
+    ctx.back(p, im, a, L, vg, plan=ordered)
+    ctx.back(p, im, b, L, vg, plan=single)
+    ctx.synchronize()
+    assert np.count_nonzero(a.cpu().numpy()) > 1000
+    np.testing.assert_array_equal(a.cpu().numpy(), b.cpu().numpy())
+
+
 def test_back_planned_fixed_point_range(gpu):
     """Tile sums are accumulated in 64-bit fixed point scaled from the
     launch's values (terms out of range go to global atomics): values spread

@@ -94,6 +94,29 @@ CASES = [
 ]
 
 
+def test_dd_plan_record_order_keeps_the_image(gpu):
+    """DD records ordered by footprint voxels per slice (kChunkSort) or not
+    (a single-use plan): one chunk per tile, so the fixed-point tile sums
+    and the images are bit-identical."""
+    sc, im, pp, ctx, p = setup(gpu, "dyadic", None, psf=True)
+    n = 40000
+    ev = events(sc, n, 141, None)
+    vals = np.random.default_rng(14).uniform(0.1, 1.0, n).astype(np.float32)
+    L = dev_lors(ev, gpu)
+    vg = torch.from_numpy(vals).to(gpu)
+    ordered = ctx.plan(p, im, L, 1 << 20, tile_shape=0)
+    single = ctx.plan(p, im, L, 1 << 20, tile_shape=0, single_use=True)
+    info = ordered.info()
+    assert info == single.info() and info["records"] > 2048 * info["chunks"]
+    a = torch.zeros(im.shape, device=gpu)
+    b = torch.zeros(im.shape, device=gpu)
+    ctx.back(p, im, a, L, vg, plan=ordered)
+    ctx.back(p, im, b, L, vg, plan=single)
+    ctx.synchronize()
+    assert np.count_nonzero(a.cpu().numpy()) > 1000
+    np.testing.assert_array_equal(a.cpu().numpy(), b.cpu().numpy())
+
+
 @pytest.mark.parametrize("name,image,tof,s64,psf,shape", CASES, ids=[c[0] for c in CASES])
 def test_dd_back_planned(gpu, name, image, tof, s64, psf, shape):
     sc, im, pp, ctx, p = setup(gpu, image, tof, s64, psf)
