@@ -201,8 +201,9 @@ def subset_major(torch, ev, n, S, dev):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU: --gpus must be 1 "
-                                   "(multi-GPU sharding is described in DESIGN.md, not built)"}))
+        print(json.dumps({"error": "one GPU only: --gpus must be 1 (the events of a "
+                                   "subset would shard across GPUs, with the tile images "
+                                   "summed by an all-reduce; that path is not built)"}))
         return 2
     import torch
     dev = torch.device("cuda:0")

@@ -550,6 +550,7 @@ __device__ __forceinline__ void tileChunk(const KArgs& a, const DTiles& T,
 	const uint32_t R = (uint32_t)pl.R;
 	const bool rPow2 = (R & (R - 1u)) == 0u;
 	const int rShift = __ffs((int)R) - 1;
+	const float rInvPow2 = ldexpf(1.f, -rShift);
 	const uint64_t nRays = (uint64_t)a.L.n * R;
 	// no record (keys are < n * R < 2^32); a key is checked against n * R
 	// where it is used, not where it is read, so that its load stays in
@@ -604,8 +605,10 @@ __device__ __forceinline__ void tileChunk(const KArgs& a, const DTiles& T,
 			if (__float_as_int(qb.w) != f)
 				return;
 			const float3 q1 = make_float3(qa.x, qa.y, qa.z), q2 = make_float3(qa.w, qb.x, qb.y);
+			// v / R: for R a power of two the product with 1/R is the same
+			// correctly rounded value (1/R exact), one multiply
 			if (R > 1u)
-				val = val / (float)R;
+				val = rPow2 ? val * rInvPow2 : val / (float)R;
 			const uint32_t pv = rc.pv;
 			const int vx = B.x0 + (int)(pv & 31u), vy = B.y0 + (int)((pv >> 5) & 31u),
 			          vz = B.z0 + (int)((pv >> 10) & 31u);
