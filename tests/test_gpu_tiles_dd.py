@@ -91,6 +91,9 @@ CASES = [
     ("dd_psf_tof", "nondyadic", 300.0, False, True, 0),
     ("dd_nondyadic_shape1", "nondyadic", None, False, False, 1),
     ("dd_tof_shape2_fp32_tile", "dyadic", 250.0, False, False, 2),
+    # PSF tables staged in LDS beside the 128 KB tiles
+    ("dd_psf_shape1", "nondyadic", None, False, True, 1),
+    ("dd_psf_tof_shape2_fp32_tile", "dyadic", 250.0, False, True, 2),
 ]
 
 
