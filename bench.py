@@ -1268,7 +1268,7 @@ def bench_autograd(args, torch, dev, stream):
     rhs = float((u.double() * op.adjoint(w).double()).sum())
     adj = abs(lhs - rhs) / abs(lhs)
     log(args, f"{wall * 1e3:.1f} ms/step (forward {fwd_ms:.1f} ms, backward {back_ms:.1f} ms); "
-              f"adjoint rel diff {adj:.2e}; loss {float(loss):.6e}")
+              f"adjoint rel diff {adj:.2e}; loss {float(loss.detach()):.6e}")
     # algorithmic bytes: forward 12 B/event + 4 B per segment + 4 B stored;
     # backward 12 + 4 B/event + 8 B per segment
     kern = [("forward A x (kEvents<MODE_FWD>)", fwd_ms, 16.0 * n + 4.0 * segs),
