@@ -150,10 +150,12 @@ hipError_t launchRayPrep(const KArgs& a, float4* tab, hipStream_t st);
 // pass 0: count; pass 1: fill (same grid in both).
 hipError_t launchBin(int pass, const KArgs& a, const DTiles& T, const DPlan& pl,
                      int grid, hipStream_t st);
-// y += backprojection of v (per LOR) through the plan's records: one
-// workgroup per chunk, or `persistent` workgroups on the chunk queue.
+// Each chunk's records (nChunks chunks, device table) ordered by their fine
+// length class, in place (kChunkSort).
 hipError_t launchChunkSort(bool dd, const DPlan& pl, const TileChunk* chunks, int64_t nChunks,
                            hipStream_t st);
+// y += backprojection of v (per LOR) through the plan's records: one
+// workgroup per chunk, or `persistent` workgroups on the chunk queue.
 hipError_t launchTileBack(const KArgs& a, const DTiles& T, const DPlanRead& pl,
                           const float* v, float* y, int persistent, hipStream_t st);
 // xb (nt frames of brickFrameVoxels floats) = bricked copy of x (nt frames
