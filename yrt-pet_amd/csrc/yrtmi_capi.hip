@@ -1441,10 +1441,8 @@ struct yrtmi_plan
 	int useTof = 0;
 	DTiles T{};
 	int64_t records = 0, nChunks = 0, bytes = 0;
-	uint32_t* key = nullptr;
-	float* alpha = nullptr;
-	uint32_t* vox = nullptr;
-	float4* ray = nullptr;  // ray table, 2 x float4 per (LOR, ray)
+	uint32_t* rec = nullptr;  // records, recWords words each (DPlan::rec)
+	float4* ray = nullptr;    // ray table (rayTabVec / kDDVec float4 per entry)
 	TileChunk* chunks = nullptr;
 	float* v = nullptr;  // per-LOR values of the em / sensitivity calls
 	unsigned int* queue = nullptr;  // chunk queue of persistent launches
@@ -1456,9 +1454,7 @@ namespace
 
 void freePlan(yrtmi_plan* q)
 {
-	hipFree(q->key);
-	hipFree(q->alpha);
-	hipFree(q->vox);
+	hipFree(q->rec);
 	hipFree(q->ray);
 	hipFree(q->chunks);
 	hipFree(q->v);
@@ -1512,9 +1508,7 @@ DPlanRead planRead(const yrtmi_plan* q)
 	r.R = q->proj->d.numRays;
 	r.chunks = q->chunks;
 	r.nChunks = q->nChunks;
-	r.key = q->key;
-	r.alpha = q->alpha;
-	r.vox = q->vox;
+	r.rec = q->rec;
 	r.ray = q->ray;
 	r.queue = q->queue;
 	r.vexp = reinterpret_cast<const int*>(q->queue + 2);
@@ -1754,10 +1748,8 @@ int yrtmi_plan_create(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
 	q->nChunks = (int64_t)chunks.size();
 	if (total > 0)
 	{
-		// DD records have no alpha
-		if (hipMalloc(&q->key, sizeof(uint32_t) * total) != hipSuccess ||
-		    (!dd && hipMalloc(&q->alpha, sizeof(float) * total) != hipSuccess) ||
-		    hipMalloc(&q->vox, sizeof(uint32_t) * total) != hipSuccess ||
+		// records of 12 B (Siddon) / 8 B (DD: no alpha)
+		if (hipMalloc(&q->rec, sizeof(uint32_t) * recWords(dd) * total) != hipSuccess ||
 		    hipMalloc(&q->chunks, sizeof(TileChunk) * chunks.size()) != hipSuccess)
 			return cleanup(fail(YRTMI_ENOMEM, "plan: records"));
 		q->bytes += total * (dd ? 8 : 12) + (int64_t)(sizeof(TileChunk) * chunks.size());
@@ -1766,9 +1758,7 @@ int yrtmi_plan_create(yrtmi_ctx* c, const yrtmi_proj* p, const yrtmi_image* im,
 		if (e == hipSuccess)
 			e = hipMemcpyAsync(q->chunks, chunks.data(), sizeof(TileChunk) * chunks.size(),
 			                   hipMemcpyHostToDevice, c->stream);
-		pl.key = q->key;
-		pl.alpha = q->alpha;
-		pl.vox = q->vox;
+		pl.rec = q->rec;
 		if (e == hipSuccess)
 			e = launchBin(1, a, T, pl, grid, c->stream);
 		std::vector<unsigned int> cur(nB);

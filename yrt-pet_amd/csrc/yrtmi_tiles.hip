@@ -294,10 +294,12 @@ __global__ void __launch_bounds__(256) kBin(DImage im, DProj pr, int64_t nRays, 
 			if (left > 0)
 			{
 				const int64_t j = (int64_t)base[b] + left - 1;
-				pl.key[j] = (uint32_t)t;
+				// one record of consecutive words (one vector store)
+				uint32_t* o = pl.rec + recWords(DD) * j;
+				o[0] = (uint32_t)t;
 				if (!DD)
-					pl.alpha[j] = a0;
-				pl.vox[j] = withFineClass(DD, pv, segs, T.tx + T.ty + T.tz);
+					o[1] = __float_as_uint(a0);
+				o[recWords(DD) - 1] = withFineClass(DD, pv, segs, T.tx + T.ty + T.tz);
 			}
 			else
 				atomicOr(pl.err, 1);
@@ -330,7 +332,7 @@ __global__ void __launch_bounds__(256) kChunkSort(DPlan pl, const TileChunk* __r
 		const int n = min(kSortWin, ch.count - w0);
 		const int64_t j0 = ch.begin + w0;
 		uint32_t k[GPW], v[GPW];
-		float al[GPW];
+		uint32_t al[GPW];
 		int cls[GPW];
 #pragma unroll
 		for (int i = 0; i < GPW; i++)
@@ -338,14 +340,15 @@ __global__ void __launch_bounds__(256) kChunkSort(DPlan pl, const TileChunk* __r
 			const int g = wv + 4 * i, r = g * 64 + lane;
 			k[i] = 0u;
 			v[i] = 0u;
-			al[i] = 0.f;
+			al[i] = 0u;
 			cls[i] = 16;  // no record
 			if (r < n)
 			{
-				k[i] = pl.key[j0 + r];
-				v[i] = pl.vox[j0 + r];
+				const uint32_t* o = pl.rec + recWords(DD) * (j0 + r);
+				k[i] = o[0];
+				v[i] = o[recWords(DD) - 1];
 				if (!DD)
-					al[i] = pl.alpha[j0 + r];
+					al[i] = o[1];
 				cls[i] = fineClassOf(DD, v[i]);
 			}
 			for (int c = 0; c < 16; c++)
@@ -393,10 +396,11 @@ __global__ void __launch_bounds__(256) kChunkSort(DPlan pl, const TileChunk* __r
 			if (cls[i] < 16)
 			{
 				const int64_t j = j0 + base[cls[i]] + offG[g][cls[i]] + __popcll(mine & below);
-				pl.key[j] = k[i];
-				pl.vox[j] = v[i];
+				uint32_t* o = pl.rec + recWords(DD) * j;
+				o[0] = k[i];
 				if (!DD)
-					pl.alpha[j] = al[i];
+					o[1] = al[i];
+				o[recWords(DD) - 1] = v[i];
 			}
 		}
 		__syncthreads();
@@ -577,10 +581,11 @@ __device__ __forceinline__ void tileChunk(const KArgs& a, const DTiles& T,
 		if (k < ch.count)
 		{
 			const int64_t j = ch.begin + k;
-			r.key = pl.key[j];
+			const uint32_t* o = pl.rec + recWords(DD) * j;
+			r.key = o[0];
 			if (!DD)
-				r.al = pl.alpha[j];
-			r.pv = pl.vox[j];
+				r.al = __uint_as_float(o[1]);
+			r.pv = o[recWords(DD) - 1];
 		}
 	};
 	auto readRay = [&](uint32_t key, Ray& q) {

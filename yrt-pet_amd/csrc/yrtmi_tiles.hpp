@@ -58,6 +58,9 @@ __host__ __device__ inline uint32_t withFineClass(bool dd, uint32_t pv, int segs
 	return pv | ((uint32_t)c << 27);
 }
 
+// Words per plan record: Siddon key, alpha, entry; DD key, entry.
+__host__ __device__ constexpr int recWords(bool dd) { return dd ? 2 : 3; }
+
 __host__ __device__ inline int fineClassOf(bool dd, uint32_t pv)
 {
 	return (int)((pv >> (dd ? 12 : 27)) & 15u);
@@ -94,10 +97,11 @@ struct DPlan
 	unsigned int* blockCnt;     // [grid][nTiles * C] records per workgroup and bin
 	unsigned int* cursor;       // [nTiles * C] fill cursor within the bin
 	const int64_t* tileOff;     // [nTiles * C] first record of the bin
-	uint32_t* key;              // lor * R + ray (DD: lor)
-	float* alpha;               // alpha where the walk enters the tile (Siddon)
-	uint32_t* vox;              // packed entry voxel (packEntry); DD: the
-	                            // record's slices in the slab (binRayDD)
+	// records of recWords(DD) words: the key (lor * R + ray; DD: lor), the
+	// alpha where the walk enters the tile (Siddon, its bits), the entry
+	// word (packEntry; DD: the record's slices in the slab, binRayDD) with
+	// its fine length class
+	uint32_t* rec;
 	const float4* ray;          // ray table (rayEntry + sidStore / ddStore, kRayPrep)
 	int* err;                   // set when a tile range would overflow
 };
@@ -129,10 +133,8 @@ struct DPlanRead
 	int32_t R;
 	const TileChunk* chunks;
 	int64_t nChunks;
-	const uint32_t* key;
-	const float* alpha;
-	const uint32_t* vox;
-	const float4* ray;  // rayEntry + sidStore (Siddon) or ddStore (DD) table
+	const uint32_t* rec;  // records, recWords(DD) words each (DPlan::rec)
+	const float4* ray;    // rayEntry + sidStore (Siddon) or ddStore (DD) table
 	unsigned int* queue;
 	// binade E of the median nonzero |v| of the launch (kVExpHist + kPickExp
 	// before each launch): sets the fixed-point scale of the tile
