@@ -97,6 +97,47 @@ CASES = [
 ]
 
 
+def test_dd_psf_tables_beyond_lds(gpu):
+    """A projection PSF of 48 rows of 57 taps (~34 KB per direction) is
+    larger than the DD kernels stage in LDS (16 KB): the planned
+    backprojection and EM ratio read it from global memory, and match the
+    direct kernels and the oracle as the staged tables do."""
+    sc = scanner()
+    im = IMAGES["dyadic"]()
+    pp = yrtmi.ProjectorParams(sc, projectorType="DD")
+    k, s_step, ks = synth.gaussian_psf_row(2.0, k_spacing=0.125, num_s=48, s_step=5.0)
+    assert k.shape == (48, 57)
+    k = k * np.linspace(0.8, 1.2, 48, dtype=np.float32)[:, None]  # rows differ
+    k[:, :20] *= 1.3  # asymmetric rows: the backprojection takes the mirrored ones
+    pp.psf_kernels, pp.psf_s_step, pp.psf_k_spacing = k, s_step, ks
+    ctx = yrtmi.Context(sc)
+    p = ctx.projector(pp)
+    n = 3000
+    ev = events(sc, n, 151, None)
+    vals = np.random.default_rng(15).uniform(0.1, 1.0, n).astype(np.float32)
+    L = dev_lors(ev, gpu)
+    vg = torch.from_numpy(vals).to(gpu)
+    plan = ctx.plan(p, im, L, tile_shape=0)
+    a = torch.zeros(im.shape, device=gpu)
+    b = torch.zeros(im.shape, device=gpu)
+    ctx.back(p, im, a, L, vg)
+    ctx.back(p, im, b, L, vg, plan=plan)
+    x = np.random.default_rng(16).uniform(0.5, 2.0, im.shape).astype(np.float32)
+    xg = torch.from_numpy(x).to(gpu)
+    mg = torch.from_numpy(np.random.default_rng(17).poisson(3.0, n).astype(np.float32)).to(gpu)
+    v = torch.zeros(n, device=gpu)
+    vp = torch.zeros(n, device=gpu)
+    ctx.em_ratio(p, im, xg, L, v, meas=mg)
+    ctx.em_ratio(p, im, xg, L, vp, meas=mg, plan=plan)
+    ctx.synchronize()
+    ref = O.back(oracle_image(im), oracle_proj(pp), oracle_lors(sc, ev), vals)
+    assert np.count_nonzero(ref) > 1000
+    check("dd_psf_global_planned_vs_direct", b.cpu().numpy(), a.cpu().numpy(), TOL_DIRECT)
+    check("dd_psf_global_planned_vs_oracle", b.cpu().numpy(), ref, TOL_ORACLE)
+    # the stored set-up gives the resolved LOR's set-up exactly
+    np.testing.assert_array_equal(vp.cpu().numpy(), v.cpu().numpy())
+
+
 def test_dd_plan_record_order_keeps_the_image(gpu):
     """DD records ordered by footprint voxels per slice (kChunkSort) or not
     (a single-use plan): one chunk per tile, so the fixed-point tile sums
