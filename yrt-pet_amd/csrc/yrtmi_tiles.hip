@@ -23,6 +23,7 @@
 #include "yrtmi_tiles.hpp"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace yrtmi
 {
@@ -519,7 +520,7 @@ __device__ __forceinline__ void tileChunk(const KArgs& a, const DTiles& T,
 		acc64[i] = 0ull;
 	// fixed-point scale of this chunk (uniform)
 	int fixed = 0, k = 0;
-	float climit = 0.f;
+	float climit = 0.f, wbound = 0.f;
 	if constexpr (SHAPE != 2)
 	{
 		if (pl.vexp != nullptr)
@@ -544,6 +545,7 @@ __device__ __forceinline__ void tileChunk(const KArgs& a, const DTiles& T,
 			// |c| < 2^(lim - k) <=> |c * 2^k| < 2^lim (inf when above the
 			// float range: then every finite c qualifies)
 			climit = ldexpf(1.f, lim - k);
+			wbound = wmax;
 			fixed = 1;
 		}
 	}
@@ -601,7 +603,11 @@ __device__ __forceinline__ void tileChunk(const KArgs& a, const DTiles& T,
 			q.val = v[DD ? key : lorOf(key)];
 		}
 	};
-	auto walk = [&](const Rec& rc, const Ray& q, auto&& add) {
+	// Siddon: a record whose value times the largest term factor (wmax:
+	// every term c = v * w of its walk is at most |v| * wmax) is within
+	// the fixed-point range walks with `addIn`, which skips the per-term
+	// range test; other records (and the FP32 tile) walk with `add`
+	auto walk = [&](const Rec& rc, const Ray& q, auto&& add, auto&& addIn, auto inRange) {
 		float val = q.val;
 		if ((uint64_t)rc.key >= nRays || val == 0.f)
 			return;
@@ -634,9 +640,20 @@ __device__ __forceinline__ void tileChunk(const KArgs& a, const DTiles& T,
 			if (!entry)
 				sidRestart<S64>(a.im, s, rc.al, vx, vy, vz, dx, dy, dz);
 			const float vs = val * s.ws;
-			sidWalk<TOF, S64>(a.im, s, a.pr.tof, B, entry, vx, vy, vz,
-			                  [&](int gx, int gy, int gz, uint32_t idx, float, float w,
-			                      uint32_t) { add(idx, gx, gy, gz, vs * w); });
+			auto go = [&](auto&& fn) {
+				sidWalk<TOF, S64>(a.im, s, a.pr.tof, B, entry, vx, vy, vz,
+				                  [&](int gx, int gy, int gz, uint32_t idx, float, float w,
+				                      uint32_t) { fn(idx, gx, gy, gz, vs * w); });
+			};
+			if constexpr (decltype(inRange)::value)
+			{
+				if (fabsf(val) * wbound < climit)
+					go(addIn);
+				else
+					go(add);
+			}
+			else
+				go(add);
 		}
 	};
 	// DD: a wave's 64 records are swept one (record, slice) pair per lane.
@@ -713,7 +730,7 @@ __device__ __forceinline__ void tileChunk(const KArgs& a, const DTiles& T,
 			}
 		}
 	};
-	auto pipeline = [&](auto&& add) {
+	auto pipeline = [&](auto&& add, auto&& addIn, auto inRange) {
 		if constexpr (DD)
 			pipelineDD(add);
 		else
@@ -730,7 +747,7 @@ __device__ __forceinline__ void tileChunk(const KArgs& a, const DTiles& T,
 				readRay(r1.key, q1);
 				Rec r2;
 				readRecord(i + 2 * NT, r2);
-				walk(r0, q0, add);
+				walk(r0, q0, add, addIn, inRange);
 				r0 = r1;
 				q0 = q1;
 				r1 = r2;
@@ -748,24 +765,28 @@ __device__ __forceinline__ void tileChunk(const KArgs& a, const DTiles& T,
 	if (fixed)
 	{
 		wordBytes(8);
-		pipeline([&](uint32_t idx, int gx, int gy, int gz, float c) {
-			if (fabsf(c) < climit)
-			{
-				const double d = fma((double)c, scale, kBias);
-				ldsAdd64(reinterpret_cast<unsigned long long*>(
-				             reinterpret_cast<char*>(acc64) + idx),
-				         (long long)((unsigned long long)__double_as_longlong(d) - kBiasBits));
-			}
-			else  // out of the fixed-point range, or not finite
-				unsafeAtomicAdd(yf + (int64_t)gz * a.im.nxy + (int64_t)gy * a.im.nx + gx, c);
-		});
+		auto addFixed = [&](uint32_t idx, float c) {
+			const double d = fma((double)c, scale, kBias);
+			ldsAdd64(reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(acc64) + idx),
+			         (long long)((unsigned long long)__double_as_longlong(d) - kBiasBits));
+		};
+		pipeline(
+		    [&](uint32_t idx, int gx, int gy, int gz, float c) {
+			    if (fabsf(c) < climit)
+				    addFixed(idx, c);
+			    else  // out of the fixed-point range, or not finite
+				    unsafeAtomicAdd(yf + (int64_t)gz * a.im.nxy + (int64_t)gy * a.im.nx + gx, c);
+		    },
+		    [&](uint32_t idx, int, int, int, float c) { addFixed(idx, c); },
+		    std::integral_constant<bool, !DD>{});
 	}
 	else
 	{
 		wordBytes(4);
-		pipeline([&](uint32_t idx, int, int, int, float c) {
+		auto addFloat = [&](uint32_t idx, int, int, int, float c) {
 			ldsAdd(reinterpret_cast<float*>(reinterpret_cast<char*>(acc) + idx), c);
-		});
+		};
+		pipeline(addFloat, addFloat, std::integral_constant<bool, false>{});
 	}
 	__syncthreads();
 	for (int i = threadIdx.x; i < NV; i += NT)
